@@ -1,0 +1,551 @@
+// nb_api.hip -- C ABI (include/gregnbody.h) over the gfx950 kernels.
+//
+// Mirrors gtd::system's public surface (gregsys.hpp:1478-1760, 2593-2607):
+// construction from bodies, setters with the reference's validation rules,
+// evolve(method) with a fresh a(x0) per call, state read-back.  Every entry
+// point converts library exceptions into status codes.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "nb_system.hpp"
+
+using namespace nbe;
+
+nb_system::~nb_system() {
+    if (bh) bh_work_free(bh);
+    if (coll) coll_work_free(coll);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+thread_local std::string g_static_err;
+
+int fail(nb_system *s, int code, const std::string &msg) {
+    if (s) s->err = msg;
+    else g_static_err = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(nb_system *s, F &&f) {
+    try {
+        if (s) NB_HIP(hipSetDevice(s->device));
+        return f();
+    } catch (const ApiError &e) {
+        return fail(s, e.code, e.what);
+    } catch (const DeviceError &e) {
+        return fail(s, NB_ERR_DEVICE, e.what);
+    } catch (const std::bad_alloc &) {
+        return fail(s, NB_ERR_CAPACITY, "host allocation failed");
+    }
+}
+
+// parse_units_format (gregsys.hpp:245-282): "Ma:b,Dc:x,Ty:z" with positive
+// integers of 1-18 digits containing a non-zero digit; G = 66743 *
+// (m_num d_num^3 t_num^2) / (m_den d_den^3 t_den^2) / 1e15, where the integer
+// products are unsigned 64-bit and the arithmetic is long double.
+bool parse_units(const char *str, long double *G, uint64_t vals[6]) {
+    if (!str) return false;
+    const char *p = str;
+    auto skip = [&]() { while (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r' || *p == '\f' || *p == '\v') ++p; };
+    auto num = [&](uint64_t *out) -> bool {
+        skip();
+        const char *b = p;
+        uint64_t v = 0;
+        bool nz = false;
+        while (*p >= '0' && *p <= '9') {
+            v = v * 10 + (uint64_t)(*p - '0');
+            if (*p != '0') nz = true;
+            ++p;
+        }
+        size_t len = (size_t)(p - b);
+        if (len < 1 || len > 18 || !nz) return false;
+        *out = v;
+        skip();
+        return true;
+    };
+    auto lit = [&](char a, char b2) -> bool {
+        skip();
+        if (*p != a && *p != b2) return false;
+        ++p;
+        return true;
+    };
+    uint64_t a, b, c, x, y, z;
+    if (!lit('m', 'M') || !num(&a) || !lit(':', ':') || !num(&b) || !lit(',', ',')) return false;
+    if (!lit('d', 'D') || !num(&c) || !lit(':', ':') || !num(&x) || !lit(',', ',')) return false;
+    if (!lit('t', 'T') || !num(&y) || !lit(':', ':') || !num(&z)) return false;
+    skip();
+    if (*p != '\0') return false;
+    uint64_t m_den = a, m_num = b, d_num = c, d_den = x, t_den = y, t_num = z;
+    long double g = 66743;
+    g *= m_num * d_num * d_num * d_num * t_num * t_num;
+    g /= m_den * d_den * d_den * d_den * t_den * t_den;
+    g /= 1000000000000000ULL;
+    *G = g;
+    vals[0] = m_num; vals[1] = m_den; vals[2] = d_num; vals[3] = d_den; vals[4] = t_num; vals[5] = t_den;
+    return true;
+}
+
+// check_option (gregsys.hpp:1406-1411)
+bool check_option(int option) {
+    uint16_t lo = (uint16_t)(option & 0x0000ffff);
+    uint16_t hi = (uint16_t)((unsigned)option >> 16);
+    return !(lo & (lo - 1)) || !lo || lo > NB_RK4 || !(hi & (hi - 1)) || hi > 2;
+}
+
+void to_soa(int64_t n, const double *aos, std::vector<double> &soa) {
+    soa.resize((size_t)(3 * n));
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) soa[(size_t)(c * n + i)] = aos[3 * i + c];
+}
+void from_soa(int64_t n, const std::vector<double> &soa, double *aos) {
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) aos[3 * i + c] = soa[(size_t)(c * n + i)];
+}
+
+}  // namespace
+
+namespace nbe {
+
+int form_of(const nb_system *s) { return s->eps > 0 ? 1 : 0; }
+
+void direct_forces(nb_system *s) {
+    const int64_t n = s->n;
+    if (s->precision == NB_PREC_MIXED) {
+        s->dw.pos32.ensure((size_t)n);
+        s->dw.nsplit = choose_nsplit(n);
+        s->dw.partial.ensure((size_t)s->dw.nsplit * 6 * (size_t)n);
+        launch_pack(s->stream, n, s->x.p, s->m.p, s->G, s->dw.pos32.p);
+        {
+            ScopedTimer t(s->prof, "force", s->stream);
+            launch_force_mixed(s->stream, n, s->dw.pos32.p, (float)(s->eps * s->eps), form_of(s) == 1,
+                               s->dw.partial.p, s->dw.nsplit);
+        }
+        ScopedTimer t(s->prof, "update", s->stream);
+        launch_update_mixed(s->stream, n, s->dw.partial.p, s->dw.nsplit, s->a.p, s->v.p, s->x.p, s->m.p, s->G,
+                            s->half_dt, s->dt, UPD_STORE, s->dw.pos32.p);
+    } else {
+        ScopedTimer t(s->prof, "force", s->stream);
+        launch_force_fp64(s->stream, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->a.p);
+    }
+    s->acc_valid = true;
+}
+
+// KDK steps from valid accelerations (evolve's KDK branch :2949-2968 with
+// leapfrog_kdk_acc_e_and_step :514-526 and s_coll when collisions are on).
+void direct_kdk_steps(nb_system *s, uint64_t steps) {
+    const bool mixed = s->precision == NB_PREC_MIXED;
+    const bool coll = s->coll_mode == NB_OVERLAP_COLL_CHECK;
+    bool need_open = true;
+    for (uint64_t st = 1; st <= steps; ++st) {
+        const int64_t n = s->n;
+        if (n == 0) break;
+        if (need_open) {
+            launch_kick_drift(s->stream, n, s->a.p, s->v.p, s->x.p, s->half_dt, s->dt);
+            if (mixed) launch_pack(s->stream, n, s->x.p, s->m.p, s->G, s->dw.pos32.p);
+        }
+        if (mixed) {
+            if (s->dw.pos32.n < (size_t)n) direct_forces(s);  // defensive: sizes follow n
+            {
+                ScopedTimer t(s->prof, "force", s->stream);
+                launch_force_mixed(s->stream, n, s->dw.pos32.p, (float)(s->eps * s->eps), form_of(s) == 1,
+                                   s->dw.partial.p, s->dw.nsplit);
+            }
+            const bool last = st == steps || coll;
+            ScopedTimer t(s->prof, "update", s->stream);
+            launch_update_mixed(s->stream, n, s->dw.partial.p, s->dw.nsplit, s->a.p, s->v.p, s->x.p, s->m.p, s->G,
+                                s->half_dt, s->dt, last ? UPD_CLOSE : UPD_CLOSE_OPEN, s->dw.pos32.p);
+            need_open = last;
+        } else {
+            {
+                ScopedTimer t(s->prof, "force", s->stream);
+                launch_force_fp64(s->stream, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->a.p);
+            }
+            launch_kick(s->stream, n, s->a.p, s->v.p, s->half_dt);
+            need_open = true;
+        }
+        ++s->step_count;
+        if (coll) {
+            collision_pass(s, s->step_count);
+            if (mixed && s->n != n && s->n > 0) {
+                // body count changed: resize split partials for the new n
+                s->dw.nsplit = choose_nsplit(s->n);
+                s->dw.partial.ensure((size_t)s->dw.nsplit * 6 * (size_t)s->n);
+            }
+        }
+    }
+}
+
+}  // namespace nbe
+
+extern "C" {
+
+int nb_create(int device, nb_system **out) {
+    if (!out) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "nb_create: out is NULL");
+    *out = nullptr;
+    return guarded(nullptr, [&]() {
+        int count = 0;
+        NB_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count)
+            throw ApiError(NB_ERR_INVALID_ARGUMENT, "nb_create: device index out of range");
+        NB_HIP(hipSetDevice(device));
+        nb_system *s = new nb_system();
+        s->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete s;
+            throw DeviceError(std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+        }
+        long double G;
+        parse_units("M1:1,D1:1,T1:1", &G, s->G_vals);
+        s->G = (double)G;
+        *out = s;
+        return NB_OK;
+    });
+}
+
+int nb_destroy(nb_system *s) {
+    if (!s) return NB_OK;
+    (void)hipSetDevice(s->device);
+    (void)hipStreamSynchronize(s->stream);
+    delete s;
+    return NB_OK;
+}
+
+const char *nb_last_error(const nb_system *s) { return s ? s->err.c_str() : g_static_err.c_str(); }
+
+int nb_set_bodies(nb_system *s, int64_t n, const double *m, const double *r, const double *pos, const double *vel,
+                  const double *rest_c, int check_overlap) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "nb_set_bodies: NULL system");
+    return guarded(s, [&]() {
+        if (n < 0 || (n > 0 && (!m || !pos || !vel)))
+            throw ApiError(NB_ERR_INVALID_ARGUMENT, "nb_set_bodies: bad arguments");
+        if (n > (int64_t)std::numeric_limits<int>::max() / 2)
+            throw ApiError(NB_ERR_CAPACITY, "nb_set_bodies: too many bodies for one context");
+        for (int64_t i = 0; i < n; ++i) {
+            if (m[i] < 0) throw ApiError(NB_ERR_NEGATIVE_MASS, "body " + std::to_string(i) + " has negative mass");
+            if (r && r[i] < 0)
+                throw ApiError(NB_ERR_NEGATIVE_RADIUS, "body " + std::to_string(i) + " has negative radius");
+        }
+        if (check_overlap && r) {
+            // system::check_overlap (gregsys.hpp:307-342) without merging
+            for (int64_t i = 0; i < n; ++i)
+                for (int64_t j = i + 1; j < n; ++j) {
+                    double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+                           dz = pos[3 * j + 2] - pos[3 * i + 2];
+                    if (std::sqrt(dx * dx + dy * dy + dz * dz) < r[i] + r[j])
+                        throw ApiError(NB_ERR_OVERLAPPING_BODIES, "The bodies with id=" + std::to_string(i) +
+                                                                      " and id=" + std::to_string(j) +
+                                                                      " that were added to this system object overlap.");
+                }
+        }
+        NB_HIP(hipStreamSynchronize(s->stream));
+        s->n = n;
+        size_t nn = (size_t)std::max<int64_t>(n, 1);
+        s->m.ensure(nn); s->r.ensure(nn); s->e.ensure(nn); s->pe.ensure(nn);
+        s->x.ensure(3 * nn); s->v.ensure(3 * nn); s->a.ensure(3 * nn);
+        if (n > 0) {
+            std::vector<double> tmp;
+            NB_HIP(hipMemcpy(s->m.p, m, (size_t)n * 8, hipMemcpyHostToDevice));
+            tmp.assign((size_t)n, 1.0);
+            NB_HIP(hipMemcpy(s->r.p, r ? r : tmp.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+            NB_HIP(hipMemcpy(s->e.p, rest_c ? rest_c : tmp.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+            to_soa(n, pos, tmp);
+            NB_HIP(hipMemcpy(s->x.p, tmp.data(), (size_t)(3 * n) * 8, hipMemcpyHostToDevice));
+            to_soa(n, vel, tmp);
+            NB_HIP(hipMemcpy(s->v.p, tmp.data(), (size_t)(3 * n) * 8, hipMemcpyHostToDevice));
+            NB_HIP(hipMemset(s->a.p, 0, (size_t)(3 * n) * 8));
+        }
+        s->labels.resize((size_t)n);
+        for (int64_t i = 0; i < n; ++i) s->labels[(size_t)i] = i;
+        s->next_label = n;
+        s->evolved = false;
+        s->acc_valid = false;
+        s->step_count = 0;
+        s->merge_log.clear();
+        return NB_OK;
+    });
+}
+
+int nb_get_state(const nb_system *cs, int64_t cap, double *m, double *r, double *pos, double *vel, double *acc,
+                 double *rest_c) {
+    nb_system *s = const_cast<nb_system *>(cs);
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "nb_get_state: NULL system");
+    return guarded(s, [&]() {
+        const int64_t n = s->n;
+        if (cap < n) throw ApiError(NB_ERR_CAPACITY, "nb_get_state: capacity smaller than the body count");
+        NB_HIP(hipStreamSynchronize(s->stream));
+        if (n == 0) return NB_OK;
+        if (m) NB_HIP(hipMemcpy(m, s->m.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+        if (r) NB_HIP(hipMemcpy(r, s->r.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+        if (rest_c) NB_HIP(hipMemcpy(rest_c, s->e.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+        std::vector<double> tmp((size_t)(3 * n));
+        const double *srcs[3] = {s->x.p, s->v.p, s->a.p};
+        double *dsts[3] = {pos, vel, acc};
+        for (int q = 0; q < 3; ++q) {
+            if (!dsts[q]) continue;
+            NB_HIP(hipMemcpy(tmp.data(), srcs[q], (size_t)(3 * n) * 8, hipMemcpyDeviceToHost));
+            from_soa(n, tmp, dsts[q]);
+        }
+        return NB_OK;
+    });
+}
+
+int nb_num_bodies(const nb_system *s, int64_t *n) {
+    if (!s || !n) return fail(const_cast<nb_system *>(s), NB_ERR_INVALID_ARGUMENT, "nb_num_bodies: NULL argument");
+    *n = s->n;
+    return NB_OK;
+}
+
+int nb_body_ids(const nb_system *cs, int64_t cap, int64_t *ids) {
+    nb_system *s = const_cast<nb_system *>(cs);
+    if (!s || !ids) return fail(s, NB_ERR_INVALID_ARGUMENT, "nb_body_ids: NULL argument");
+    if (cap < s->n) return fail(s, NB_ERR_CAPACITY, "nb_body_ids: capacity smaller than the body count");
+    std::copy(s->labels.begin(), s->labels.end(), ids);
+    return NB_OK;
+}
+
+int nb_set_units(nb_system *s, const char *units) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    long double G;
+    uint64_t vals[6];
+    if (!parse_units(units, &G, vals))
+        return fail(s, NB_ERR_INVALID_ARGUMENT,
+                    std::string("The units_format string passed, \"") + (units ? units : "(null)") +
+                        "\", does not match the format required:\n\"Ma:b,Dc:x,Ty:z\"");
+    s->G = (double)G;
+    std::copy(vals, vals + 6, s->G_vals);
+    s->acc_valid = false;
+    return NB_OK;
+}
+
+int nb_set_G(nb_system *s, double G) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    if (!(G > 0) || !std::isfinite(G)) return fail(s, NB_ERR_INVALID_ARGUMENT, "G must be positive and finite");
+    s->G = G;
+    s->acc_valid = false;
+    return NB_OK;
+}
+
+int nb_get_G(const nb_system *s, double *G) {
+    if (!s || !G) return fail(const_cast<nb_system *>(s), NB_ERR_INVALID_ARGUMENT, "NULL argument");
+    *G = s->G;
+    return NB_OK;
+}
+
+int nb_set_timestep(nb_system *s, double dt) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    if (!(dt > 0)) return fail(s, NB_ERR_INVALID_ARGUMENT, "Time-step cannot be zero or negative.");
+    s->dt = dt;
+    s->half_dt = dt / 2;
+    return NB_OK;
+}
+
+int nb_set_iterations(nb_system *s, uint64_t it) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    if (!it) return fail(s, NB_ERR_INVALID_ARGUMENT, "cannot have zero iterations");
+    s->iterations = it;
+    return NB_OK;
+}
+
+int nb_set_softening(nb_system *s, double eps) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    if (eps != 0.0 && !(eps >= std::sqrt(std::numeric_limits<double>::min())))
+        return fail(s, NB_ERR_INVALID_ARGUMENT, "softening below sqrt(DBL_MIN)");
+    if (!std::isfinite(eps)) return fail(s, NB_ERR_INVALID_ARGUMENT, "softening must be finite");
+    s->eps = eps;
+    s->acc_valid = false;
+    return NB_OK;
+}
+
+int nb_set_bh_opening_angle(nb_system *s, double theta) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    if (theta < 0 || theta >= 3.14159265358979323846)
+        return fail(s, NB_ERR_INVALID_ARGUMENT, "opening angle must lie in [0, pi)");
+    s->theta = theta;
+    s->acc_valid = false;
+    return NB_OK;
+}
+
+int nb_set_collisions(nb_system *s, int mode, int mergers, double min_mom) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    if (mode != NB_NO_COLL_CHECK && mode != NB_OVERLAP_COLL_CHECK)
+        return fail(s, NB_ERR_NOT_SUPPORTED, "only collisions = 0 or 3 (overlap check) are supported");
+    if (min_mom < 0) return fail(s, NB_ERR_INVALID_ARGUMENT, "min_tot_com_mom cannot be negative");
+    s->coll_mode = mode;
+    s->mergers = mergers != 0;
+    s->min_tot_com_mom = min_mom;
+    return NB_OK;
+}
+
+int nb_set_precision(nb_system *s, int p) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    if (p != NB_PREC_FP64 && p != NB_PREC_MIXED) return fail(s, NB_ERR_INVALID_ARGUMENT, "unknown precision");
+    s->precision = p;
+    s->acc_valid = false;
+    return NB_OK;
+}
+
+int nb_evolve(nb_system *s, int method, int64_t *elapsed_ns) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    if (elapsed_ns) *elapsed_ns = 0;
+    return guarded(s, [&]() {
+        if (!s->n || !check_option(method)) return NB_OK;  // evolve returns 0 ns (:2611-2612)
+        auto t0 = std::chrono::high_resolution_clock::now();
+        if ((method & NB_TWO_BODY) == NB_TWO_BODY) {
+            if (s->n != 2) throw ApiError(NB_ERR_TWO_BODY, "A two-body integration can only be performed if there are two bodies present.");
+            throw ApiError(NB_ERR_NOT_SUPPORTED, "two_body integration is empty in the reference");
+        }
+        if ((method & NB_LEAPFROG_KDK) != NB_LEAPFROG_KDK || (method & (NB_EULER | NB_MODIFIED_EULER | NB_MIDPOINT)))
+            throw ApiError(NB_ERR_NOT_SUPPORTED, "only leapfrog_kdk (optionally | barnes_hut) is on this engine's path");
+        s->step_count = 0;
+        if ((method & NB_BARNES_HUT) == NB_BARNES_HUT) {
+            if (s->coll_mode != NB_NO_COLL_CHECK)
+                throw ApiError(NB_ERR_NOT_SUPPORTED, "Barnes-Hut with collisions (bh_s_coll) is not on this engine's path");
+            bh_forces(s, nullptr);
+            launch_kick(s->stream, s->n, s->a.p, s->v.p, s->half_dt);  // initial half kick (:2910-2913)
+            bh_steps(s, s->iterations);
+            s->acc_method = NB_BARNES_HUT;
+        } else {
+            direct_forces(s);
+            direct_kdk_steps(s, s->iterations);
+            s->acc_method = NB_LEAPFROG_KDK;
+        }
+        NB_HIP(hipStreamSynchronize(s->stream));
+        s->prof.collect();
+        s->evolved = true;
+        auto t1 = std::chrono::high_resolution_clock::now();
+        if (elapsed_ns) *elapsed_ns = (int64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+        return NB_OK;
+    });
+}
+
+int nb_advance(nb_system *s, uint64_t steps, int64_t *elapsed_ns) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    if (elapsed_ns) *elapsed_ns = 0;
+    return guarded(s, [&]() {
+        if (!s->evolved || !s->acc_valid)
+            throw ApiError(NB_ERR_NO_EVOLUTION, "nb_advance needs a previous nb_evolve with unchanged parameters");
+        auto t0 = std::chrono::high_resolution_clock::now();
+        if (s->acc_method == NB_BARNES_HUT) bh_steps(s, steps);
+        else direct_kdk_steps(s, steps);
+        NB_HIP(hipStreamSynchronize(s->stream));
+        s->prof.collect();
+        auto t1 = std::chrono::high_resolution_clock::now();
+        if (elapsed_ns) *elapsed_ns = (int64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+        return NB_OK;
+    });
+}
+
+int nb_energy(nb_system *s, double *ke, double *pe) {
+    if (!s || !ke || !pe) return fail(s, NB_ERR_INVALID_ARGUMENT, "NULL argument");
+    return guarded(s, [&]() {
+        const int64_t n = s->n;
+        *ke = 0;
+        *pe = 0;
+        if (n == 0) return NB_OK;
+        {
+            ScopedTimer t(s->prof, "energy", s->stream);
+            launch_pe_fp64(s->stream, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->pe.p);
+        }
+        std::vector<double> hpe((size_t)n), hm((size_t)n), hv((size_t)(3 * n));
+        NB_HIP(hipMemcpyAsync(hpe.data(), s->pe.p, (size_t)n * 8, hipMemcpyDeviceToHost, s->stream));
+        NB_HIP(hipMemcpyAsync(hm.data(), s->m.p, (size_t)n * 8, hipMemcpyDeviceToHost, s->stream));
+        NB_HIP(hipMemcpyAsync(hv.data(), s->v.p, (size_t)(3 * n) * 8, hipMemcpyDeviceToHost, s->stream));
+        NB_HIP(hipStreamSynchronize(s->stream));
+        s->prof.collect();
+        // body-order totals (calc_energy :620-625); KE = 0.5*m*|v|*|v| (set_ke)
+        volatile double tke = 0, tpe = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            double vx = hv[(size_t)i], vy = hv[(size_t)(n + i)], vz = hv[(size_t)(2 * n + i)];
+            double vm2 = vx * vx;
+            vm2 = vm2 + vy * vy;
+            vm2 = vm2 + vz * vz;
+            double vm = std::sqrt(vm2);
+            double k = 0.5 * hm[(size_t)i];
+            k = k * vm;
+            k = k * vm;
+            tpe = tpe + hpe[(size_t)i];
+            tke = tke + k;
+        }
+        *ke = tke;
+        *pe = tpe;
+        return NB_OK;
+    });
+}
+
+int nb_merge_log(const nb_system *cs, int64_t cap, int64_t *records, int64_t *count) {
+    nb_system *s = const_cast<nb_system *>(cs);
+    if (!s || !count) return fail(s, NB_ERR_INVALID_ARGUMENT, "NULL argument");
+    int64_t nrec = (int64_t)(s->merge_log.size() / 4);
+    *count = nrec;
+    if (records) {
+        if (cap < nrec) return fail(s, NB_ERR_CAPACITY, "nb_merge_log: capacity too small");
+        std::copy(s->merge_log.begin(), s->merge_log.end(), records);
+    }
+    return NB_OK;
+}
+
+int nb_tree_topology(nb_system *s, int64_t cap, int32_t *depth, int32_t *octant, int64_t *body, int64_t *parent,
+                     double *mass, double *com, double *side, int64_t *count) {
+    if (!s || !count) return fail(s, NB_ERR_INVALID_ARGUMENT, "NULL argument");
+    return guarded(s, [&]() { return bh_topology(s, cap, depth, octant, body, parent, mass, com, side, count); });
+}
+
+int nb_bh_accelerations(nb_system *s, double *acc, int64_t *interactions) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    return guarded(s, [&]() {
+        if (s->n == 0) return NB_OK;
+        bh_forces(s, interactions);
+        s->acc_valid = false;  // cached a no longer matches an evolve in progress
+        NB_HIP(hipStreamSynchronize(s->stream));
+        s->prof.collect();
+        if (acc) {
+            std::vector<double> tmp((size_t)(3 * s->n));
+            NB_HIP(hipMemcpy(tmp.data(), s->a.p, tmp.size() * 8, hipMemcpyDeviceToHost));
+            from_soa(s->n, tmp, acc);
+        }
+        return NB_OK;
+    });
+}
+
+int nb_adjoint(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT, const double *gvT, double *gx0,
+               double *gv0, double *gm, int64_t *fwd_ns, int64_t *bwd_ns) {
+    if (!s || !gxT || !gvT || !gx0 || !gv0 || !gm) return fail(s, NB_ERR_INVALID_ARGUMENT, "NULL argument");
+    return guarded(s, [&]() {
+        if (s->n == 0) throw ApiError(NB_ERR_EMPTY_SYSTEM, "This operation is not permitted on empty system objects (no bodies).");
+        if (steps == 0 || ckpt == 0) throw ApiError(NB_ERR_INVALID_ARGUMENT, "steps and ckpt_every must be positive");
+        if (s->coll_mode != NB_NO_COLL_CHECK)
+            throw ApiError(NB_ERR_NOT_SUPPORTED, "the adjoint covers collision-free KDK only");
+        adjoint_run(s, steps, ckpt, gxT, gvT, gx0, gv0, gm, fwd_ns, bwd_ns);
+        return NB_OK;
+    });
+}
+
+int nb_profile_enable(nb_system *s, int on) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    return guarded(s, [&]() {
+        NB_HIP(hipStreamSynchronize(s->stream));
+        s->prof.clear();
+        s->prof.on = on != 0;
+        return NB_OK;
+    });
+}
+
+int nb_profile_read(nb_system *s, const char *name, double *total_ms, int64_t *launches) {
+    if (!s || !name || !total_ms || !launches) return fail(s, NB_ERR_INVALID_ARGUMENT, "NULL argument");
+    return guarded(s, [&]() {
+        NB_HIP(hipStreamSynchronize(s->stream));
+        s->prof.collect();
+        auto it = s->prof.recs.find(name);
+        *total_ms = it == s->prof.recs.end() ? 0.0 : it->second.total_ms;
+        *launches = it == s->prof.recs.end() ? 0 : it->second.launches;
+        return NB_OK;
+    });
+}
+
+}  // extern "C"

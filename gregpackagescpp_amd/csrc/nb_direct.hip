@@ -1,0 +1,308 @@
+// nb_direct.hip -- direct-summation force and leapfrog-KDK update kernels (gfx950).
+//
+// Reference path: gtd::system::cumulative_acc (gregsys.hpp:352-357), loop_kdk
+// (:477-513), evolve's KDK branch (:2946-2968) and the softened pair form of
+// bh_acc (:371-375).
+//
+// Two force kernels:
+//  * k_force_mixed: FP32 pair forces.  Every lane owns IPL i-bodies; the
+//    j-body stream (x, y, z, G*m as float4) is wave-uniform, so it is read with
+//    scalar loads into SGPRs and each VALU op takes its j operand straight from
+//    an SGPR: no LDS, no barriers.  Per interaction: 3 sub, 3 fma (r^2+eps^2),
+//    1 v_rsq_f32, 3 mul, 3 fma = 13 VALU slots.  Sums over a tile of j run in
+//    plain FP32 and are folded into a Kahan-compensated (sum, comp) pair per
+//    tile.  The j range is split over gridDim.y so that N = 64K fills 256 CUs;
+//    the split partials are reduced in FP64 by k_update_mixed.
+//  * k_force_fp64: FP64, one lane per i, j ascending, FP contraction off, the
+//    exact operation order of the reference's pairwise update (so results are
+//    bit-identical to an IEEE-double restatement of it).
+#include "nb_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace nbe {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 64;  // j-bodies summed in plain FP32 before each Kahan fold
+
+__global__ void k_pack(int64_t n, const double *__restrict__ x, const double *__restrict__ m, double G,
+                       float4 *__restrict__ pos32) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pos32[i] = make_float4((float)x[i], (float)x[n + i], (float)x[2 * n + i], (float)(G * m[i]));
+}
+
+__device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+template <int IPL, bool SOFT>
+__global__ __launch_bounds__(kBlock) void k_force_mixed(const float4 *__restrict__ pos, int n, int jchunk, float eps2,
+                                                        float *__restrict__ partial) {
+#pragma clang fp contract(fast)
+    const int split = blockIdx.y;
+    const int j0 = split * jchunk;
+    const int j1 = min(n, j0 + jchunk);
+    const int ibase = blockIdx.x * (kBlock * IPL) + threadIdx.x;
+    float xi[IPL], yi[IPL], zi[IPL];
+    float sx[IPL], sy[IPL], sz[IPL], cx[IPL], cy[IPL], cz[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        int i = ibase + k * kBlock;
+        float4 p = i < n ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z;
+        sx[k] = sy[k] = sz[k] = cx[k] = cy[k] = cz[k] = 0.f;
+    }
+    for (int jt = j0; jt < j1; jt += kTile) {
+        const int je = min(j1, jt + kTile);
+        float px[IPL], py[IPL], pz[IPL];
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) px[k] = py[k] = pz[k] = 0.f;
+#pragma unroll 8
+        for (int j = jt; j < je; ++j) {
+            const float4 pj = pos[j];
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) {
+                float dx = pj.x - xi[k];
+                float dy = pj.y - yi[k];
+                float dz = pj.z - zi[k];
+                float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+                float inv = __builtin_amdgcn_rsqf(r2);
+                if (!SOFT) inv = r2 > 0.f ? inv : 0.f;  // self pair at eps = 0
+                float w = pj.w * inv;
+                w = w * (inv * inv);
+                px[k] = __builtin_fmaf(w, dx, px[k]);
+                py[k] = __builtin_fmaf(w, dy, py[k]);
+                pz[k] = __builtin_fmaf(w, dz, pz[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            kahan_add(sx[k], cx[k], px[k]);
+            kahan_add(sy[k], cy[k], py[k]);
+            kahan_add(sz[k], cz[k], pz[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        int i = ibase + k * kBlock;
+        if (i < n) {
+            float *pp = partial + (size_t)split * 6 * n + i;
+            pp[0] = sx[k];
+            pp[(size_t)n] = sy[k];
+            pp[2 * (size_t)n] = sz[k];
+            pp[3 * (size_t)n] = cx[k];
+            pp[4 * (size_t)n] = cy[k];
+            pp[5 * (size_t)n] = cz[k];
+        }
+    }
+}
+
+__global__ void k_update_mixed(int64_t n, const float *__restrict__ partial, int nsplit, double *__restrict__ a,
+                               double *__restrict__ v, double *__restrict__ x, const double *__restrict__ m, double G,
+                               double half_dt, double dt, int mode, float4 *__restrict__ pos32) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double s = 0.0;
+        for (int q = 0; q < nsplit; ++q) {
+            const float *pp = partial + (size_t)q * 6 * n + i;
+            s += (double)pp[(size_t)c * n] - (double)pp[(size_t)(c + 3) * n];
+        }
+        acc[c] = s;
+        a[c * n + i] = s;
+    }
+    if (mode == UPD_STORE) return;
+    double vv[3] = {v[i], v[n + i], v[2 * n + i]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vv[c] += half_dt * acc[c];  // closing half kick (loop_kdk :499)
+    if (mode == UPD_CLOSE_OPEN) {
+        double xx[3] = {x[i], x[n + i], x[2 * n + i]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            vv[c] += half_dt * acc[c];  // next step: kick half (evolve :2950-2955)
+            xx[c] += dt * vv[c];        // drift
+            x[c * n + i] = xx[c];
+        }
+        pos32[i] = make_float4((float)xx[0], (float)xx[1], (float)xx[2], (float)(G * m[i]));
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c * n + i] = vv[c];
+}
+
+// FP64 pairwise force, reference operation order: for body i the contributions
+// of j = 0..n-1 (j != i) are added in ascending j, which is the order in which
+// the reference's i<j double loop delivers them (gregsys.hpp:477-497).
+template <int FORM>
+__global__ __launch_bounds__(kBlock) void k_force_fp64(int64_t n, const double *__restrict__ x,
+                                                       const double *__restrict__ m, double G, double eps2,
+                                                       double *__restrict__ a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    const int64_t ii = live ? i : 0;
+    const double xi = x[ii], yi = x[n + ii], zi = x[2 * n + ii];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+        const double xj = x[j], yj = x[n + j], zj = x[2 * n + j], mj = m[j];
+        if (j == ii) continue;
+        double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+        if (FORM == 0) {
+            // cumulative_acc: r^3 = |(d.d) d|, k = (G m_j) / r^3
+            double s = dx * dx + dy * dy + dz * dz;
+            double wx = s * dx, wy = s * dy, wz = s * dz;
+            double r3 = __builtin_sqrt(wx * wx + wy * wy + wz * wz);
+            double k = (G * mj) / r3;
+            ax += k * dx; ay += k * dy; az += k * dz;
+        } else {
+            // bh_acc form: G*(m/s) * (d/sqrt(s)), s = d.d + eps^2
+            double s = dx * dx + dy * dy + dz * dz + eps2;
+            double sq = __builtin_sqrt(s);
+            double k = G * (mj / s);
+            ax += k * (dx / sq); ay += k * (dy / sq); az += k * (dz / sq);
+        }
+    }
+    if (live) {
+        a[i] = ax;
+        a[n + i] = ay;
+        a[2 * n + i] = az;
+    }
+}
+
+// Per-body potential energy, reference order (cumulative_pe :358-363 then the
+// per-body halving of calc_energy :620): the pair term -(G*m_lo*m_hi)/r uses
+// the lower index's mass first, as the reference's b1/b2 roles do.
+template <int FORM>
+__global__ __launch_bounds__(kBlock) void k_pe_fp64(int64_t n, const double *__restrict__ x,
+                                                    const double *__restrict__ m, double G, double eps2,
+                                                    double *__restrict__ pe) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    const int64_t ii = live ? i : 0;
+    const double xi = x[ii], yi = x[n + ii], zi = x[2 * n + ii], mi = m[ii];
+    double acc = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+        const double xj = x[j], yj = x[n + j], zj = x[2 * n + j], mj = m[j];
+        if (j == ii) continue;
+        double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+        double r = FORM == 0 ? __builtin_sqrt(dx * dx + dy * dy + dz * dz)
+                             : __builtin_sqrt(dx * dx + dy * dy + dz * dz + eps2);
+        double gmm = j < ii ? (G * mj) * mi : (G * mi) * mj;
+        acc += -gmm / r;
+    }
+    if (live) pe[i] = acc / 2.0;
+}
+
+__global__ void k_kick(int64_t n, const double *__restrict__ a, double *__restrict__ v, double coef) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    v[i] += coef * a[i];
+}
+
+__global__ void k_kick_drift(int64_t n, const double *__restrict__ a, double *__restrict__ v,
+                             double *__restrict__ x, double half_dt, double dt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    double vv = v[i] + half_dt * a[i];
+    v[i] = vv;
+    x[i] += dt * vv;
+}
+
+__global__ void k_drift(int64_t n, const double *__restrict__ v, double *__restrict__ x, double dt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    x[i] += dt * v[i];
+}
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+int choose_nsplit(int64_t n) {
+    // Aim for ~8 waves per SIMD on 256 CUs (4 SIMDs each) at IPL = 2:
+    // waves = ceil(n / 512) * 4 * nsplit.
+    int64_t wgs = (n + 2 * kBlock - 1) / (2 * kBlock);
+    int64_t want = 2048;  // workgroups
+    int64_t s = (want + wgs - 1) / wgs;
+    if (s < 1) s = 1;
+    if (s > 64) s = 64;
+    // keep at least one full Kahan tile of j per split
+    while (s > 1 && n / s < kTile) --s;
+    return (int)s;
+}
+
+void launch_pack(hipStream_t s, int64_t n, const double *x, const double *m, double G, float4 *pos32) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, x, m, G, pos32);
+    NB_HIP(hipGetLastError());
+}
+
+void launch_force_mixed(hipStream_t s, int64_t n, const float4 *pos32, float eps2, bool soft, float *partial,
+                        int nsplit) {
+    if (n <= 0) return;
+    constexpr int IPL = 2;
+    int jchunk = (int)((n + nsplit - 1) / nsplit);
+    jchunk = (jchunk + kTile - 1) / kTile * kTile;
+    dim3 grid((unsigned)((n + kBlock * IPL - 1) / (kBlock * IPL)), (unsigned)nsplit);
+    if (soft)
+        hipLaunchKernelGGL((k_force_mixed<IPL, true>), grid, dim3(kBlock), 0, s, pos32, (int)n, jchunk, eps2,
+                           partial);
+    else
+        hipLaunchKernelGGL((k_force_mixed<IPL, false>), grid, dim3(kBlock), 0, s, pos32, (int)n, jchunk, eps2,
+                           partial);
+    NB_HIP(hipGetLastError());
+}
+
+void launch_update_mixed(hipStream_t s, int64_t n, const float *partial, int nsplit, double *a, double *v, double *x,
+                         const double *m, double G, double half_dt, double dt, int mode, float4 *pos32) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_update_mixed, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, partial, nsplit, a, v, x, m, G,
+                       half_dt, dt, mode, pos32);
+    NB_HIP(hipGetLastError());
+}
+
+void launch_force_fp64(hipStream_t s, int64_t n, const double *x, const double *m, double G, double eps2, int form,
+                       double *a) {
+    if (n <= 0) return;
+    if (form == 0)
+        hipLaunchKernelGGL(k_force_fp64<0>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, x, m, G, eps2, a);
+    else
+        hipLaunchKernelGGL(k_force_fp64<1>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, x, m, G, eps2, a);
+    NB_HIP(hipGetLastError());
+}
+
+void launch_pe_fp64(hipStream_t s, int64_t n, const double *x, const double *m, double G, double eps2, int form,
+                    double *pe) {
+    if (n <= 0) return;
+    if (form == 0)
+        hipLaunchKernelGGL(k_pe_fp64<0>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, x, m, G, eps2, pe);
+    else
+        hipLaunchKernelGGL(k_pe_fp64<1>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, x, m, G, eps2, pe);
+    NB_HIP(hipGetLastError());
+}
+
+void launch_kick(hipStream_t s, int64_t n, const double *a, double *v, double coef) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_kick, dim3(blocks_for(3 * n)), dim3(kBlock), 0, s, n, a, v, coef);
+    NB_HIP(hipGetLastError());
+}
+
+void launch_kick_drift(hipStream_t s, int64_t n, const double *a, double *v, double *x, double half_dt, double dt) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(3 * n)), dim3(kBlock), 0, s, n, a, v, x, half_dt, dt);
+    NB_HIP(hipGetLastError());
+}
+
+void launch_drift(hipStream_t s, int64_t n, const double *v, double *x, double dt) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_drift, dim3(blocks_for(3 * n)), dim3(kBlock), 0, s, n, v, x, dt);
+    NB_HIP(hipGetLastError());
+}
+
+}  // namespace nbe

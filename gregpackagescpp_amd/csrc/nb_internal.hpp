@@ -1,0 +1,120 @@
+// nb_internal.hpp -- shared host/device declarations of the gfx950 N-body engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/gregnbody.h"
+
+namespace nbe {
+
+// Error carried from a failing HIP call up to the C ABI.
+struct Status {
+    int code = NB_OK;
+    std::string msg;
+};
+
+#define NB_HIP(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t _e = (expr);                                                                        \
+        if (_e != hipSuccess) {                                                                        \
+            throw ::nbe::DeviceError(std::string(#expr) + ": " + hipGetErrorString(_e));               \
+        }                                                                                              \
+    } while (0)
+
+struct DeviceError {
+    std::string what;
+    explicit DeviceError(std::string w) : what(std::move(w)) {}
+};
+struct ApiError {
+    int code;
+    std::string what;
+    ApiError(int c, std::string w) : code(c), what(std::move(w)) {}
+};
+
+// Device buffer owned by the library.
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void ensure(size_t count) {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) return;
+        NB_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+};
+
+// Per-kernel-class event timing (enabled by nb_profile_enable).
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+        double total_ms = 0;
+        int64_t launches = 0;
+    };
+    std::map<std::string, Rec> recs;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get();
+    void begin(const char *name, hipStream_t s, hipEvent_t *out);
+    void end(const char *name, hipStream_t s, hipEvent_t start);
+    void collect();  // after a stream sync
+    void clear();
+    ~Profiler();
+};
+
+struct ScopedTimer {
+    Profiler &p;
+    const char *name;
+    hipStream_t s;
+    hipEvent_t start = nullptr;
+    ScopedTimer(Profiler &pp, const char *n, hipStream_t st) : p(pp), name(n), s(st) {
+        if (p.on) p.begin(name, s, &start);
+    }
+    ~ScopedTimer() {
+        if (p.on && start) p.end(name, s, start);
+    }
+};
+
+// ---- direct summation (nb_direct.hip) ----
+struct DirectWork {
+    DBuf<float4> pos32;    // x, y, z, G*m in FP32
+    DBuf<float> partial;   // [split][6][n]: Kahan sum and compensation per component
+    int nsplit = 1;
+};
+
+// Launchers (stream-ordered, no sync).
+void launch_pack(hipStream_t s, int64_t n, const double *x, const double *m, double G, float4 *pos32);
+void launch_force_mixed(hipStream_t s, int64_t n, const float4 *pos32, float eps2, bool soft, float *partial,
+                        int nsplit);
+// Reduce the split partials to FP64 accelerations, then the closing half kick
+// (v += half_dt*a); when `next` the next step's opening half kick and drift
+// (v += half_dt*a; x += dt*v) and the FP32 repack follow in the same pass.
+void launch_update_mixed(hipStream_t s, int64_t n, const float *partial, int nsplit, double *a, double *v, double *x,
+                         const double *m, double G, double half_dt, double dt, int mode, float4 *pos32);
+void launch_force_fp64(hipStream_t s, int64_t n, const double *x, const double *m, double G, double eps2, int form,
+                       double *a);
+void launch_kick(hipStream_t s, int64_t n, const double *a, double *v, double coef);
+void launch_kick_drift(hipStream_t s, int64_t n, const double *a, double *v, double *x, double half_dt, double dt);
+void launch_drift(hipStream_t s, int64_t n, const double *v, double *x, double dt);
+void launch_pe_fp64(hipStream_t s, int64_t n, const double *x, const double *m, double G, double eps2, int form,
+                    double *pe);
+int choose_nsplit(int64_t n);
+
+// update modes for launch_update_mixed
+enum { UPD_STORE = 0, UPD_CLOSE = 1, UPD_CLOSE_OPEN = 2 };
+
+}  // namespace nbe

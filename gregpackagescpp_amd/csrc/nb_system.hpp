@@ -1,0 +1,63 @@
+// nb_system.hpp -- the context behind the C ABI handle (one gtd::system on one device).
+#pragma once
+#include "nb_internal.hpp"
+
+namespace nbe {
+struct BHWork;    // nb_bh.hip
+struct CollWork;  // nb_coll.hip
+void bh_work_free(BHWork *w);
+void coll_work_free(CollWork *w);
+}  // namespace nbe
+
+struct nb_system {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // parameters (gregsys.hpp:169-212)
+    double G = 6.6743e-11;
+    uint64_t G_vals[6] = {1, 1, 1, 1, 1, 1};
+    double dt = 1.0, half_dt = 0.5;
+    uint64_t iterations = 1000;
+    double eps = 0.0;
+    double theta = 1.0471975511965977;  // PI/3 (gregsys.hpp:212)
+    int coll_mode = NB_NO_COLL_CHECK;
+    bool mergers = false;
+    double min_tot_com_mom = 1e10;       // BILLION*10 (gregsys.hpp:209)
+    int precision = NB_PREC_FP64;
+
+    // state: SoA in HBM, [x block | y block | z block] for vectors
+    int64_t n = 0;
+    nbe::DBuf<double> m, r, e, x, v, a, pe;
+    std::vector<int64_t> labels;
+    int64_t next_label = 0;
+    bool evolved = false;
+    bool acc_valid = false;
+    int acc_method = 0;  // method whose accelerations are cached
+    uint64_t step_count = 0;
+    std::vector<int64_t> merge_log;  // 4 per record
+
+    nbe::DirectWork dw;
+    nbe::BHWork *bh = nullptr;
+    nbe::CollWork *coll = nullptr;
+    nbe::Profiler prof;
+
+    ~nb_system();
+};
+
+namespace nbe {
+// direct summation driver pieces used by several modules (nb_api.hip)
+void direct_forces(nb_system *s);                    // a(x) into s->a (and pos32 for the mixed path)
+void direct_kdk_steps(nb_system *s, uint64_t steps); // steps of KDK from valid a
+int form_of(const nb_system *s);                     // 0 = unsoftened reference form, 1 = softened
+// collisions after a step's closing kick (nb_coll.hip)
+void collision_pass(nb_system *s, uint64_t step);
+// Barnes-Hut (nb_bh.hip)
+void bh_forces(nb_system *s, int64_t *interactions);  // a(x) into s->a by the tree walk
+void bh_steps(nb_system *s, uint64_t steps);
+int bh_topology(nb_system *s, int64_t cap, int32_t *depth, int32_t *octant, int64_t *body, int64_t *parent,
+                double *mass, double *com, double *side, int64_t *count);
+// adjoint (nb_adjoint.hip)
+void adjoint_run(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT, const double *gvT, double *gx0,
+                 double *gv0, double *gm, int64_t *fwd_ns, int64_t *bwd_ns);
+}  // namespace nbe

@@ -1,0 +1,188 @@
+/*
+ * gregnbody.h -- C ABI of the MI355X (gfx950) N-body engine.
+ *
+ * This is the drop-in boundary for the force-and-step path of the reference
+ * library's gtd::system (GregW10/GregPackagesCPP, gregsys.hpp).  Every entry
+ * point names the reference interface it replaces.  Conventions:
+ *   - every function returns an int status: NB_OK (0) or a negative code that
+ *     mirrors one of the reference's exception classes; nb_last_error() gives
+ *     the message;
+ *   - host arrays are caller-owned and copied; device buffers are owned by the
+ *     library; a context owns one HIP stream on one device;
+ *   - a context is not thread-safe (as gtd::system is not, gregbod.hpp:277);
+ *     distinct contexts may be driven from distinct host threads.
+ * Positions/velocities/accelerations are passed as interleaved (x, y, z)
+ * triples of length 3*n; masses, radii and restitution coefficients as n
+ * values.  All physical state is IEEE double.
+ */
+#ifndef GREGNBODY_H
+#define GREGNBODY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (exception classes of gregbod.hpp:19-93, greg8tree.hpp:11-46,
+ *      gregsys.hpp:125-134) ---- */
+#define NB_OK 0
+#define NB_ERR_INVALID_ARGUMENT (-1)      /* std::invalid_argument                 */
+#define NB_ERR_NEGATIVE_MASS (-2)         /* negative_mass_error                   */
+#define NB_ERR_NEGATIVE_RADIUS (-3)       /* negative_radius_error                 */
+#define NB_ERR_OVERLAPPING_BODIES (-4)    /* overlapping_bodies_error              */
+#define NB_ERR_TWO_BODY (-5)              /* two_body_error                        */
+#define NB_ERR_EMPTY_SYSTEM (-6)          /* empty_system_error                    */
+#define NB_ERR_INVALID_PLACEMENT (-7)     /* invalid_body_placement (barnes_hut)   */
+#define NB_ERR_INDISTINGUISHABLE (-8)     /* indistinguishable_bodies (barnes_hut) */
+#define NB_ERR_NO_EVOLUTION (-9)          /* no_evolution_error                    */
+#define NB_ERR_NOT_SUPPORTED (-10)        /* method outside this engine's path     */
+#define NB_ERR_CAPACITY (-11)             /* caller buffer or engine limit too small */
+#define NB_ERR_DEVICE (-12)               /* HIP runtime failure                   */
+
+/* ---- integration method bit flags: gtd::system constants gregsys.hpp:156-165 ---- */
+#define NB_TWO_BODY 1
+#define NB_EULER 2
+#define NB_MODIFIED_EULER 4
+#define NB_MIDPOINT 8
+#define NB_LEAPFROG_KDK 16
+#define NB_LEAPFROG_DKD 32
+#define NB_RK4 64
+#define NB_RK3_8 128
+#define NB_BARNES_HUT 65536
+#define NB_FAST_MULTIPOLE 131072
+
+/* ---- collision modes: gtd::system template argument `collisions` (gregsys.hpp:166-168) ---- */
+#define NB_NO_COLL_CHECK 0
+#define NB_OVERLAP_COLL_CHECK 3
+
+/* ---- arithmetic of the pairwise force (an engine choice, not in the reference) ---- */
+#define NB_PREC_FP64 0   /* FP64 forces in the reference's operation order       */
+#define NB_PREC_MIXED 1  /* FP32 pair forces, Kahan-compensated FP32 sums, FP64 state */
+
+typedef struct nb_system nb_system;
+
+/* Lifecycle.  Replaces gtd::system's constructors (gregsys.hpp:1478-1578) and
+ * destructor; G starts at the SI value (system() : G{G_SI}, :1478). */
+int nb_create(int device, nb_system **out);
+int nb_destroy(nb_system *sys);
+/* Message of the last failing call on this context (static storage if sys is NULL). */
+const char *nb_last_error(const nb_system *sys);
+
+/* State in.  Replaces system(const std::vector<body>&, ..., allow_overlapping_bodies)
+ * (gregsys.hpp:1526-1545) and body validation check_mass/check_radius
+ * (gregbod.hpp:329-336).  r and rest_c may be NULL (radius 1, restitution 1, the
+ * body<> defaults gregbod.hpp:907-916).  When check_overlap != 0 the O(N^2)
+ * construction-time overlap test of system::check_overlap (gregsys.hpp:307-342)
+ * runs and NB_ERR_OVERLAPPING_BODIES is returned on the first overlap. */
+int nb_set_bodies(nb_system *sys, int64_t n, const double *m, const double *r, const double *pos,
+                  const double *vel, const double *rest_c, int check_overlap);
+/* State out (any pointer may be NULL).  Replaces iteration / operator[] over
+ * bodies (gregsys.hpp:3178-3213, 3383-3387) with body::pos()/vel()/mass()/rad()
+ * (gregbod.hpp:444-459).  cap is the capacity in bodies. */
+int nb_get_state(const nb_system *sys, int64_t cap, double *m, double *r, double *pos, double *vel, double *acc,
+                 double *rest_c);
+/* num_bodies() (gregsys.hpp:1627). */
+int nb_num_bodies(const nb_system *sys, int64_t *n);
+/* Body labels: input index for original bodies; merged bodies get n0, n0+1, ...
+ * in creation order (stands in for body_counter ids, gregbod.hpp:99-207). */
+int nb_body_ids(const nb_system *sys, int64_t cap, int64_t *ids);
+
+/* Parameters.  Setters return NB_ERR_INVALID_ARGUMENT where the reference's
+ * returns false. */
+int nb_set_units(nb_system *sys, const char *units_format); /* set_G_units / parse_units_format :245-282, 1655 */
+int nb_set_G(nb_system *sys, double G);                     /* engine extension: G directly (G = 1 configs)   */
+int nb_get_G(const nb_system *sys, double *G);              /* G_val (:1633)                                    */
+int nb_set_timestep(nb_system *sys, double dt);             /* set_timestep (:1642-1648)                        */
+int nb_set_iterations(nb_system *sys, uint64_t iterations); /* set_iterations (:1636-1641)                      */
+/* set_softening (:1660-1666): eps >= sqrt(DBL_MIN); eps == 0 restores the
+ * unsoftened direct kernel (engine extension).  eps > 0 selects the reference's
+ * softened (bh_acc, :371-375) pair form for direct summation too. */
+int nb_set_softening(nb_system *sys, double eps);
+int nb_set_bh_opening_angle(nb_system *sys, double theta); /* set_bh_opening_angle (:1667-1672) */
+/* Collision handling = the `collisions` template argument plus -DGREGSYS_MERGERS
+ * and set_min_tot_com_mom (:1674-1679, default 1e10 at :209). */
+int nb_set_collisions(nb_system *sys, int mode, int mergers, double min_tot_com_mom);
+int nb_set_precision(nb_system *sys, int precision);
+
+/* evolve(int integration_method) (gregsys.hpp:2593-2607): runs `iterations`
+ * steps of NB_LEAPFROG_KDK (direct summation) or NB_LEAPFROG_KDK|NB_BARNES_HUT.
+ * Like the reference, each call first recomputes a(x0).  elapsed_ns (may be
+ * NULL) receives the host wall time of the call (evolve's return value). */
+int nb_evolve(nb_system *sys, int method, int64_t *elapsed_ns);
+/* Engine extension: continue the previous evolve for `steps` more KDK steps
+ * without recomputing a(x0) (the cached accelerations are those of the
+ * current positions).  Returns NB_ERR_NO_EVOLUTION if nothing was evolved. */
+int nb_advance(nb_system *sys, uint64_t steps, int64_t *elapsed_ns);
+/* Total kinetic and potential energy of the current state in FP64, reference
+ * formulae (set_ke gregbod.hpp:917-920, cumulative_pe gregsys.hpp:358-363,
+ * calc_energy :608-630; bh_pe's softened form when eps > 0). */
+int nb_energy(nb_system *sys, double *ke, double *pe);
+/* Merge records since nb_set_bodies: 4 int64 per record
+ * (step, outer body label, absorbed body label, merged body label). */
+int nb_merge_log(const nb_system *sys, int64_t cap, int64_t *records, int64_t *count);
+
+/* Barnes-Hut octree of the current positions (bh_tree::add_bodies
+ * greg8tree.hpp:1066-1078, calc_com :211-282) exported in DFS pre-order
+ * (octants 0..7): depth, octant within parent (-1 root), body index (-1
+ * internal), parent pre-order index, mass, com (3 per node), side (top.x-btm.x).
+ * *count receives the node count; NB_ERR_CAPACITY if cap is too small. */
+int nb_tree_topology(nb_system *sys, int64_t cap, int32_t *depth, int32_t *octant, int64_t *body, int64_t *parent,
+                     double *mass, double *com, double *side, int64_t *count);
+/* Accelerations of the current positions by the Barnes-Hut walk without
+ * stepping (acc: 3n), and the number of accepted cells. */
+int nb_bh_accelerations(nb_system *sys, double *acc, int64_t *interactions);
+
+/* Adjoint of `steps` softened direct KDK steps from the current state (no
+ * counterpart in the reference): given dL/dx_T and dL/dv_T (3n each), writes
+ * dL/dx_0, dL/dv_0 (3n) and dL/dm (n).  Checkpoints every ckpt_every steps,
+ * recomputing each segment forward.  The state is left at time T. */
+int nb_adjoint(nb_system *sys, uint64_t steps, uint64_t ckpt_every, const double *dl_dx_T, const double *dl_dv_T,
+               double *dl_dx0, double *dl_dv0, double *dl_dm, int64_t *fwd_ns, int64_t *bwd_ns);
+
+/* Kernel timing (engine extension): when enabled, every launch of the named
+ * kernel class is bracketed by HIP events on the context's stream.
+ * names: "force", "update", "energy", "bh_build", "bh_walk", "adjoint". */
+int nb_profile_enable(nb_system *sys, int on);
+int nb_profile_read(nb_system *sys, const char *name, double *total_ms, int64_t *launches);
+
+/* ---- batched ensembles: many independent small systems in one persistent
+ *      launch (the run_sim thread-per-system pattern, simsup.hpp:209-391) ---- */
+typedef struct nb_batch nb_batch;
+/* offsets[nsys+1] partitions the body arrays; G, dt, eps shared. */
+int nb_batch_create(int device, int64_t nsys, const int64_t *offsets, const double *m, const double *r,
+                    const double *pos, const double *vel, const double *rest_c, nb_batch **out);
+int nb_batch_destroy(nb_batch *b);
+const char *nb_batch_last_error(const nb_batch *b);
+int nb_batch_set_params(nb_batch *b, double G, double dt, double eps, int mode, int mergers, double min_tot_com_mom,
+                        int precision);
+int nb_batch_evolve(nb_batch *b, uint64_t steps, int64_t *elapsed_ns);
+/* Per-system body counts (nsys) and the state packed per system in body
+ * order; cap is in bodies. */
+int nb_batch_get_state(const nb_batch *b, int64_t cap, int64_t *counts, double *m, double *r, double *pos,
+                       double *vel, int64_t *ids);
+/* Merge records of all systems: 5 int64 per record
+ * (system, step, outer label, absorbed label, merged label). */
+int nb_batch_merge_log(const nb_batch *b, int64_t cap, int64_t *records, int64_t *count);
+
+/* ---- seeded initial conditions (SplitMix64 stream; engine utility) ---- */
+/* Uniform ball (the reference's own recipe r = R cbrt(U), cos(th) = 1-2U,
+ * phi = 2 pi U, gregsys.hpp:1818-1824), masses U[mlo, mhi], Gaussian velocities
+ * (Box-Muller, sigma), shifted to the centre-of-mass frame. */
+int nb_ic_uniform_ball(uint64_t seed, int64_t n, double mlo, double mhi, double radius, double sigma, double *m,
+                       double *pos, double *vel);
+/* Plummer sphere, G = M = 1, scale a, equal masses 1/n (Aarseth, Henon &
+ * Wielen 1974), centre-of-mass frame.  offset/vel_offset shift the sphere. */
+int nb_ic_plummer(uint64_t seed, int64_t n, double a, const double *offset, const double *vel_offset, double *m,
+                  double *pos, double *vel);
+/* Ensemble of comet strings around a central body: system s has 1 + n_s bodies
+ * (central body first); counts[nsys] receives n_s + 1 and offsets[nsys+1] the
+ * packing.  When m == NULL only counts/offsets are produced (sizing pass). */
+int nb_ic_comet_ensemble(uint64_t seed, int64_t nsys, int64_t nmin, int64_t nmax, double central_mass,
+                         double central_radius, double G, int64_t *counts, int64_t *offsets, double *m, double *r,
+                         double *pos, double *vel, double *rest_c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GREGNBODY_H */

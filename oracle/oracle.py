@@ -1,0 +1,278 @@
+"""ctypes binding of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is the CPU restatement of the reference's force-and-step path
+(nbody_oracle.cpp).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+LD = np.longdouble
+_LD = C.c_longdouble
+_LDP = C.POINTER(C.c_longdouble)
+_D = C.POINTER(C.c_double)
+_I64 = C.POINTER(C.c_int64)
+_I32 = C.POINTER(C.c_int32)
+
+FORM_NEWTON = 0
+FORM_SOFT = 1
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE, "oracle"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_direct_acc_ld.argtypes = [C.c_int64, C.c_int, _LDP, _LDP, _LDP, _LDP, _LDP]
+        L.orc_direct_acc_d.argtypes = [C.c_int64, C.c_int, C.c_double, C.c_double, _D, _D, _D]
+        L.orc_energy_ld.argtypes = [C.c_int64, C.c_int, _LDP, _LDP, _LDP, _LDP, _LDP, _LDP, _LDP]
+        L.orc_energy_d.argtypes = [C.c_int64, C.c_int, C.c_double, C.c_double, _D, _D, _D, _D, _D]
+        L.orc_kdk_ld.argtypes = [_I64, C.c_int, _LDP, _LDP, _LDP, C.c_int64, C.c_int, _LDP, _LDP, _LDP, _LDP, _LDP,
+                                 _LDP, _LDP, _I64, _I64, C.c_int64, _I64]
+        L.orc_kdk_d.argtypes = [_I64, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int, C.c_double,
+                                _D, _D, _D, _D, _D, _D, _I64, _I64, C.c_int64, _I64]
+        L.orc_s_coll_d.argtypes = [_I64, C.c_int, C.c_double, C.c_int64, _D, _D, _D, _D, _D, _D, _I64, _I64, _I64,
+                                   C.c_int64, _I64]
+        L.orc_bh_tree_d.restype = C.c_int64
+        L.orc_bh_tree_d.argtypes = [C.c_int64, _D, _D, C.c_int64, _I32, _I32, _I64, _I64, _D, _D, _D, _D]
+        L.orc_bh_acc_d.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, _D, _D, _D, _I64, _I64]
+        L.orc_bh_accepted_d.restype = C.c_int64
+        L.orc_bh_accepted_d.argtypes = [C.c_int64, C.c_double, _D, _D, C.c_int64, C.c_int64, _I64]
+        L.orc_bh_kdk_d.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64, _D, _D,
+                                   _D, _D]
+        L.orc_adjoint_d.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64, _D, _D, _D, _D, _D, _D,
+                                    _D, _D]
+        L.orc_forward_d.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64, _D, _D, _D]
+        _lib = L
+    return _lib
+
+
+def _ld(a):
+    return np.ascontiguousarray(a, dtype=LD)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+_keep = []
+
+
+def _sc(x):
+    """Pointer to a long double scalar (kept alive until the next call)."""
+    a = np.array([x], dtype=LD)
+    _keep.append(a)
+    del _keep[:-16]
+    return a.ctypes.data_as(_LDP)
+
+
+def _lp(a):
+    return None if a is None else a.ctypes.data_as(_LDP)
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(_I64)
+
+
+def direct_acc(m, pos, G, eps=0.0, form=FORM_NEWTON, dtype=LD):
+    n = len(m)
+    if dtype == LD:
+        m, pos = _ld(m), _ld(pos)
+        acc = np.zeros((n, 3), LD)
+        lib().orc_direct_acc_ld(n, form, _sc(G), _sc(eps), _lp(m), _lp(pos), _lp(acc))
+    else:
+        m, pos = _d(m), _d(pos)
+        acc = np.zeros((n, 3))
+        lib().orc_direct_acc_d(n, form, G, eps, _dp(m), _dp(pos), _dp(acc))
+    return acc
+
+
+def energy(m, pos, vel, G, eps=0.0, form=FORM_NEWTON, dtype=LD):
+    n = len(m)
+    if dtype == LD:
+        m, pos, vel = _ld(m), _ld(pos), _ld(vel)
+        out = np.zeros(2, LD)
+        lib().orc_energy_ld(n, form, _sc(G), _sc(eps), _lp(m), _lp(pos), _lp(vel), _lp(out[0:1]), _lp(out[1:2]))
+        return out[0], out[1]
+    m, pos, vel = _d(m), _d(pos), _d(vel)
+    ke, pe = C.c_double(), C.c_double()
+    lib().orc_energy_d(n, form, G, eps, _dp(m), _dp(pos), _dp(vel), C.byref(ke), C.byref(pe))
+    return ke.value, pe.value
+
+
+def kdk(m, pos, vel, G, dt, steps, eps=0.0, form=FORM_NEWTON, dtype=LD, coll=0, r=None, e=None, min_mom=1e10,
+        labels=None, log_cap=100000):
+    """Leapfrog KDK (+ optional s_coll: coll=1 bounce, coll=2 mergers). Returns dict."""
+    n = len(m)
+    conv = _ld if dtype == LD else _d
+    ptr = _lp if dtype == LD else _dp
+    m, pos, vel = conv(m).copy(), conv(pos).copy(), conv(vel).copy()
+    r = conv(r if r is not None else np.zeros(n)).copy()
+    e = conv(e if e is not None else np.ones(n)).copy()
+    acc = np.zeros((n, 3), dtype=m.dtype)
+    lab = np.ascontiguousarray(np.arange(n) if labels is None else labels, dtype=np.int64).copy()
+    log = np.zeros((log_cap, 4), dtype=np.int64)
+    nn = C.c_int64(n)
+    loglen = C.c_int64()
+    if dtype == LD:
+        lib().orc_kdk_ld(C.byref(nn), form, _sc(G), _sc(dt), _sc(eps), steps, coll, _sc(min_mom), ptr(m), ptr(r),
+                         ptr(e), ptr(pos), ptr(vel), ptr(acc), _ip(lab), _ip(log), log_cap, C.byref(loglen))
+    else:
+        lib().orc_kdk_d(C.byref(nn), form, float(G), float(dt), float(eps), steps, coll, float(min_mom), ptr(m),
+                        ptr(r), ptr(e), ptr(pos), ptr(vel), ptr(acc), _ip(lab), _ip(log), log_cap, C.byref(loglen))
+    k = nn.value
+    return dict(n=k, m=m[:k], r=r[:k], e=e[:k], pos=pos[:k], vel=vel[:k], acc=acc[:k], labels=lab[:k],
+                log=log[: min(loglen.value, log_cap)])
+
+
+def s_coll(m, r, e, pos, vel, acc, labels, next_label, merge=True, min_mom=1e10, step=1, log_cap=10000):
+    n = len(m)
+    m, r, e = _d(m).copy(), _d(r).copy(), _d(e).copy()
+    pos, vel, acc = _d(pos).copy(), _d(vel).copy(), _d(acc).copy()
+    lab = np.ascontiguousarray(labels, dtype=np.int64).copy()
+    nl = C.c_int64(next_label)
+    nn = C.c_int64(n)
+    log = np.zeros((log_cap, 4), dtype=np.int64)
+    loglen = C.c_int64()
+    lib().orc_s_coll_d(C.byref(nn), int(merge), min_mom, step, _dp(m), _dp(r), _dp(e), _dp(pos), _dp(vel), _dp(acc),
+                       _ip(lab), C.byref(nl), _ip(log), log_cap, C.byref(loglen))
+    k = nn.value
+    return dict(n=k, m=m[:k], r=r[:k], e=e[:k], pos=pos[:k], vel=vel[:k], acc=acc[:k], labels=lab[:k],
+                next_label=nl.value, log=log[: loglen.value])
+
+
+def bh_tree(m, pos, cap=None):
+    n = len(m)
+    m, pos = _d(m), _d(pos)
+    cap = cap or 4 * n + 64
+    while True:
+        depth, octant = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        body, parent = np.zeros(cap, np.int64), np.zeros(cap, np.int64)
+        mass, com, side = np.zeros(cap), np.zeros((cap, 3)), np.zeros(cap)
+        box = np.zeros(6)
+        k = lib().orc_bh_tree_d(n, _dp(m), _dp(pos), cap, depth.ctypes.data_as(_I32), octant.ctypes.data_as(_I32),
+                                _ip(body), _ip(parent), _dp(mass), _dp(com), _dp(side), _dp(box))
+        if k == -1:
+            cap *= 2
+            continue
+        if k < 0:
+            raise RuntimeError(f"oracle tree build failed: {k}")
+        return dict(depth=depth[:k], octant=octant[:k], body=body[:k], parent=parent[:k], mass=mass[:k],
+                    com=com[:k], side=side[:k], box=box)
+
+
+def bh_acc(m, pos, G, eps, theta, per_body=False):
+    n = len(m)
+    m, pos = _d(m), _d(pos)
+    acc = np.zeros((n, 3))
+    cnt = C.c_int64()
+    pb = np.zeros(n, np.int64) if per_body else None
+    rc = lib().orc_bh_acc_d(n, G, eps, theta, _dp(m), _dp(pos), _dp(acc), C.byref(cnt), _ip(pb))
+    if rc:
+        raise RuntimeError(f"oracle bh_acc failed: {rc}")
+    return (acc, cnt.value, pb) if per_body else (acc, cnt.value)
+
+
+def bh_accepted(m, pos, theta, body):
+    n = len(m)
+    out = np.zeros(8 * n + 64, np.int64)
+    k = lib().orc_bh_accepted_d(n, theta, _dp(_d(m)), _dp(_d(pos)), body, out.size, _ip(out))
+    return out[:k]
+
+
+def bh_kdk(m, pos, vel, G, dt, eps, theta, steps):
+    n = len(m)
+    m, pos, vel = _d(m), _d(pos).copy(), _d(vel).copy()
+    acc = np.zeros((n, 3))
+    rc = lib().orc_bh_kdk_d(n, G, dt, eps, theta, steps, _dp(m), _dp(pos), _dp(vel), _dp(acc))
+    if rc:
+        raise RuntimeError(f"oracle bh_kdk failed: {rc}")
+    return pos, vel, acc
+
+
+def adjoint(m, x0, v0, G, dt, eps, steps, gxT, gvT):
+    n = len(m)
+    gx0, gv0, gm = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    lib().orc_adjoint_d(n, G, dt, eps, steps, _dp(_d(m)), _dp(_d(x0)), _dp(_d(v0)), _dp(_d(gxT)), _dp(_d(gvT)),
+                        _dp(gx0), _dp(gv0), _dp(gm))
+    return gx0, gv0, gm
+
+
+def forward(m, x0, v0, G, dt, eps, steps):
+    x, v = _d(x0).copy(), _d(v0).copy()
+    lib().orc_forward_d(len(m), G, dt, eps, steps, _dp(_d(m)), _dp(x), _dp(v))
+    return x, v
+
+
+# ---------------- compiled reference (oracle/_ref, this container only) ----------------
+def ref_available():
+    return os.path.exists(os.path.join(REF_DIR, "ref_harness")) and os.path.exists(
+        os.path.join(REF_DIR, "ref_harness_merge"))
+
+
+def write_ref_input(path, m, pos, vel, r=None, e=None, steps=0, dt=1.0):
+    n = len(m)
+    r = np.zeros(n) if r is None else r
+    e = np.ones(n) if e is None else e
+    rec = np.zeros((n, 9))
+    rec[:, 0], rec[:, 1], rec[:, 2] = m, r, e
+    rec[:, 3:6], rec[:, 6:9] = np.asarray(pos).reshape(n, 3), np.asarray(vel).reshape(n, 3)
+    with open(path, "wb") as f:
+        f.write(np.array([n, steps], dtype=np.int64).tobytes())
+        f.write(np.array([dt], dtype=np.float64).tobytes())
+        f.write(rec.astype(np.float64).tobytes())
+
+
+def run_ref(mode, inp, out, merge=False, timeout=3600):
+    exe = os.path.join(REF_DIR, "ref_harness_merge" if merge else "ref_harness")
+    subprocess.run([exe, mode, inp, out], check=True, timeout=timeout)
+
+
+def read_ref_kdk(path, n):
+    a = np.fromfile(path, dtype=LD)
+    return a.reshape(n, 6)
+
+
+def read_ref_coll(path):
+    raw = open(path, "rb").read()
+    n0 = int(np.frombuffer(raw[:8], np.int64)[0])
+    ids0 = np.frombuffer(raw[8:8 + 8 * n0], np.int64).copy()
+    off = 8 + 8 * n0
+    rec_sz = 16 + 8 * 16
+    k = (len(raw) - off) // rec_sz
+    steps = np.zeros(k, np.int64)
+    ids = np.zeros(k, np.int64)
+    vals = np.zeros((k, 8), LD)
+    for q in range(k):
+        b = raw[off + q * rec_sz: off + (q + 1) * rec_sz]
+        hdr = np.frombuffer(b[:16], np.int64)
+        steps[q], ids[q] = hdr
+        vals[q] = np.frombuffer(b[16:], LD)
+    return ids0, steps, ids, vals
+
+
+def read_ref_tree(path):
+    raw = open(path, "rb").read()
+    cnt = int(np.frombuffer(raw[:8], np.int64)[0])
+    rec = np.frombuffer(raw[8:], dtype=np.dtype([("d", "<i8"), ("o", "<i8"), ("b", "<i8"), ("m", "<f8")]))
+    assert rec.shape[0] == cnt
+    return rec

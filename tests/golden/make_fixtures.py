@@ -1,0 +1,82 @@
+"""Generate golden vectors from the COMPILED REFERENCE (oracle/_ref).
+
+Runs only where /root/reference is mounted (the harness is built from its
+headers by oracle/Makefile).  Inputs come from the engine's seeded IC
+generators; outputs are the reference's own results, stored as .npz:
+
+  ref_kdk_n{N}_s{S}.npz  gtd::system<ld,...>::evolve(leapfrog_kdk), G = SI,
+                         dt = 10 s, uniform-ball ICs (seed 0): inputs (double)
+                         and final pos/vel (x87 long double).
+  ref_tree_plummer_n{N}.npz  bh_tree<double> of a Plummer sphere (seed 5):
+                         DFS pre-order depth, octant, body index, node mass.
+  ref_coll_{bounce,merge}_seed{K}.npz  system<ld,...,collisions=3> without /
+                         with -DGREGSYS_MERGERS: every body after every step.
+
+Usage: python tests/golden/make_fixtures.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle as O  # noqa: E402
+from gregpackagescpp_amd import capi  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+TMP = os.path.join(ROOT, "_work", "fixtures_tmp")
+
+KDK_CASES = [(2, 1), (2, 1000), (3, 10), (64, 10), (64, 1000), (1024, 10), (1024, 1000)]
+TREE_CASES = [2, 3, 17, 1000, 16384]
+COLL_CASES = [(24, 300, s) for s in range(4)]
+
+
+def coll_ics(n, seed):
+    """Rubble cluster: bodies of 1e11-1e13 kg at 500 kg/m^3 in a 12 km box."""
+    rng = np.random.default_rng(seed)
+    m = 10 ** rng.uniform(11, 13, n)
+    r = (3 * m / (4 * np.pi * 500)) ** (1 / 3)
+    pos = rng.uniform(-6e3, 6e3, (n, 3))
+    vel = rng.normal(0, 0.5, (n, 3))
+    e = rng.uniform(0.1, 0.9, n)
+    return m, r, e, pos, vel
+
+
+def main():
+    if not O.ref_available():
+        sys.exit("oracle/_ref is not built (needs /root/reference): nothing generated")
+    os.makedirs(TMP, exist_ok=True)
+    inp, out = os.path.join(TMP, "in.bin"), os.path.join(TMP, "out.bin")
+    for n, steps in KDK_CASES:
+        m, pos, vel = capi.ic_uniform_ball(0, n, 1e20, 1e22, 1e9, 1e3)
+        O.write_ref_input(inp, m, pos, vel, steps=steps, dt=10.0)
+        O.run_ref("kdk", inp, out)
+        res = O.read_ref_kdk(out, n)
+        np.savez(os.path.join(OUT, f"ref_kdk_n{n}_s{steps}.npz"), m=m, pos=pos, vel=vel, dt=10.0, steps=steps,
+                 pos_T=res[:, :3], vel_T=res[:, 3:])
+        print("kdk", n, steps)
+    for n in TREE_CASES:
+        m, pos, vel = capi.ic_plummer(5, n)
+        O.write_ref_input(inp, m, pos, vel)
+        O.run_ref("tree", inp, out)
+        rec = O.read_ref_tree(out)
+        np.savez(os.path.join(OUT, f"ref_tree_plummer_n{n}.npz"), m=m, pos=pos, depth=rec["d"], octant=rec["o"],
+                 body=rec["b"], mass=rec["m"])
+        print("tree", n, len(rec))
+    for n, steps, seed in COLL_CASES:
+        m, r, e, pos, vel = coll_ics(n, seed)
+        for merge in (False, True):
+            O.write_ref_input(inp, m, pos, vel, r=r, e=e, steps=steps, dt=5.0)
+            O.run_ref("coll", inp, out, merge=merge)
+            ids0, st, ids, vals = O.read_ref_coll(out)
+            tag = "merge" if merge else "bounce"
+            np.savez(os.path.join(OUT, f"ref_coll_{tag}_seed{seed}.npz"), m=m, r=r, e=e, pos=pos, vel=vel, dt=5.0,
+                     steps=steps, ids0=ids0, step=st, ids=ids, vals=vals)
+            print("coll", tag, seed, len(st))
+
+
+if __name__ == "__main__":
+    main()
