@@ -263,6 +263,13 @@ void s_coll(std::vector<CBody<T>> &bods, bool merge, T min_tot_com_mom, int64_t 
     }
 }
 
+/* Optional per-step trace of every body after the step (and its s_coll):
+ * (step, label) + (m, r, x, y, z, vx, vy, vz) -- the same record the reference
+ * harness dumps from evolve's per-body callback. */
+bool g_trace_on = false;
+std::vector<int64_t> g_trace_hdr;
+std::vector<long double> g_trace_vals;
+
 /* Leapfrog KDK, direct summation (gregsys.hpp:2946-2968 with loop_kdk :477-513
  * and leapfrog_kdk_acc_e_and_step :514-526): initial a(x0); per step
  * v += half_dt*a ; x += dt*v ; recompute a ; v += half_dt*a ; then s_coll when
@@ -301,6 +308,14 @@ int kdk(int64_t *n_io, int form, T G, T dt, T eps, int64_t steps, int coll, T mi
         for (auto &b : bods)
             for (int k = 0; k < 3; ++k) b.v[k] += half_dt * b.a[k];
         if (coll) s_coll<T>(bods, coll == 2, min_mom, s, next_label, log);
+        if (g_trace_on)
+            for (auto &b : bods) {
+                g_trace_hdr.push_back(s);
+                g_trace_hdr.push_back(b.label);
+                long double v[8] = {(long double)b.m, (long double)b.r, (long double)b.p[0], (long double)b.p[1],
+                                    (long double)b.p[2], (long double)b.v[0], (long double)b.v[1], (long double)b.v[2]};
+                g_trace_vals.insert(g_trace_vals.end(), v, v + 8);
+            }
     }
     n = (int64_t)bods.size();
     *n_io = n;
@@ -749,6 +764,17 @@ int orc_adjoint_d(int64_t n, double G, double dt, double eps, int64_t steps, con
     std::copy(lx.begin(), lx.end(), gx0);
     std::copy(lv.begin(), lv.end(), gv0);
     return 0;
+}
+
+void orc_trace(int on) {
+    g_trace_on = on != 0;
+    g_trace_hdr.clear();
+    g_trace_vals.clear();
+}
+int64_t orc_trace_len(void) { return (int64_t)(g_trace_hdr.size() / 2); }
+void orc_trace_copy(int64_t *hdr, long double *vals) {
+    std::copy(g_trace_hdr.begin(), g_trace_hdr.end(), hdr);
+    std::copy(g_trace_vals.begin(), g_trace_vals.end(), vals);
 }
 
 /* Forward-only helper for finite differences: final x, v of the softened KDK. */

@@ -58,6 +58,9 @@ def lib():
         L.orc_adjoint_d.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64, _D, _D, _D, _D, _D, _D,
                                     _D, _D]
         L.orc_forward_d.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64, _D, _D, _D]
+        L.orc_trace.argtypes = [C.c_int]
+        L.orc_trace_len.restype = C.c_int64
+        L.orc_trace_copy.argtypes = [_I64, _LDP]
         _lib = L
     return _lib
 
@@ -142,6 +145,22 @@ def kdk(m, pos, vel, G, dt, steps, eps=0.0, form=FORM_NEWTON, dtype=LD, coll=0, 
     k = nn.value
     return dict(n=k, m=m[:k], r=r[:k], e=e[:k], pos=pos[:k], vel=vel[:k], acc=acc[:k], labels=lab[:k],
                 log=log[: min(loglen.value, log_cap)])
+
+
+def kdk_trace(*args, **kw):
+    """kdk() that also returns every body's state after every step:
+    (steps[k], labels[k], vals[k, 8] = m, r, x, y, z, vx, vy, vz)."""
+    L = lib()
+    L.orc_trace(1)
+    try:
+        res = kdk(*args, **kw)
+        k = L.orc_trace_len()
+        hdr = np.zeros((k, 2), np.int64)
+        vals = np.zeros((k, 8), LD)
+        L.orc_trace_copy(_ip(hdr), _lp(vals))
+    finally:
+        L.orc_trace(0)
+    return res, hdr[:, 0], hdr[:, 1], vals
 
 
 def s_coll(m, r, e, pos, vel, acc, labels, next_label, merge=True, min_mom=1e10, step=1, log_cap=10000):

@@ -55,7 +55,7 @@ def main():
         O.write_ref_input(inp, m, pos, vel, steps=steps, dt=10.0)
         O.run_ref("kdk", inp, out)
         res = O.read_ref_kdk(out, n)
-        np.savez(os.path.join(OUT, f"ref_kdk_n{n}_s{steps}.npz"), m=m, pos=pos, vel=vel, dt=10.0, steps=steps,
+        np.savez_compressed(os.path.join(OUT, f"ref_kdk_n{n}_s{steps}.npz"), m=m, pos=pos, vel=vel, dt=10.0, steps=steps,
                  pos_T=res[:, :3], vel_T=res[:, 3:])
         print("kdk", n, steps)
     for n in TREE_CASES:
@@ -63,7 +63,7 @@ def main():
         O.write_ref_input(inp, m, pos, vel)
         O.run_ref("tree", inp, out)
         rec = O.read_ref_tree(out)
-        np.savez(os.path.join(OUT, f"ref_tree_plummer_n{n}.npz"), m=m, pos=pos, depth=rec["d"], octant=rec["o"],
+        np.savez_compressed(os.path.join(OUT, f"ref_tree_plummer_n{n}.npz"), m=m, pos=pos, depth=rec["d"], octant=rec["o"],
                  body=rec["b"], mass=rec["m"])
         print("tree", n, len(rec))
     for n, steps, seed in COLL_CASES:
@@ -73,7 +73,7 @@ def main():
             O.run_ref("coll", inp, out, merge=merge)
             ids0, st, ids, vals = O.read_ref_coll(out)
             tag = "merge" if merge else "bounce"
-            np.savez(os.path.join(OUT, f"ref_coll_{tag}_seed{seed}.npz"), m=m, r=r, e=e, pos=pos, vel=vel, dt=5.0,
+            np.savez_compressed(os.path.join(OUT, f"ref_coll_{tag}_seed{seed}.npz"), m=m, r=r, e=e, pos=pos, vel=vel, dt=5.0,
                      steps=steps, ids0=ids0, step=st, ids=ids, vals=vals)
             print("coll", tag, seed, len(st))
 
