@@ -1,13 +1,694 @@
-// nb_bh.hip -- Barnes-Hut octree (placeholder until the tree kernels land).
+// nb_bh.hip -- Barnes-Hut octree build, centre-of-mass pass and tree walk (gfx950).
+//
+// Reference path: bh_tree::find_min_max / add_bodies (greg8tree.hpp:1012-1078),
+// bh_cube::set_corners / dispatch (:81-210), calc_com (:211-282),
+// cell_iterator (:702-755) with bh_acc (gregsys.hpp:371-375), and evolve's
+// KDK + Barnes-Hut loop (gregsys.hpp:2893-2936).
+//
+// The reference inserts bodies one by one into a pointer octree whose leaves
+// hold exactly one body.  That topology depends only on each body's sequence
+// of octant digits under the midpoint recursion mid = (lo + hi)/2 from the
+// padded root cube, so it is rebuilt here without insertion:
+//  1. bounding cube (FP64 min/max reduction; pad = L/20 -- the reference's
+//     long-double quotient rounded to double equals the double quotient);
+//  2. per body, 42 levels of octant digits by the same FP64 recursion, packed
+//     into two 63-bit keys (levels 1-21, 22-42);
+//  3. radix sort by the high key, gather of the low key, and an insertion-sort
+//     fix of the rare runs with equal high keys;
+//  4. c[s] = shared digits of sorted neighbours s, s+1.  An internal node is a
+//     maximal run sharing L >= 1 digits; nodes starting at s are levels
+//     (c[s-1], c[s]] plus the leaf of s, so an exclusive scan of
+//     1 + max(0, c[s]-c[s-1]) numbers every node in DFS pre-order with octants
+//     0..7 -- the order cell_iterator visits them.  A node's last position
+//     (first e >= s with c[e] < L) comes from a sparse-table range minimum;
+//     its skip pointer (next node outside its subtree) is the first node
+//     starting at e + 1;
+//  5. node mass and COM bottom up in FP64 (a leaf climbs while it is the last
+//     child to finish), children summed in reverse octant order as calc_com;
+//  6. the walk is stackless over (node, skip): descend = node + 1, accept or
+//     skip = skip[node].  Two walks: per lane with the reference's per-body
+//     MAC in FP64 (parity), and per wave with FP32 monopoles where a node is
+//     opened for all 64 lanes when any lane needs it (node data is
+//     wave-uniform, so it arrives through scalar loads).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "nb_sort.hpp"
 #include "nb_system.hpp"
 
+#pragma clang fp contract(off)
+
 namespace nbe {
-struct BHWork {};
+
+namespace {
+constexpr int kBlock = 256;
+constexpr int kLevels = 42;   // octant digits per body
+constexpr int kHiLevels = 21; // digits in the high key
+constexpr int kMaxLog = 32;
+
+enum { BH_ERR_NONE = 0, BH_ERR_PLACEMENT = 1, BH_ERR_INDIST = 2, BH_ERR_DEPTH = 3 };
+}  // namespace
+
+struct BHWork {
+    int64_t cap_n = 0, cap_nodes = 0;
+    DBuf<double> part;      // bbox partials
+    DBuf<double> box;       // btm[3], top[3]
+    DBuf<int> err;          // BH_ERR_*
+    DBuf<uint64_t> khi, klo, klo_tmp;
+    DBuf<uint32_t> idx;
+    DBuf<uint8_t> st;       // sparse table [log][n]
+    DBuf<uint32_t> cnt;     // nodes starting at s, then their exclusive scan
+    DBuf<uint32_t> blk;     // scan block sums
+    DBuf<uint32_t> total;   // node count
+    // nodes (DFS pre-order)
+    DBuf<int32_t> first, last, skip, parent, body;
+    DBuf<uint8_t> depth, nchild;
+    DBuf<int8_t> octant;
+    DBuf<double> mass, com, side;
+    DBuf<uint32_t> ready;
+    DBuf<float4> f4;        // com (FP32) and G*mass
+    DBuf<float> r2open;     // (side/theta)^2, -1 for leaves
+    DBuf<float4> pos32s;    // sorted body positions (FP32) and G*m
+    DBuf<double> accs;      // sorted accelerations, SoA
+    DBuf<unsigned long long> icount;
+    RadixSorter sorter;
+    int64_t nodes = 0;
+    uint32_t *h_total = nullptr;
+    int *h_err = nullptr;
+    ~BHWork() {
+        if (h_total) (void)hipHostFree(h_total);
+        if (h_err) (void)hipHostFree(h_err);
+    }
+};
+
 void bh_work_free(BHWork *w) { delete w; }
-void bh_forces(nb_system *, int64_t *) { throw ApiError(NB_ERR_NOT_SUPPORTED, "Barnes-Hut not built yet"); }
-void bh_steps(nb_system *, uint64_t) { throw ApiError(NB_ERR_NOT_SUPPORTED, "Barnes-Hut not built yet"); }
-int bh_topology(nb_system *, int64_t, int32_t *, int32_t *, int64_t *, int64_t *, double *, double *, double *,
-                int64_t *) {
-    throw ApiError(NB_ERR_NOT_SUPPORTED, "Barnes-Hut not built yet");
+
+namespace {
+
+// ---- 1. bounding cube ----
+__global__ __launch_bounds__(kBlock) void k_bbox_partial(int64_t n, const double *__restrict__ x,
+                                                         double *__restrict__ part) {
+    __shared__ double sm[6][kBlock];
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        for (int c = 0; c < 3; ++c) {
+            double v = x[c * n + i];
+            mn[c] = fmin(mn[c], v);
+            mx[c] = fmax(mx[c], v);
+        }
+    for (int c = 0; c < 3; ++c) {
+        sm[c][threadIdx.x] = mn[c];
+        sm[3 + c][threadIdx.x] = mx[c];
+    }
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int c = 0; c < 3; ++c) {
+                sm[c][threadIdx.x] = fmin(sm[c][threadIdx.x], sm[c][threadIdx.x + off]);
+                sm[3 + c][threadIdx.x] = fmax(sm[3 + c][threadIdx.x], sm[3 + c][threadIdx.x + off]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sm[threadIdx.x][0];
 }
+
+// find_min_max (greg8tree.hpp:1012-1047): longest edge with the reference's
+// tie rule, pad = L/20, btm = min - pad, L' = L + 2*pad, top = btm + L'.
+__global__ void k_bbox_final(int nparts, const double *__restrict__ part, double *__restrict__ box) {
+    if (threadIdx.x != 0) return;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = 0; b < nparts; ++b)
+        for (int c = 0; c < 3; ++c) {
+            mn[c] = fmin(mn[c], part[b * 6 + c]);
+            mx[c] = fmax(mx[c], part[b * 6 + 3 + c]);
+        }
+    double xr = mx[0] - mn[0], yr = mx[1] - mn[1], zr = mx[2] - mn[2];
+    double le = xr > yr ? (xr > zr ? xr : zr) : (yr > zr ? yr : zr);
+    double pad = le / 20.0;
+    for (int c = 0; c < 3; ++c) box[c] = mn[c] - pad;
+    le += 2 * pad;
+    for (int c = 0; c < 3; ++c) box[3 + c] = box[c] + le;
+}
+
+// ---- 2. octant digits by the reference's midpoint recursion ----
+__global__ __launch_bounds__(kBlock) void k_keys(int64_t n, const double *__restrict__ x,
+                                                 const double *__restrict__ box, uint64_t *__restrict__ khi,
+                                                 uint64_t *__restrict__ klo, uint32_t *__restrict__ idx,
+                                                 int *__restrict__ err) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double p[3], lo[3], hi[3];
+    bool inside = true;
+    for (int c = 0; c < 3; ++c) {
+        p[c] = x[c * n + i];
+        lo[c] = box[c];
+        hi[c] = box[3 + c];
+        inside = inside && lo[c] <= p[c] && p[c] < hi[c];  // between (gregvec.hpp:2115-2120)
+    }
+    if (!inside) atomicMax(err, BH_ERR_PLACEMENT);
+    uint64_t h = 0, l = 0;
+    for (int lev = 1; lev <= kLevels; ++lev) {
+        unsigned d = 0;
+        for (int c = 0; c < 3; ++c) {
+            double mid = (lo[c] + hi[c]) * 0.5;  // (btm + top)/2.0l, exact halving
+            bool up = p[c] >= mid;
+            d |= (unsigned)up << c;
+            lo[c] = up ? mid : lo[c];
+            hi[c] = up ? hi[c] : mid;
+        }
+        if (lev <= kHiLevels) h = (h << 3) | d;
+        else l = (l << 3) | d;
+    }
+    khi[i] = h;
+    klo[i] = l;
+    idx[i] = (uint32_t)i;
+}
+
+__global__ void k_gather_lo(int64_t n, const uint64_t *__restrict__ klo, const uint32_t *__restrict__ idx,
+                            uint64_t *__restrict__ out) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s < n) out[s] = klo[idx[s]];
+}
+
+// Runs of equal high keys are ordered by the low key (insertion sort, runs are short).
+__global__ void k_fix_ties(int64_t n, const uint64_t *__restrict__ khi, uint64_t *__restrict__ klo,
+                           uint32_t *__restrict__ idx) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n - 1) return;
+    if (khi[s] != khi[s + 1] || (s > 0 && khi[s - 1] == khi[s])) return;
+    int64_t e = s + 1;
+    while (e + 1 < n && khi[e + 1] == khi[s]) ++e;
+    for (int64_t a = s + 1; a <= e; ++a) {
+        uint64_t kl = klo[a];
+        uint32_t id = idx[a];
+        int64_t b = a - 1;
+        while (b >= s && (klo[b] > kl || (klo[b] == kl && idx[b] > id))) {
+            klo[b + 1] = klo[b];
+            idx[b + 1] = idx[b];
+            --b;
+        }
+        klo[b + 1] = kl;
+        idx[b + 1] = id;
+    }
+}
+
+__device__ __forceinline__ int shared_digits(uint64_t h0, uint64_t l0, uint64_t h1, uint64_t l1) {
+    if (h0 != h1) return (__clzll(h0 ^ h1) - 1) / 3;  // keys use the low 63 bits
+    if (l0 != l1) return kHiLevels + (__clzll(l0 ^ l1) - 1) / 3;
+    return kLevels;
+}
+
+// ---- 4. shared digits, node counts ----
+__global__ void k_lcp(int64_t n, const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo,
+                      const uint32_t *__restrict__ idx, const double *__restrict__ x, uint8_t *__restrict__ c,
+                      int *__restrict__ err) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    if (s == n - 1) {
+        c[s] = 0;
+        return;
+    }
+    int d = shared_digits(khi[s], klo[s], khi[s + 1], klo[s + 1]);
+    if (d >= kLevels) {
+        uint32_t a = idx[s], b = idx[s + 1];
+        bool same = x[a] == x[b] && x[n + a] == x[n + b] && x[2 * n + a] == x[2 * n + b];
+        atomicMax(err, same ? BH_ERR_INDIST : BH_ERR_DEPTH);  // indistinguishable_bodies (greg8tree.hpp:171)
+        d = kLevels - 1;
+    }
+    c[s] = (uint8_t)d;
+}
+
+__global__ void k_node_counts(int64_t n, const uint8_t *__restrict__ c, uint32_t *__restrict__ cnt) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    int prev = s > 0 ? c[s - 1] : 0;
+    cnt[s] = 1u + (uint32_t)max(0, (int)c[s] - prev);
+}
+
+// Exclusive scan in three launches (block scan, scan of block sums, add).
+__global__ __launch_bounds__(1024) void k_scan_blocks(int64_t n, uint32_t *__restrict__ a, uint32_t *__restrict__ blk) {
+    __shared__ uint32_t sm[1024];
+    const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    uint32_t v = i < n ? a[i] : 0;
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t t = threadIdx.x >= (unsigned)off ? sm[threadIdx.x - off] : 0;
+        __syncthreads();
+        sm[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < n) a[i] = sm[threadIdx.x] - v;
+    if (threadIdx.x == 1023) blk[blockIdx.x] = sm[1023];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(int nb, uint32_t *__restrict__ blk, uint32_t *__restrict__ total) {
+    __shared__ uint32_t sm[1024];
+    // sequential chunks per thread: nb may exceed 1024
+    const int per = (nb + 1023) / 1024;
+    const int b = threadIdx.x * per, e = min(nb, b + per);
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i) s += blk[i];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t t = threadIdx.x >= (unsigned)off ? sm[threadIdx.x - off] : 0;
+        __syncthreads();
+        sm[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? sm[threadIdx.x - 1] : 0;
+    for (int i = b; i < e; ++i) {
+        uint32_t v = blk[i];
+        blk[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023) *total = 1u + sm[1023];  // + root
+}
+
+__global__ __launch_bounds__(1024) void k_scan_add(int64_t n, uint32_t *__restrict__ a, const uint32_t *__restrict__ blk) {
+    const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    if (i < n) a[i] += blk[blockIdx.x];
+}
+
+// ---- sparse table of range minima over c ----
+__global__ void k_st_level(int64_t n, const uint8_t *__restrict__ prev, uint8_t *__restrict__ cur, int64_t half) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint8_t a = prev[i];
+    uint8_t b = i + half < n ? prev[i + half] : 0;
+    cur[i] = a < b ? a : b;
+}
+
+__device__ __forceinline__ unsigned digit_at(uint64_t hi, uint64_t lo, int level) {
+    return level <= kHiLevels ? (unsigned)(hi >> (3 * (kHiLevels - level))) & 7u
+                              : (unsigned)(lo >> (3 * (kLevels - level))) & 7u;
+}
+
+// ---- nodes in DFS pre-order ----
+__global__ void k_nodes(int64_t n, int logn, const uint8_t *__restrict__ st, const uint32_t *__restrict__ base,
+                        const uint32_t *__restrict__ total_p, const uint64_t *__restrict__ khi,
+                        const uint64_t *__restrict__ klo, const uint32_t *__restrict__ idx,
+                        int32_t *__restrict__ first, int32_t *__restrict__ last, int32_t *__restrict__ skip,
+                        int32_t *__restrict__ body, uint8_t *__restrict__ depth, int8_t *__restrict__ octant) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const uint8_t *c = st;  // level 0 of the sparse table
+    const int32_t total = (int32_t)*total_p;
+    const int cs = c[s];
+    const int cp = s > 0 ? c[s - 1] : 0;
+    const uint64_t h = khi[s], l = klo[s];
+    if (s == 0) {
+        first[0] = 0;
+        last[0] = (int32_t)(n - 1);
+        skip[0] = total;
+        body[0] = -1;
+        depth[0] = 0;
+        octant[0] = -1;
+    }
+    int32_t node = 1 + (int32_t)base[s];
+    for (int L = cp + 1; L <= cs; ++L, ++node) {
+        // last position: first e >= s with c[e] < L (c[n-1] = 0)
+        int64_t e = s;
+        for (int k = logn - 1; k >= 0; --k) {
+            int64_t len = (int64_t)1 << k;
+            if (e + len - 1 <= n - 1 && st[(size_t)k * n + e] >= L) e += len;
+        }
+        first[node] = (int32_t)s;
+        last[node] = (int32_t)e;
+        skip[node] = e >= n - 1 ? total : 1 + (int32_t)base[e + 1];
+        body[node] = -1;
+        depth[node] = (uint8_t)L;
+        octant[node] = (int8_t)digit_at(h, l, L);
+    }
+    const int ld = (cs > cp ? cs : cp) + 1;
+    first[node] = (int32_t)s;
+    last[node] = (int32_t)s;
+    skip[node] = node + 1;
+    body[node] = (int32_t)idx[s];
+    depth[node] = (uint8_t)ld;
+    octant[node] = (int8_t)digit_at(h, l, ld);
+}
+
+__global__ void k_children(int32_t total, const int32_t *__restrict__ skip, const int32_t *__restrict__ body,
+                           int32_t *__restrict__ parent, uint8_t *__restrict__ nchild, uint32_t *__restrict__ ready) {
+    const int32_t x = blockIdx.x * kBlock + threadIdx.x;
+    if (x >= total) return;
+    ready[x] = 0;
+    if (x == 0) parent[0] = -1;
+    if (body[x] >= 0) {
+        nchild[x] = 0;
+        return;
+    }
+    int32_t ch = x + 1, k = 0;
+    const int32_t end = skip[x];
+    while (ch < end) {
+        parent[ch] = x;
+        ch = skip[ch];
+        ++k;
+    }
+    nchild[x] = (uint8_t)k;
+}
+
+// side = top.x - btm.x of the node, replaying set_corners' recursion on x.
+__global__ void k_side(int32_t total, const double *__restrict__ box, const int32_t *__restrict__ first,
+                       const uint8_t *__restrict__ depth, const uint64_t *__restrict__ khi,
+                       const uint64_t *__restrict__ klo, double *__restrict__ side) {
+    const int32_t x = blockIdx.x * kBlock + threadIdx.x;
+    if (x >= total) return;
+    const int32_t s = first[x];
+    const uint64_t h = khi[s], l = klo[s];
+    double lo = box[0], hi = box[3];
+    const int dep = depth[x];
+    for (int L = 1; L <= dep; ++L) {
+        double mid = (lo + hi) * 0.5;
+        bool up = digit_at(h, l, L) & 1u;
+        lo = up ? mid : lo;
+        hi = up ? hi : mid;
+    }
+    side[x] = hi - lo;
+}
+
+// ---- 5. mass and COM bottom up ----
+__global__ void k_com(int64_t n, int32_t total, const uint32_t *__restrict__ base, const uint8_t *__restrict__ c,
+                      const int32_t *__restrict__ body, const int32_t *__restrict__ parent,
+                      const int32_t *__restrict__ skip, const uint8_t *__restrict__ nchild,
+                      const double *__restrict__ m, const double *__restrict__ x, double *__restrict__ mass,
+                      double *__restrict__ com, uint32_t *__restrict__ ready) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const int cs = c[s], cp = s > 0 ? c[s - 1] : 0;
+    int32_t leaf = 1 + (int32_t)base[s] + max(0, cs - cp);
+    const int32_t b = body[leaf];
+    mass[leaf] = m[b];
+    for (int k = 0; k < 3; ++k) com[(size_t)k * total + leaf] = x[k * n + b];
+    int32_t p = parent[leaf];
+    while (p >= 0) {
+        __threadfence();
+        uint32_t done = atomicAdd(&ready[p], 1u);
+        if (done != (uint32_t)nchild[p] - 1u) return;
+        __threadfence();
+        int32_t kids[8];
+        int nk = 0;
+        for (int32_t ch = p + 1; ch < skip[p] && nk < 8; ch = skip[ch]) kids[nk++] = ch;
+        double ms = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int q = nk - 1; q >= 0; --q) {  // reverse octant order (calc_com :262-266)
+            const int32_t ch = kids[q];
+            const double mc = mass[ch];
+            ms += mc;
+            sx += mc * com[ch];
+            sy += mc * com[(size_t)total + ch];
+            sz += mc * com[2 * (size_t)total + ch];
+        }
+        mass[p] = ms;
+        com[p] = sx / ms;
+        com[(size_t)total + p] = sy / ms;
+        com[2 * (size_t)total + p] = sz / ms;
+        p = parent[p];
+    }
+}
+
+__global__ void k_walk_prep(int32_t total, const double *__restrict__ mass, const double *__restrict__ com,
+                            const double *__restrict__ side, const int32_t *__restrict__ body, double G, double theta,
+                            float4 *__restrict__ f4, float *__restrict__ r2open) {
+    const int32_t x = blockIdx.x * kBlock + threadIdx.x;
+    if (x >= total) return;
+    f4[x] = make_float4((float)com[x], (float)com[(size_t)total + x], (float)com[2 * (size_t)total + x],
+                        (float)(G * mass[x]));
+    if (body[x] >= 0) r2open[x] = -1.0f;
+    else if (theta > 0) {
+        double r = side[x] / theta;
+        r2open[x] = (float)(r * r);
+    } else r2open[x] = INFINITY;
+}
+
+__global__ void k_sorted_pos(int64_t n, const uint32_t *__restrict__ idx, const double *__restrict__ x,
+                             const double *__restrict__ m, double G, float4 *__restrict__ out) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t b = idx[s];
+    out[s] = make_float4((float)x[b], (float)x[n + b], (float)x[2 * n + b], (float)(G * m[b]));
+}
+
+// ---- 6a. per-wave walk, FP32 monopoles ----
+__global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, const float4 *__restrict__ pos,
+                                                      const float4 *__restrict__ f4, const float *__restrict__ r2open,
+                                                      const int32_t *__restrict__ skip, float eps2, bool soft,
+                                                      double *__restrict__ accs,
+                                                      unsigned long long *__restrict__ icount) {
+#pragma clang fp contract(fast)
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool live = s < n;
+    const float4 p = live ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint64_t active = __ballot(live);
+    if (active == 0) return;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    unsigned cnt = 0;
+    int32_t node = 1;
+    while (node < total) {
+        node = __builtin_amdgcn_readfirstlane(node);
+        const float4 q = f4[node];
+        const float r2o = r2open[node];
+        const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        const bool acc_lane = d2 > r2o;
+        if ((__ballot(acc_lane) & active) == active) {
+            float s2 = d2 + eps2;
+            float inv = __builtin_amdgcn_rsqf(s2);
+            if (!soft) inv = s2 > 0.f ? inv : 0.f;  // own leaf at eps = 0
+            float w = q.w * inv;
+            w = w * (inv * inv);
+            ax = __builtin_fmaf(w, dx, ax);
+            ay = __builtin_fmaf(w, dy, ay);
+            az = __builtin_fmaf(w, dz, az);
+            ++cnt;
+            node = skip[node];
+        } else {
+            node = node + 1;
+        }
+    }
+    if (live) {
+        accs[s] = ax;
+        accs[n + s] = ay;
+        accs[2 * n + s] = az;
+    }
+    // cnt is wave-uniform: one atomic per wave
+    if (__lane_id() == (unsigned)__ffsll((unsigned long long)active) - 1)
+        atomicAdd(icount, (unsigned long long)cnt * (unsigned long long)__popcll(active));
+}
+
+// ---- 6b. per-lane walk with the reference's per-body MAC, FP64 ----
+__global__ __launch_bounds__(kBlock) void k_walk_exact(int64_t n, int32_t total, const uint32_t *__restrict__ idx,
+                                                       const uint32_t *__restrict__ base, const uint8_t *__restrict__ c,
+                                                       const double *__restrict__ x, const double *__restrict__ mass,
+                                                       const double *__restrict__ com, const double *__restrict__ side,
+                                                       const int32_t *__restrict__ body,
+                                                       const int32_t *__restrict__ skip, double G, double eps2,
+                                                       double theta, double *__restrict__ accs,
+                                                       unsigned long long *__restrict__ icount) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t b = idx[s];
+    const double px = x[b], py = x[n + b], pz = x[2 * n + b];
+    const int cs = c[s], cp = s > 0 ? c[s - 1] : 0;
+    const int32_t own = 1 + (int32_t)base[s] + max(0, cs - cp);
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    unsigned long long cnt = 0;
+    int32_t node = 1;
+    while (node < total) {
+        if (node == own) {
+            node = node + 1;
+            continue;
+        }
+        const double dx = com[node] - px, dy = com[(size_t)total + node] - py, dz = com[2 * (size_t)total + node] - pz;
+        bool accept = body[node] >= 0;
+        if (!accept) accept = side[node] / __builtin_sqrt(dx * dx + dy * dy + dz * dz) < theta;  // is_source_cell
+        if (accept) {
+            double s2 = dx * dx + dy * dy + dz * dz + eps2;
+            double sq = __builtin_sqrt(s2);
+            double k = G * (mass[node] / s2);
+            ax += k * (dx / sq);
+            ay += k * (dy / sq);
+            az += k * (dz / sq);
+            ++cnt;
+            node = skip[node];
+        } else {
+            node = node + 1;
+        }
+    }
+    accs[s] = ax;
+    accs[n + s] = ay;
+    accs[2 * n + s] = az;
+    atomicAdd(icount, cnt);
+}
+
+__global__ void k_unsort_acc(int64_t n, const uint32_t *__restrict__ idx, const double *__restrict__ accs,
+                             double *__restrict__ a) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t b = idx[s];
+    a[b] = accs[s];
+    a[n + b] = accs[n + s];
+    a[2 * n + b] = accs[2 * n + s];
+}
+
+inline unsigned blocks(int64_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
+
+void raise_bh_error(int e) {
+    if (e == BH_ERR_PLACEMENT)
+        throw ApiError(NB_ERR_INVALID_PLACEMENT, "Barnes-Hut: a body does not lie within the root cell (invalid_body_placement)");
+    if (e == BH_ERR_INDIST)
+        throw ApiError(NB_ERR_INDISTINGUISHABLE, "Barnes-Hut: two bodies share a position (indistinguishable_bodies)");
+    if (e == BH_ERR_DEPTH)
+        throw ApiError(NB_ERR_CAPACITY, "Barnes-Hut: two bodies share more than 42 octant levels");
+}
+
+}  // namespace
+
+// Builds the octree of the current positions into s->bh (synchronises once to
+// size the node arrays and to surface build errors).
+static void bh_build(nb_system *s) {
+    const int64_t n = s->n;
+    if (!s->bh) s->bh = new BHWork();
+    BHWork &w = *s->bh;
+    hipStream_t st = s->stream;
+    if (n < 2) throw ApiError(NB_ERR_INVALID_PLACEMENT, "Barnes-Hut: the root cell of fewer than two bodies is empty (invalid_body_placement)");
+    if (n >= (int64_t)1 << 31) throw ApiError(NB_ERR_CAPACITY, "Barnes-Hut: too many bodies");
+    ScopedTimer tm(s->prof, "bh_build", st);
+    if (w.cap_n < n) {
+        w.khi.ensure(n); w.klo.ensure(n); w.klo_tmp.ensure(n); w.idx.ensure(n);
+        w.cnt.ensure(n); w.pos32s.ensure(n); w.accs.ensure(3 * (size_t)n);
+        w.st.ensure((size_t)kMaxLog * n);
+        w.blk.ensure((size_t)(n + 1023) / 1024 + 1);
+        w.sorter.ensure(n);
+        w.cap_n = n;
+    }
+    w.part.ensure(6 * 1024); w.box.ensure(6); w.err.ensure(1); w.total.ensure(1); w.icount.ensure(1);
+    if (!w.h_total) NB_HIP(hipHostMalloc((void **)&w.h_total, sizeof(uint32_t), hipHostMallocDefault));
+    if (!w.h_err) NB_HIP(hipHostMalloc((void **)&w.h_err, sizeof(int), hipHostMallocDefault));
+
+    const int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, st, n, s->x.p, w.part.p);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, nparts, w.part.p, w.box.p);
+    NB_HIP(hipMemsetAsync(w.err.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kBlock), 0, st, n, s->x.p, w.box.p, w.khi.p, w.klo.p, w.idx.p,
+                       w.err.p);
+    w.sorter.sort(st, w.khi.p, w.idx.p, n, 3 * kHiLevels);
+    hipLaunchKernelGGL(k_gather_lo, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.klo.p, w.idx.p, w.klo_tmp.p);
+    std::swap(w.klo.p, w.klo_tmp.p);
+    hipLaunchKernelGGL(k_fix_ties, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p);
+    // sparse table level 0 is c itself
+    hipLaunchKernelGGL(k_lcp, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p, s->x.p, w.st.p,
+                       w.err.p);
+    int logn = 1;
+    while (((int64_t)1 << logn) <= n) ++logn;
+    for (int k = 1; k < logn; ++k)
+        hipLaunchKernelGGL(k_st_level, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.st.p + (size_t)(k - 1) * n,
+                           w.st.p + (size_t)k * n, (int64_t)1 << (k - 1));
+    hipLaunchKernelGGL(k_node_counts, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.st.p, w.cnt.p);
+    const int nb = (int)((n + 1023) / 1024);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nb, w.blk.p, w.total.p);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
+    NB_HIP(hipMemcpyAsync(w.h_total, w.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    NB_HIP(hipMemcpyAsync(w.h_err, w.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    NB_HIP(hipStreamSynchronize(st));
+    raise_bh_error(*w.h_err);
+    const int32_t total = (int32_t)*w.h_total;
+    if (w.cap_nodes < total) {
+        size_t t = (size_t)total + total / 4;
+        w.first.ensure(t); w.last.ensure(t); w.skip.ensure(t); w.parent.ensure(t); w.body.ensure(t);
+        w.depth.ensure(t); w.nchild.ensure(t); w.octant.ensure(t); w.mass.ensure(t); w.com.ensure(3 * t);
+        w.side.ensure(t); w.ready.ensure(t); w.f4.ensure(t); w.r2open.ensure(t);
+        w.cap_nodes = (int64_t)t;
+    }
+    w.nodes = total;
+    hipLaunchKernelGGL(k_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, logn, w.st.p, w.cnt.p, w.total.p, w.khi.p,
+                       w.klo.p, w.idx.p, w.first.p, w.last.p, w.skip.p, w.body.p, w.depth.p, w.octant.p);
+    hipLaunchKernelGGL(k_children, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.skip.p, w.body.p, w.parent.p,
+                       w.nchild.p, w.ready.p);
+    hipLaunchKernelGGL(k_side, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.box.p, w.first.p, w.depth.p,
+                       w.khi.p, w.klo.p, w.side.p);
+    // com is SoA with stride `total`
+    hipLaunchKernelGGL(k_com, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.cnt.p, w.st.p, w.body.p, w.parent.p,
+                       w.skip.p, w.nchild.p, s->m.p, s->x.p, w.mass.p, w.com.p, w.ready.p);
+    NB_HIP(hipGetLastError());
+}
+
+void bh_forces(nb_system *s, int64_t *interactions) {
+    const int64_t n = s->n;
+    bh_build(s);
+    BHWork &w = *s->bh;
+    hipStream_t st = s->stream;
+    const int32_t total = (int32_t)w.nodes;
+    NB_HIP(hipMemsetAsync(w.icount.p, 0, sizeof(unsigned long long), st));
+    {
+        ScopedTimer tm(s->prof, "bh_walk", st);
+        if (s->precision == NB_PREC_MIXED) {
+            hipLaunchKernelGGL(k_walk_prep, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.mass.p, w.com.p,
+                               w.side.p, w.body.p, s->G, s->theta, w.f4.p, w.r2open.p);
+            hipLaunchKernelGGL(k_sorted_pos, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.idx.p, s->x.p, s->m.p, s->G,
+                               w.pos32s.p);
+            hipLaunchKernelGGL(k_walk_wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.f4.p,
+                               w.r2open.p, w.skip.p, (float)(s->eps * s->eps), s->eps > 0, w.accs.p, w.icount.p);
+        } else {
+            hipLaunchKernelGGL(k_walk_exact, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.cnt.p, w.st.p,
+                               s->x.p, w.mass.p, w.com.p, w.side.p, w.body.p, w.skip.p, s->G, s->eps * s->eps,
+                               s->theta, w.accs.p, w.icount.p);
+        }
+    }
+    hipLaunchKernelGGL(k_unsort_acc, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.idx.p, w.accs.p, s->a.p);
+    NB_HIP(hipGetLastError());
+    if (interactions) {
+        unsigned long long c = 0;
+        NB_HIP(hipMemcpyAsync(&c, w.icount.p, sizeof(c), hipMemcpyDeviceToHost, st));
+        NB_HIP(hipStreamSynchronize(st));
+        *interactions = (int64_t)c;
+    }
+    s->acc_valid = true;
+}
+
+// evolve's KDK + Barnes-Hut loop body (gregsys.hpp:2914-2936): drift a full
+// step, rebuild, walk, kick a full step (velocities stay half a step ahead).
+void bh_steps(nb_system *s, uint64_t steps) {
+    for (uint64_t k = 0; k < steps; ++k) {
+        launch_drift(s->stream, s->n, s->v.p, s->x.p, s->dt);
+        bh_forces(s, nullptr);
+        launch_kick(s->stream, s->n, s->a.p, s->v.p, s->dt);
+        ++s->step_count;
+    }
+}
+
+int bh_topology(nb_system *s, int64_t cap, int32_t *depth, int32_t *octant, int64_t *body, int64_t *parent,
+                double *mass, double *com, double *side, int64_t *count) {
+    bh_build(s);
+    BHWork &w = *s->bh;
+    const int64_t t = w.nodes;
+    *count = t;
+    if (cap < t) throw ApiError(NB_ERR_CAPACITY, "nb_tree_topology: capacity smaller than the node count");
+    std::vector<uint8_t> d8((size_t)t), nc((size_t)t);
+    std::vector<int8_t> o8((size_t)t);
+    std::vector<int32_t> b32((size_t)t), p32((size_t)t);
+    std::vector<double> cm(3 * (size_t)t);
+    hipStream_t st = s->stream;
+    NB_HIP(hipMemcpyAsync(d8.data(), w.depth.p, (size_t)t, hipMemcpyDeviceToHost, st));
+    NB_HIP(hipMemcpyAsync(o8.data(), w.octant.p, (size_t)t, hipMemcpyDeviceToHost, st));
+    NB_HIP(hipMemcpyAsync(b32.data(), w.body.p, (size_t)t * 4, hipMemcpyDeviceToHost, st));
+    NB_HIP(hipMemcpyAsync(p32.data(), w.parent.p, (size_t)t * 4, hipMemcpyDeviceToHost, st));
+    if (mass) NB_HIP(hipMemcpyAsync(mass, w.mass.p, (size_t)t * 8, hipMemcpyDeviceToHost, st));
+    if (side) NB_HIP(hipMemcpyAsync(side, w.side.p, (size_t)t * 8, hipMemcpyDeviceToHost, st));
+    NB_HIP(hipMemcpyAsync(cm.data(), w.com.p, (size_t)t * 24, hipMemcpyDeviceToHost, st));
+    NB_HIP(hipStreamSynchronize(st));
+    for (int64_t k = 0; k < t; ++k) {
+        if (depth) depth[k] = d8[(size_t)k];
+        if (octant) octant[k] = o8[(size_t)k];
+        if (body) body[k] = b32[(size_t)k];
+        if (parent) parent[k] = p32[(size_t)k];
+        if (com)
+            for (int q = 0; q < 3; ++q) com[3 * k + q] = cm[(size_t)q * t + k];
+    }
+    return NB_OK;
+}
+
 }  // namespace nbe

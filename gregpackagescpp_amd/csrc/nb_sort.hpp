@@ -1,0 +1,15 @@
+// nb_sort.hpp -- device radix sort (nb_sort.hip).
+#pragma once
+#include "nb_internal.hpp"
+
+namespace nbe {
+
+struct RadixSorter {
+    DBuf<uint64_t> k2;
+    DBuf<uint32_t> v2;
+    DBuf<uint32_t> hist;
+    void ensure(int64_t n);
+    void sort(hipStream_t s, uint64_t *keys, uint32_t *vals, int64_t n, int bits);
+};
+
+}  // namespace nbe

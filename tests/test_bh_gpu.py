@@ -1,0 +1,179 @@
+"""Barnes-Hut on the GPU through the C ABI, against the oracle and the
+compiled reference's tree fixtures.
+
+Bars:
+  * topology (node count, DFS pre-order depth / octant / leaf body / parent):
+    bit-exact, up to the full 1,048,576-body two-Plummer workload;
+  * node mass and COM: bit-exact when the masses are powers of two (the
+    workload's 1/N); otherwise within 1e-13 relative, because the engine sums
+    children where the reference sums bodies in insertion order;
+  * FP64 per-lane walk (reference per-body MAC): accelerations and the
+    accepted-cell count bit-identical to the oracle for power-of-two masses;
+  * FP32 per-wave walk: error against the direct sum no worse than 1.5x the
+    reference-MAC walk's (the wave opens a superset of the cells).
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gregpackagescpp_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def system(m, pos, vel=None, prec=capi.PREC_FP64, G=1.0, eps=0.01, theta=0.5, dt=1e-3):
+    s = capi.System(0)
+    s.set_G(G)
+    s.set_softening(eps)
+    s.set_bh_opening_angle(theta)
+    s.set_timestep(dt)
+    s.set_precision(prec)
+    s.set_bodies(m, pos, np.zeros_like(pos) if vel is None else vel)
+    return s
+
+
+def check_topology(m, pos, exact_mass):
+    t = system(m, pos).tree()
+    o = O.bh_tree(m, pos)
+    assert len(t["depth"]) == len(o["depth"])
+    for k in ("depth", "octant", "body", "parent"):
+        assert np.array_equal(t[k], o[k]), k
+    assert np.array_equal(t["side"], o["side"])
+    if exact_mass:
+        assert np.array_equal(t["mass"], o["mass"])
+        assert np.array_equal(t["com"], o["com"])
+    else:
+        assert (np.abs(t["mass"] - o["mass"]) / o["mass"]).max() < 1e-13
+        assert np.abs(t["com"] - o["com"]).max() / np.abs(o["com"]).max() < 1e-13
+
+
+@pytest.mark.parametrize("n", [2, 3, 1000, 4096])
+def test_topology_plummer(n):
+    print(f"case topology plummer n={n}", flush=True)
+    m, pos, _ = capi.ic_plummer(5, n)
+    check_topology(m, pos, exact_mass=(n & (n - 1)) == 0)
+
+
+def test_topology_two_plummer_uniform_ball():
+    m, pos, _ = capi.ic_two_plummer(2, 32768)
+    check_topology(m, pos, exact_mass=True)
+    m, pos, _ = capi.ic_uniform_ball(4, 5000, 1e20, 1e22, 1e9, 1e3)
+    check_topology(m, pos, exact_mass=False)
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden",
+                                                               "ref_tree_*.npz"))), ids=os.path.basename)
+def test_topology_vs_reference_fixture(path):
+    f = np.load(path)
+    t = system(f["m"], f["pos"]).tree()
+    assert len(t["depth"]) == len(f["depth"])
+    assert np.array_equal(t["depth"], f["depth"])
+    assert np.array_equal(t["octant"], f["octant"])
+    assert np.array_equal(t["body"], f["body"])
+    n = len(f["m"])
+    if (n & (n - 1)) == 0:
+        assert np.array_equal(t["mass"], f["mass"])
+    else:
+        assert (np.abs(t["mass"] - f["mass"]) / f["mass"]).max() < 1e-13
+
+
+def test_full_size_topology():
+    """Two 524,288-body Plummer spheres (the 1M workload): node count and leaf
+    membership bit-exact against the oracle's insertion-built tree."""
+    m, pos, _ = capi.ic_two_plummer(2, 1 << 20)
+    print("case full-size topology n=1048576", flush=True)
+    t = system(m, pos).tree()
+    o = O.bh_tree(m, pos)
+    assert len(t["depth"]) == len(o["depth"])
+    for k in ("depth", "octant", "body", "parent"):
+        assert np.array_equal(t[k], o[k]), k
+    assert np.array_equal(t["mass"], o["mass"])
+
+
+def test_exact_walk_bit_identical():
+    m, pos, _ = capi.ic_two_plummer(7, 8192)
+    s = system(m, pos)
+    acc, cnt = s.bh_accelerations()
+    ref, rcnt = O.bh_acc(m, pos, 1.0, 0.01, 0.5)
+    assert cnt == rcnt
+    assert np.array_equal(acc, ref)
+
+
+def test_exact_walk_theta_zero_is_direct():
+    m, pos, _ = capi.ic_plummer(3, 512)
+    s = system(m, pos, theta=0.0)
+    acc, cnt = s.bh_accelerations()
+    assert cnt == 512 * 511
+    ref = O.direct_acc(m, pos, 1.0, 0.01, O.FORM_SOFT, dtype=np.float64)
+    assert np.abs(acc - ref).max() / np.abs(ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("n", [4096, 65536])
+def test_wave_walk_accuracy(n):
+    print(f"case wave walk n={n}", flush=True)
+    m, pos, _ = capi.ic_two_plummer(9, n)
+    s = system(m, pos, prec=capi.PREC_MIXED)
+    acc, cnt = s.bh_accelerations()
+    sample = np.arange(0, n, max(1, n // 2048))
+    d = system(m, pos)
+    exact, _ = d.bh_accelerations()
+    ref = O.direct_acc(m, pos, 1.0, 0.01, O.FORM_SOFT, dtype=np.float64)[sample] if n <= 8192 else None
+    if ref is None:
+        # direct-sum reference on the GPU FP64 path for the sample is the whole force; use a CPU pass on the sample
+        ref = direct_sample(m, pos, sample)
+    e_wave = np.linalg.norm(acc[sample] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    e_exact = np.linalg.norm(exact[sample] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    print(f"median rel err: wave {np.median(e_wave):.2e}, per-body MAC {np.median(e_exact):.2e}; "
+          f"interactions {cnt}")
+    assert np.median(e_wave) <= 1.5 * np.median(e_exact) + 1e-6
+    assert np.percentile(e_wave, 99) < 0.05
+
+
+def direct_sample(m, pos, sample, G=1.0, eps=0.01):
+    out = np.zeros((len(sample), 3))
+    for k, i in enumerate(sample):
+        d = pos - pos[i]
+        r2 = (d * d).sum(1) + eps * eps
+        w = G * m / (r2 * np.sqrt(r2))
+        w[i] = 0.0
+        out[k] = (w[:, None] * d).sum(0)
+    return out
+
+
+def test_bh_kdk_bit_identical():
+    m, pos, vel = capi.ic_two_plummer(11, 4096)
+    s = system(m, pos, vel)
+    s.set_iterations(5)
+    s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)
+    st = s.state()
+    rp, rv, ra = O.bh_kdk(m, pos, vel, 1.0, 1e-3, 0.01, 0.5, 5)
+    assert np.array_equal(st["pos"], rp)
+    assert np.array_equal(st["vel"], rv)
+
+
+def test_bh_mixed_kdk_tracks_fp64():
+    m, pos, vel = capi.ic_two_plummer(12, 16384)
+    out = []
+    for prec in (capi.PREC_MIXED, capi.PREC_FP64):
+        s = system(m, pos, vel, prec=prec)
+        s.set_iterations(10)
+        s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)
+        out.append(s.state())
+    dv = out[0]["vel"] - vel
+    dv_ref = out[1]["vel"] - vel
+    rel = np.linalg.norm(dv - dv_ref, axis=1) / np.linalg.norm(dv_ref, axis=1)
+    assert np.median(rel) < 1e-2
+
+
+def test_bh_errors():
+    s = system(np.ones(3), np.array([[0.0, 0, 0], [1.0, 1, 1], [1.0, 1, 1]]))
+    with pytest.raises(capi.NBodyError) as e:
+        s.tree()
+    assert e.value.code == capi.NB_ERR_INDISTINGUISHABLE
+    s = system(np.ones(1), np.zeros((1, 3)))
+    with pytest.raises(capi.NBodyError) as e:
+        s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)
+    assert e.value.code == capi.NB_ERR_INVALID_PLACEMENT

@@ -4,8 +4,8 @@ Bars:
   * FP64 path (NB_PREC_FP64): bit-identical to the IEEE-double restatement of
     the reference's operation order (oracle *_d), for both pair forms.
   * FP64 path vs the compiled reference's long-double KDK fixtures: positions
-    within 1e-12 relative, velocity increments within 1e-9 relative (only the
-    arithmetic type differs).
+    within 1e-12 relative, velocities within 1e-9 of the largest increment or
+    8 FP64 ulps of |v| (only the arithmetic type differs).
   * mixed path (FP32 forces, Kahan-compensated FP32 sums, FP64 state): the
     velocity increment v_T - v_0 within 1e-5 relative of the long-double
     oracle's (FP32 pair forces carry ~1e-7 relative error per term).
@@ -73,8 +73,9 @@ def test_fp64_vs_reference_fixture(path):
     scale = np.abs(f["pos"]).max()
     assert np.abs(st["pos"] - pos_T).max() / scale < 1e-12
     dv_ref = vel_T - f["vel"]
-    dv = st["vel"] - f["vel"]
-    assert np.abs(dv - dv_ref).max() / np.abs(dv_ref).max() < 1e-9
+    # velocities: 1e-9 of the largest increment, or a few FP64 ulps of |v|
+    tol = max(1e-9 * float(np.abs(dv_ref).max()), 8 * np.finfo(np.float64).eps * float(np.abs(vel_T).max()))
+    assert float(np.abs(st["vel"] - vel_T).max()) < tol
 
 
 @pytest.mark.parametrize("n,eps", [(257, 1e6), (1024, 1e6), (130, 0.0)])
