@@ -62,17 +62,113 @@ def cpu_baseline(m, pos, vel):
                       f"({steps + 1} force passes, {t:.1f} s)"}
 
 
+def hbm_copy():
+    """Plain device copy of known size (256 MiB read + 256 MiB written per
+    launch, 3 launches): the calibration run for FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    n = 64 * 1024 * 1024
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"hbm_copy_bytes_per_launch": 4 * n, "launches": 3}))
+
+
+BH_BODIES = 1 << 20
+BH_THETA = 0.5
+
+
+def bench_bh(capi, steps):
+    """Barnes-Hut KDK on two 524,288-body Plummer spheres (theta = 0.5, eps =
+    0.01, dt = 1e-3, full rebuild every step).  Interactions are I_alg: the
+    cells accepted under the reference's per-body MAC, counted by the FP64
+    per-lane walk on the same positions."""
+    m, pos, vel = capi.ic_two_plummer(2, BH_BODIES)
+    s = capi.System(0)
+    s.set_G(1.0)
+    s.set_timestep(DT)
+    s.set_softening(EPS)
+    s.set_bh_opening_angle(BH_THETA)
+    s.set_bodies(m, pos, vel)
+    s.set_precision(capi.PREC_FP64)
+    _, i_alg = s.bh_accelerations()  # reference per-body MAC count on the initial positions
+    s.set_precision(capi.PREC_MIXED)
+    s.set_iterations(1)
+    s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)  # warm-up
+    t0 = time.perf_counter()
+    s.advance(steps)
+    t = (time.perf_counter() - t0) / steps
+    s.profile(True)
+    s.advance(2)
+    b_ms, nb = s.profile_read("bh_build")
+    w_ms, nw = s.profile_read("bh_walk")
+    s.profile(False)
+    _, wave_cnt = s.bh_accelerations()
+    s.close()
+    return {"value": i_alg / t, "unit": "interactions/s", "bodies": BH_BODIES, "theta": BH_THETA,
+            "ms_per_step": t * 1e3, "interactions_per_step": i_alg, "wave_walk_interactions": wave_cnt,
+            "build_ms": b_ms / max(nb, 1), "walk_ms": w_ms / max(nw, 1), "steps": steps,
+            "workload": "BH leapfrog KDK, two Plummer spheres of 524288 at x=+-10, vrel=0.5, theta=0.5, "
+                        "eps=0.01, dt=1e-3, FP32 wave walk, FP64 state and node COM"}
+
+
+def bench_adjoint(capi):
+    """Adjoint workload: N = 4096 uniform ball (seed 4), eps = 1e6 m, dt = 10 s,
+    2000 KDK steps, checkpoints every 64.  ratio = backward wall time (segment
+    recompute + reverse sweep) / forward wall time (with checkpoint saves)."""
+    n, steps, ckpt = 4096, 2000, 64
+    m, x, v = capi.ic_uniform_ball(4, n, 1e20, 1e22, 1e9, 1e3)
+    s = capi.System(0)
+    s.set_G(6.6743e-11)
+    s.set_timestep(10.0)
+    s.set_softening(1e6)
+    s.set_precision(capi.PREC_FP64_FAST)
+    s.set_bodies(m, x, v)
+    rng = np.random.default_rng(0)
+    gx = rng.normal(size=(n, 3))
+    gv = np.zeros((n, 3))
+    s.adjoint(64, 64, gx, gv)  # warm-up (allocations, code objects)
+    s.set_bodies(m, x, v)
+    _, _, _, fns, bns = s.adjoint(steps, ckpt, gx, gv)
+    s.close()
+    return {"value": bns / fns, "unit": "backward/forward wall-time ratio", "higher_is_better": False,
+            "forward_ms": fns / 1e6, "backward_ms": bns / 1e6, "bodies": n, "steps": steps, "ckpt_every": ckpt,
+            "workload": "adjoint of direct KDK, uniform ball N=4096 (seed 4), eps=1e6 m, dt=10 s, 2000 steps, "
+                        "FP64 split kernels"}
+
+
+def cpu_bh_baseline(m, pos):
+    """Oracle (corrected Barnes-Hut restatement, FP64, one thread) on a 65,536-body sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    t0 = time.perf_counter()
+    _, cnt = O.bh_acc(m, pos, 1.0, EPS, BH_THETA)
+    t = time.perf_counter() - t0
+    return {"value": cnt / t, "unit": "interactions/s", "cores": 1, "kind": "port",
+            "sample": f"oracle insertion octree + per-body MAC walk, 1 thread, {len(m)}-body two-Plummer sample, "
+                      f"one force evaluation ({cnt} interactions, {t:.1f} s)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-bh", action="store_true")
+    ap.add_argument("--bh-steps", type=int, default=5)
+    ap.add_argument("--hbm-copy", action="store_true", help="run only the HBM copy calibration kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs one system per GPU and measures one MI355X; --gpus {args.gpus} is not "
               f"supported (multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
+    if args.hbm_copy:
+        hbm_copy()
+        return
 
     from gregpackagescpp_amd import capi
 
@@ -129,9 +225,15 @@ def main():
                      "avg_launch_ms": avg_force_s * 1e3, "update_kernel_avg_ms": upd_ms / max(launches, 1),
                      "flop_per_interaction": FLOP_PER_INTERACTION},
     }
+    s.close()
+    if not args.no_bh:
+        line["bh"] = bench_bh(capi, args.bh_steps)
+        line["adjoint"] = bench_adjoint(capi)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(m, pos, vel)
-    s.close()
+        if not args.no_bh:
+            bm, bp, _ = capi.ic_two_plummer(2, 65536)
+            line["bh"]["cpu_baseline"] = cpu_bh_baseline(bm, bp)
     print(json.dumps(line), flush=True)
 
 

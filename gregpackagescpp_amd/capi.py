@@ -38,6 +38,7 @@ NO_COLL_CHECK = 0
 OVERLAP_COLL_CHECK = 3
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_FP64_FAST = 2
 
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)

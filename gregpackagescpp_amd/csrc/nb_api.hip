@@ -128,6 +128,16 @@ void direct_forces(nb_system *s) {
         ScopedTimer t(s->prof, "update", s->stream);
         launch_update_mixed(s->stream, n, s->dw.partial.p, s->dw.nsplit, s->a.p, s->v.p, s->x.p, s->m.p, s->G,
                             s->half_dt, s->dt, UPD_STORE, s->dw.pos32.p);
+    } else if (s->precision == NB_PREC_FP64_FAST) {
+        s->dw.nsplit64 = choose_nsplit64(n);
+        s->dw.partial64.ensure((size_t)s->dw.nsplit64 * 3 * (size_t)n);
+        {
+            ScopedTimer t(s->prof, "force", s->stream);
+            launch_force_fp64_split(s->stream, n, s->x.p, s->m.p, s->eps * s->eps, form_of(s) == 1,
+                                    s->dw.partial64.p, s->dw.nsplit64);
+        }
+        launch_update_fp64_split(s->stream, n, s->dw.partial64.p, s->dw.nsplit64, s->G, s->a.p, s->v.p, s->x.p,
+                                 s->half_dt, s->dt, UPD_STORE);
     } else {
         ScopedTimer t(s->prof, "force", s->stream);
         launch_force_fp64(s->stream, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->a.p);
@@ -139,6 +149,7 @@ void direct_forces(nb_system *s) {
 // leapfrog_kdk_acc_e_and_step :514-526 and s_coll when collisions are on).
 void direct_kdk_steps(nb_system *s, uint64_t steps) {
     const bool mixed = s->precision == NB_PREC_MIXED;
+    const bool fast64 = s->precision == NB_PREC_FP64_FAST;
     const bool coll = s->coll_mode == NB_OVERLAP_COLL_CHECK;
     bool need_open = true;
     for (uint64_t st = 1; st <= steps; ++st) {
@@ -148,7 +159,18 @@ void direct_kdk_steps(nb_system *s, uint64_t steps) {
             launch_kick_drift(s->stream, n, s->a.p, s->v.p, s->x.p, s->half_dt, s->dt);
             if (mixed) launch_pack(s->stream, n, s->x.p, s->m.p, s->G, s->dw.pos32.p);
         }
-        if (mixed) {
+        if (fast64) {
+            if (s->dw.partial64.n < (size_t)s->dw.nsplit64 * 3 * (size_t)n) direct_forces(s);
+            {
+                ScopedTimer t(s->prof, "force", s->stream);
+                launch_force_fp64_split(s->stream, n, s->x.p, s->m.p, s->eps * s->eps, form_of(s) == 1,
+                                        s->dw.partial64.p, s->dw.nsplit64);
+            }
+            const bool last = st == steps || coll;
+            launch_update_fp64_split(s->stream, n, s->dw.partial64.p, s->dw.nsplit64, s->G, s->a.p, s->v.p, s->x.p,
+                                     s->half_dt, s->dt, last ? UPD_CLOSE : UPD_CLOSE_OPEN);
+            need_open = last;
+        } else if (mixed) {
             if (s->dw.pos32.n < (size_t)n) direct_forces(s);  // defensive: sizes follow n
             {
                 ScopedTimer t(s->prof, "force", s->stream);
@@ -384,7 +406,8 @@ int nb_set_collisions(nb_system *s, int mode, int mergers, double min_mom) {
 
 int nb_set_precision(nb_system *s, int p) {
     if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
-    if (p != NB_PREC_FP64 && p != NB_PREC_MIXED) return fail(s, NB_ERR_INVALID_ARGUMENT, "unknown precision");
+    if (p != NB_PREC_FP64 && p != NB_PREC_MIXED && p != NB_PREC_FP64_FAST)
+        return fail(s, NB_ERR_INVALID_ARGUMENT, "unknown precision");
     s->precision = p;
     s->acc_valid = false;
     return NB_OK;

@@ -115,14 +115,30 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(int64_t n, const double
 
 // find_min_max (greg8tree.hpp:1012-1047): longest edge with the reference's
 // tie rule, pad = L/20, btm = min - pad, L' = L + 2*pad, top = btm + L'.
-__global__ void k_bbox_final(int nparts, const double *__restrict__ part, double *__restrict__ box) {
-    if (threadIdx.x != 0) return;
-    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int b = 0; b < nparts; ++b)
+__global__ __launch_bounds__(kBlock) void k_bbox_final(int nparts, const double *__restrict__ part,
+                                                       double *__restrict__ box) {
+    __shared__ double sm[6][kBlock];
+    double lmn[3] = {INFINITY, INFINITY, INFINITY}, lmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nparts; b += kBlock)
         for (int c = 0; c < 3; ++c) {
-            mn[c] = fmin(mn[c], part[b * 6 + c]);
-            mx[c] = fmax(mx[c], part[b * 6 + 3 + c]);
+            lmn[c] = fmin(lmn[c], part[b * 6 + c]);
+            lmx[c] = fmax(lmx[c], part[b * 6 + 3 + c]);
         }
+    for (int c = 0; c < 3; ++c) {
+        sm[c][threadIdx.x] = lmn[c];
+        sm[3 + c][threadIdx.x] = lmx[c];
+    }
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int c = 0; c < 3; ++c) {
+                sm[c][threadIdx.x] = fmin(sm[c][threadIdx.x], sm[c][threadIdx.x + off]);
+                sm[3 + c][threadIdx.x] = fmax(sm[3 + c][threadIdx.x], sm[3 + c][threadIdx.x + off]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double mn[3] = {sm[0][0], sm[1][0], sm[2][0]}, mx[3] = {sm[3][0], sm[4][0], sm[5][0]};
     double xr = mx[0] - mn[0], yr = mx[1] - mn[1], zr = mx[2] - mn[2];
     double le = xr > yr ? (xr > zr ? xr : zr) : (yr > zr ? yr : zr);
     double pad = le / 20.0;
@@ -370,43 +386,41 @@ __global__ void k_side(int32_t total, const double *__restrict__ box, const int3
     side[x] = hi - lo;
 }
 
-// ---- 5. mass and COM bottom up ----
-__global__ void k_com(int64_t n, int32_t total, const uint32_t *__restrict__ base, const uint8_t *__restrict__ c,
-                      const int32_t *__restrict__ body, const int32_t *__restrict__ parent,
-                      const int32_t *__restrict__ skip, const uint8_t *__restrict__ nchild,
-                      const double *__restrict__ m, const double *__restrict__ x, double *__restrict__ mass,
-                      double *__restrict__ com, uint32_t *__restrict__ ready) {
+// ---- 5. mass and COM bottom up, one launch per depth (deepest first) ----
+__global__ void k_com_leaves(int64_t n, int32_t total, const uint32_t *__restrict__ base,
+                             const uint8_t *__restrict__ c, const int32_t *__restrict__ body,
+                             const double *__restrict__ m, const double *__restrict__ x, double *__restrict__ mass,
+                             double *__restrict__ com) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
     const int cs = c[s], cp = s > 0 ? c[s - 1] : 0;
-    int32_t leaf = 1 + (int32_t)base[s] + max(0, cs - cp);
+    const int32_t leaf = 1 + (int32_t)base[s] + max(0, cs - cp);
     const int32_t b = body[leaf];
     mass[leaf] = m[b];
     for (int k = 0; k < 3; ++k) com[(size_t)k * total + leaf] = x[k * n + b];
-    int32_t p = parent[leaf];
-    while (p >= 0) {
-        __threadfence();
-        uint32_t done = atomicAdd(&ready[p], 1u);
-        if (done != (uint32_t)nchild[p] - 1u) return;
-        __threadfence();
-        int32_t kids[8];
-        int nk = 0;
-        for (int32_t ch = p + 1; ch < skip[p] && nk < 8; ch = skip[ch]) kids[nk++] = ch;
-        double ms = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-        for (int q = nk - 1; q >= 0; --q) {  // reverse octant order (calc_com :262-266)
-            const int32_t ch = kids[q];
-            const double mc = mass[ch];
-            ms += mc;
-            sx += mc * com[ch];
-            sy += mc * com[(size_t)total + ch];
-            sz += mc * com[2 * (size_t)total + ch];
-        }
-        mass[p] = ms;
-        com[p] = sx / ms;
-        com[(size_t)total + p] = sy / ms;
-        com[2 * (size_t)total + p] = sz / ms;
-        p = parent[p];
+}
+
+__global__ void k_com_level(int32_t total, int level, const uint8_t *__restrict__ depth,
+                            const int32_t *__restrict__ body, const int32_t *__restrict__ skip,
+                            double *__restrict__ mass, double *__restrict__ com) {
+    const int32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= total || depth[p] != level || body[p] >= 0) return;
+    int32_t kids[8];
+    int nk = 0;
+    for (int32_t ch = p + 1; ch < skip[p] && nk < 8; ch = skip[ch]) kids[nk++] = ch;
+    double ms = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int q = nk - 1; q >= 0; --q) {  // reverse octant order (calc_com :262-266)
+        const int32_t ch = kids[q];
+        const double mc = mass[ch];
+        ms += mc;
+        sx += mc * com[ch];
+        sy += mc * com[(size_t)total + ch];
+        sz += mc * com[2 * (size_t)total + ch];
     }
+    mass[p] = ms;
+    com[p] = sx / ms;
+    com[(size_t)total + p] = sy / ms;
+    com[2 * (size_t)total + p] = sz / ms;
 }
 
 __global__ void k_walk_prep(int32_t total, const double *__restrict__ mass, const double *__restrict__ com,
@@ -570,7 +584,7 @@ static void bh_build(nb_system *s) {
 
     const int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, st, n, s->x.p, w.part.p);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, nparts, w.part.p, w.box.p);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, st, nparts, w.part.p, w.box.p);
     NB_HIP(hipMemsetAsync(w.err.p, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kBlock), 0, st, n, s->x.p, w.box.p, w.khi.p, w.klo.p, w.idx.p,
                        w.err.p);
@@ -611,8 +625,11 @@ static void bh_build(nb_system *s) {
     hipLaunchKernelGGL(k_side, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.box.p, w.first.p, w.depth.p,
                        w.khi.p, w.klo.p, w.side.p);
     // com is SoA with stride `total`
-    hipLaunchKernelGGL(k_com, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.cnt.p, w.st.p, w.body.p, w.parent.p,
-                       w.skip.p, w.nchild.p, s->m.p, s->x.p, w.mass.p, w.com.p, w.ready.p);
+    hipLaunchKernelGGL(k_com_leaves, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.cnt.p, w.st.p, w.body.p,
+                       s->m.p, s->x.p, w.mass.p, w.com.p);
+    for (int L = kLevels - 1; L >= 0; --L)
+        hipLaunchKernelGGL(k_com_level, dim3(blocks(total)), dim3(kBlock), 0, st, total, L, w.depth.p, w.body.p,
+                           w.skip.p, w.mass.p, w.com.p);
     NB_HIP(hipGetLastError());
 }
 

@@ -16,6 +16,8 @@
 //  * k_force_fp64: FP64, one lane per i, j ascending, FP contraction off, the
 //    exact operation order of the reference's pairwise update (so results are
 //    bit-identical to an IEEE-double restatement of it).
+#include <algorithm>
+
 #include "nb_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -175,6 +177,74 @@ __global__ __launch_bounds__(kBlock) void k_force_fp64(int64_t n, const double *
     }
 }
 
+// FP64 reciprocal square root: v_rsq_f64 seed and two Newton steps
+// y <- y (3 - s y^2)/2 (each step doubles the correct bits).
+__device__ __forceinline__ double rsqrt64(double s) {
+    double y = __builtin_amdgcn_rsq(s);
+    double h = 0.5 * s;
+    y = y * __builtin_fma(-h * y, y, 1.5);
+    y = y * __builtin_fma(-h * y, y, 1.5);
+    return y;
+}
+
+// FP64 pair forces with the j range split over gridDim.y (fills the GPU at
+// small N).  Partials [split][3][n] are combined in fixed split order, so the
+// result is deterministic (but not the reference's summation order).
+template <bool SOFT>
+__global__ __launch_bounds__(kBlock) void k_force_fp64_split(int64_t n, const double *__restrict__ x,
+                                                             const double *__restrict__ m, int jchunk, double eps2,
+                                                             double *__restrict__ partial) {
+#pragma clang fp contract(fast)
+    const int split = blockIdx.y;
+    const int64_t j0 = (int64_t)split * jchunk;
+    const int64_t j1 = min(n, j0 + jchunk);
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t ii = i < n ? i : 0;
+    const double xi = x[ii], yi = x[n + ii], zi = x[2 * n + ii];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int64_t j = j0; j < j1; ++j) {
+        const double dx = x[j] - xi, dy = x[n + j] - yi, dz = x[2 * n + j] - zi;
+        const double s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
+        double inv = rsqrt64(s);
+        if (!SOFT) inv = s > 0.0 ? inv : 0.0;
+        const double w = m[j] * (inv * inv * inv);
+        ax = __builtin_fma(w, dx, ax);
+        ay = __builtin_fma(w, dy, ay);
+        az = __builtin_fma(w, dz, az);
+    }
+    if (i < n) {
+        double *pp = partial + (size_t)split * 3 * n + i;
+        pp[0] = ax;
+        pp[n] = ay;
+        pp[2 * n] = az;
+    }
+}
+
+__global__ void k_update_fp64_split(int64_t n, const double *__restrict__ partial, int nsplit, double G,
+                                    double *__restrict__ a, double *__restrict__ v, double *__restrict__ x,
+                                    double half_dt, double dt, int mode) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double s = 0.0;
+        for (int q = 0; q < nsplit; ++q) s += partial[(size_t)q * 3 * n + (size_t)c * n + i];
+        acc[c] = G * s;
+        a[c * n + i] = acc[c];
+    }
+    if (mode == UPD_STORE) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double vv = v[c * n + i] + half_dt * acc[c];
+        if (mode == UPD_CLOSE_OPEN) {
+            vv += half_dt * acc[c];
+            x[c * n + i] += dt * vv;
+        }
+        v[c * n + i] = vv;
+    }
+}
+
 // Per-body potential energy, reference order (cumulative_pe :358-363 then the
 // per-body halving of calc_energy :620): the pair term -(G*m_lo*m_hi)/r uses
 // the lower index's mass first, as the reference's b1/b2 roles do.
@@ -274,6 +344,34 @@ void launch_force_fp64(hipStream_t s, int64_t n, const double *x, const double *
         hipLaunchKernelGGL(k_force_fp64<0>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, x, m, G, eps2, a);
     else
         hipLaunchKernelGGL(k_force_fp64<1>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, x, m, G, eps2, a);
+    NB_HIP(hipGetLastError());
+}
+
+int choose_nsplit64(int64_t n) {
+    int64_t wgs = (n + kBlock - 1) / kBlock;
+    int64_t sp = (2048 + wgs - 1) / wgs;
+    sp = std::max<int64_t>(1, std::min<int64_t>(sp, 128));
+    while (sp > 1 && n / sp < 32) --sp;
+    return (int)sp;
+}
+
+void launch_force_fp64_split(hipStream_t s, int64_t n, const double *x, const double *m, double eps2, bool soft,
+                             double *partial, int nsplit) {
+    if (n <= 0) return;
+    const int jchunk = (int)((n + nsplit - 1) / nsplit);
+    dim3 grid(blocks_for(n), (unsigned)nsplit);
+    if (soft)
+        hipLaunchKernelGGL(k_force_fp64_split<true>, grid, dim3(kBlock), 0, s, n, x, m, jchunk, eps2, partial);
+    else
+        hipLaunchKernelGGL(k_force_fp64_split<false>, grid, dim3(kBlock), 0, s, n, x, m, jchunk, eps2, partial);
+    NB_HIP(hipGetLastError());
+}
+
+void launch_update_fp64_split(hipStream_t s, int64_t n, const double *partial, int nsplit, double G, double *a,
+                              double *v, double *x, double half_dt, double dt, int mode) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_update_fp64_split, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, partial, nsplit, G, a, v, x,
+                       half_dt, dt, mode);
     NB_HIP(hipGetLastError());
 }
 

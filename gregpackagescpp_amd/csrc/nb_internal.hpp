@@ -94,6 +94,8 @@ struct DirectWork {
     DBuf<float4> pos32;    // x, y, z, G*m in FP32
     DBuf<float> partial;   // [split][6][n]: Kahan sum and compensation per component
     int nsplit = 1;
+    DBuf<double> partial64;  // [split][3][n] for the FP64 split kernels
+    int nsplit64 = 1;
 };
 
 // Launchers (stream-ordered, no sync).
@@ -113,6 +115,12 @@ void launch_drift(hipStream_t s, int64_t n, const double *v, double *x, double d
 void launch_pe_fp64(hipStream_t s, int64_t n, const double *x, const double *m, double G, double eps2, int form,
                     double *pe);
 int choose_nsplit(int64_t n);
+// FP64 j-split forces (deterministic split order, not the reference order)
+int choose_nsplit64(int64_t n);
+void launch_force_fp64_split(hipStream_t s, int64_t n, const double *x, const double *m, double eps2, bool soft,
+                             double *partial, int nsplit);
+void launch_update_fp64_split(hipStream_t s, int64_t n, const double *partial, int nsplit, double G, double *a,
+                              double *v, double *x, double half_dt, double dt, int mode);
 
 // update modes for launch_update_mixed
 enum { UPD_STORE = 0, UPD_CLOSE = 1, UPD_CLOSE_OPEN = 2 };

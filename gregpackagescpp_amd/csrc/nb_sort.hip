@@ -32,29 +32,39 @@ __global__ __launch_bounds__(kThreads) void k_hist(const uint64_t *__restrict__ 
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of m counts by one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan(uint32_t *__restrict__ hist, int m) {
-    __shared__ uint32_t part[1024];
+// Block-wide inclusive scan of one value per thread (256 threads).
+__device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t *sm) {
     const int t = threadIdx.x;
-    const int per = (m + 1023) / 1024;
-    const int b = t * per;
-    const int e = min(m, b + per);
-    uint32_t s = 0;
-    for (int i = b; i < e; ++i) s += hist[i];
-    part[t] = s;
+    sm[t] = v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = t >= off ? part[t - off] : 0;
+    for (int off = 1; off < kThreads; off <<= 1) {
+        uint32_t u = t >= off ? sm[t - off] : 0;
         __syncthreads();
-        part[t] += v;
+        sm[t] += u;
         __syncthreads();
     }
-    uint32_t run = t ? part[t - 1] : 0;
-    for (int i = b; i < e; ++i) {
-        uint32_t c = hist[i];
-        hist[i] = run;
-        run += c;
+    uint32_t r = sm[t];
+    __syncthreads();
+    return r;
+}
+
+// One workgroup per digit: exclusive scan of that digit's per-tile counts
+// (in place) and the digit's total.
+__global__ __launch_bounds__(kThreads) void k_tile_scan(uint32_t *__restrict__ hist, int ntiles,
+                                                        uint32_t *__restrict__ dtot) {
+    __shared__ uint32_t sm[kThreads];
+    uint32_t *row = hist + (size_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (int b = 0; b < ntiles; b += kThreads) {
+        const int i = b + threadIdx.x;
+        const uint32_t v = i < ntiles ? row[i] : 0;
+        const uint32_t inc = block_incl_scan(v, sm);
+        const uint32_t chunk = sm[kThreads - 1];
+        if (i < ntiles) row[i] = carry + inc - v;
+        carry += chunk;
+        __syncthreads();  // sm is rewritten by the next chunk
     }
+    if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -64,13 +74,17 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 
 __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                       uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, int n,
-                                                      int shift, const uint32_t *__restrict__ hist, int ntiles) {
+                                                      int shift, const uint32_t *__restrict__ hist, int ntiles,
+                                                      const uint32_t *__restrict__ dtot) {
     __shared__ uint32_t gbase[256];
     __shared__ uint32_t running[256];
     __shared__ uint32_t wcnt[kWaves][256];
     const int t = threadIdx.x;
     const int wave = t >> 6;
-    gbase[t] = hist[t * ntiles + blockIdx.x];
+    // digit bases: exclusive scan of the digit totals
+    const uint32_t dt = dtot[t];
+    const uint32_t dinc = block_incl_scan(dt, running);
+    gbase[t] = dinc - dt + hist[t * ntiles + blockIdx.x];
     running[t] = 0;
     for (int w = 0; w < kWaves; ++w) wcnt[w][t] = 0;
     __syncthreads();
@@ -115,6 +129,7 @@ void RadixSorter::ensure(int64_t n) {
     k2.ensure((size_t)std::max<int64_t>(n, 1));
     v2.ensure((size_t)std::max<int64_t>(n, 1));
     hist.ensure((size_t)256 * std::max(nt, 1));
+    dtot.ensure(256);
 }
 
 // Sorts keys/vals in place (the result lands back in the caller's buffers).
@@ -123,13 +138,13 @@ void RadixSorter::sort(hipStream_t s, uint64_t *keys, uint32_t *vals, int64_t n,
     if (n <= 1) return;
     ensure(n);
     const int nt = (int)((n + kTileItems - 1) / kTileItems);
-    const int m = 256 * nt;
     uint64_t *ka = keys, *kb = k2.p;
     uint32_t *va = vals, *vb = v2.p;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_hist, dim3(nt), dim3(kThreads), 0, s, ka, (int)n, shift, hist.p, nt);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, hist.p, m);
-        hipLaunchKernelGGL(k_scatter, dim3(nt), dim3(kThreads), 0, s, ka, va, kb, vb, (int)n, shift, hist.p, nt);
+        hipLaunchKernelGGL(k_tile_scan, dim3(256), dim3(kThreads), 0, s, hist.p, nt, dtot.p);
+        hipLaunchKernelGGL(k_scatter, dim3(nt), dim3(kThreads), 0, s, ka, va, kb, vb, (int)n, shift, hist.p, nt,
+                           dtot.p);
         std::swap(ka, kb);
         std::swap(va, vb);
     }

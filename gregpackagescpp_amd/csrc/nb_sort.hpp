@@ -7,7 +7,8 @@ namespace nbe {
 struct RadixSorter {
     DBuf<uint64_t> k2;
     DBuf<uint32_t> v2;
-    DBuf<uint32_t> hist;
+    DBuf<uint32_t> hist;  // [digit][tile] counts, then in-digit tile offsets
+    DBuf<uint32_t> dtot;  // per-digit totals
     void ensure(int64_t n);
     void sort(hipStream_t s, uint64_t *keys, uint32_t *vals, int64_t n, int bits);
 };

@@ -57,8 +57,9 @@ extern "C" {
 #define NB_OVERLAP_COLL_CHECK 3
 
 /* ---- arithmetic of the pairwise force (an engine choice, not in the reference) ---- */
-#define NB_PREC_FP64 0   /* FP64 forces in the reference's operation order       */
-#define NB_PREC_MIXED 1  /* FP32 pair forces, Kahan-compensated FP32 sums, FP64 state */
+#define NB_PREC_FP64 0       /* FP64 forces in the reference's operation order (parity) */
+#define NB_PREC_MIXED 1      /* FP32 pair forces, Kahan-compensated FP32 sums, FP64 state */
+#define NB_PREC_FP64_FAST 2  /* FP64 forces, j range split over the GPU, fixed combine order */
 
 typedef struct nb_system nb_system;
 
