@@ -44,9 +44,29 @@ namespace {
 template <typename T> inline T t_sqrt(T v);
 template <> inline long double t_sqrt<long double>(long double v) { return sqrtl(v); }
 template <> inline double t_sqrt<double>(double v) { return std::sqrt(v); }
+/* Cube root of the merged radius (gregbod.hpp:936, cbrtl).  The long-double
+ * instantiation calls cbrtl as the reference does.  The double instantiation
+ * uses a fixed Newton iteration built from IEEE basic operations only
+ * (a = f 2^e with e a multiple of 3, y <- y - (y^3 - f)/(3 y^2) seven times
+ * from y = 1), so that it is reproducible bit for bit on any IEEE machine:
+ * the GPU's FP64 collision pass uses the same iteration. */
+inline double newton_cbrt(double a) {
+    if (!(a > 0) || !std::isfinite(a)) return std::cbrt(a);
+    int e;
+    double f = std::frexp(a, &e);
+    int r = ((e % 3) + 3) % 3;
+    f = std::ldexp(f, r);
+    e -= r;
+    double y = 1.0;
+    for (int k = 0; k < 7; ++k) {
+        double y2 = y * y;
+        y = y - (y2 * y - f) / (3.0 * y2);
+    }
+    return std::ldexp(y, e / 3);
+}
 template <typename T> inline T t_cbrt(T v);
 template <> inline long double t_cbrt<long double>(long double v) { return cbrtl(v); }
-template <> inline double t_cbrt<double>(double v) { return std::cbrt(v); }
+template <> inline double t_cbrt<double>(double v) { return newton_cbrt(v); }
 
 enum { FORM_NEWTON = 0, FORM_SOFT = 1 };
 
