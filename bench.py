@@ -29,6 +29,8 @@ N_BODIES = 65536
 EPS, DT = 0.01, 1e-3
 FLOP_PER_INTERACTION = 20
 PEAK_FP32_TFLOPS = 157.3
+PROFILED_FORCE_TRAFFIC = {"bodies": 65536, "ipl": 2, "waves": 16, "fetch_kib": 7709.8, "write_kib": 49152.0,
+                          "fetch_factor": 2.0, "write_factor": 1.0}
 
 
 def cpu_baseline(m, pos, vel):
@@ -95,6 +97,7 @@ def bench_bh(capi, steps):
     s.set_precision(capi.PREC_FP64)
     _, i_alg = s.bh_accelerations()  # reference per-body MAC count on the initial positions
     s.set_precision(capi.PREC_MIXED)
+    s.autotune(capi.LEAPFROG_KDK | capi.BARNES_HUT)  # project autotuner: bucket size of the wave walk
     s.set_iterations(1)
     s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)  # warm-up
     t0 = time.perf_counter()
@@ -106,9 +109,11 @@ def bench_bh(capi, steps):
     w_ms, nw = s.profile_read("bh_walk")
     s.profile(False)
     _, wave_cnt = s.bh_accelerations()
+    bucket = s.tuning("bh_bucket")
     s.close()
     return {"value": i_alg / t, "unit": "interactions/s", "bodies": BH_BODIES, "theta": BH_THETA,
             "ms_per_step": t * 1e3, "interactions_per_step": i_alg, "wave_walk_interactions": wave_cnt,
+            "tuned": {"bh_bucket": bucket},
             "build_ms": b_ms / max(nb, 1), "walk_ms": w_ms / max(nw, 1), "steps": steps,
             "workload": "BH leapfrog KDK, two Plummer spheres of 524288 at x=+-10, vrel=0.5, theta=0.5, "
                         "eps=0.01, dt=1e-3, FP32 wave walk, FP64 state and node COM"}
@@ -152,6 +157,32 @@ def cpu_bh_baseline(m, pos):
                       f"one force evaluation ({cnt} interactions, {t:.1f} s)"}
 
 
+def profile_run(capi, args):
+    """Fixed-configuration runs for counter collection (no autotuning, no HIP
+    events): the kernels seen are exactly those of the timed step."""
+    s = capi.System(0)
+    s.set_G(1.0)
+    s.set_timestep(DT)
+    s.set_softening(EPS)
+    s.set_precision(capi.PREC_MIXED)
+    if args.profile_direct:
+        m, pos, vel = capi.ic_plummer(1, N_BODIES)
+        s.set_bodies(m, pos, vel)
+        s.set_tuning("direct_ipl", 2)
+        s.set_tuning("direct_waves", 16)
+        s.set_iterations(args.steps)
+        s.evolve(capi.LEAPFROG_KDK)
+    else:
+        m, pos, vel = capi.ic_two_plummer(2, BH_BODIES)
+        s.set_bodies(m, pos, vel)
+        s.set_bh_opening_angle(BH_THETA)
+        s.set_tuning("bh_bucket", 32)
+        s.set_iterations(args.bh_steps)
+        s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)
+    s.close()
+    print("profile run done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +192,10 @@ def main():
     ap.add_argument("--no-bh", action="store_true")
     ap.add_argument("--bh-steps", type=int, default=5)
     ap.add_argument("--hbm-copy", action="store_true", help="run only the HBM copy calibration kernel")
+    ap.add_argument("--profile-direct", action="store_true",
+                    help="for rocprofv3: the 65,536-body mixed direct step with fixed tuning, no events, no JSON")
+    ap.add_argument("--profile-bh", action="store_true",
+                    help="for rocprofv3: the 1M-body BH step (mixed, bucket 32), no events, no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs one system per GPU and measures one MI355X; --gpus {args.gpus} is not "
@@ -172,6 +207,10 @@ def main():
 
     from gregpackagescpp_amd import capi
 
+    if args.profile_direct or args.profile_bh:
+        profile_run(capi, args)
+        return
+
     n = N_BODIES
     m, pos, vel = capi.ic_plummer(1, n)
     s = capi.System(0)
@@ -180,6 +219,8 @@ def main():
     s.set_softening(EPS)
     s.set_precision(capi.PREC_MIXED)
     s.set_bodies(m, pos, vel)
+    s.autotune(capi.LEAPFROG_KDK)  # project autotuner: i-bodies per lane, j-split width
+    tuned = {"direct_ipl": s.tuning("direct_ipl"), "direct_waves": s.tuning("direct_waves")}
     s.set_iterations(max(1, args.warmup))
     s.evolve(capi.LEAPFROG_KDK)  # warm-up: a(x0) + W steps
 
@@ -199,11 +240,14 @@ def main():
     avg_force_s = force_ms / launches * 1e-3
     achieved = FLOP_PER_INTERACTION * n * (n - 1) / avg_force_s / 1e12
 
+    # HBM bytes per launch of k_force_mixed from rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # (profiles/k_force_mixed_plummer65536.md), each scaled by what the counter
+    # reads for a copy kernel of known size; valid for the configuration it was
+    # measured at (its write traffic scales with the j-split count).
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "force_mixed_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    prof = PROFILED_FORCE_TRAFFIC
+    if n == prof["bodies"] and tuned["direct_ipl"] == prof["ipl"] and tuned["direct_waves"] == prof["waves"]:
+        traffic = (prof["fetch_kib"] * prof["fetch_factor"] + prof["write_kib"] * prof["write_factor"]) * 1024
 
     line = {
         "metric": "interactions/s at N=64K direct & 1M BH; adjoint/forward time ratio",
@@ -224,6 +268,7 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "avg_launch_ms": avg_force_s * 1e3, "update_kernel_avg_ms": upd_ms / max(launches, 1),
                      "flop_per_interaction": FLOP_PER_INTERACTION},
+        "tuned": tuned,
     }
     s.close()
     if not args.no_bh:

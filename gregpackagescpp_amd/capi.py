@@ -70,6 +70,9 @@ SIGNATURES = {
     "nb_tree_topology": (C.c_int, [_P, C.c_int64, _I32, _I32, _I64, _I64, _D, _D, _D, _I64]),
     "nb_bh_accelerations": (C.c_int, [_P, _D, _I64]),
     "nb_adjoint": (C.c_int, [_P, C.c_uint64, C.c_uint64, _D, _D, _D, _D, _D, _I64, _I64]),
+    "nb_set_tuning": (C.c_int, [_P, C.c_char_p, C.c_int64]),
+    "nb_get_tuning": (C.c_int, [_P, C.c_char_p, _I64]),
+    "nb_autotune": (C.c_int, [_P, C.c_int, C.c_char_p]),
     "nb_profile_enable": (C.c_int, [_P, C.c_int]),
     "nb_profile_read": (C.c_int, [_P, C.c_char_p, _D, _I64]),
     "nb_batch_create": (C.c_int, [C.c_int, C.c_int64, _I64, _D, _D, _D, _D, _D, C.POINTER(_P)]),
@@ -282,6 +285,17 @@ class System:
         self._ck(self._L.nb_adjoint(self.h, steps, ckpt, dp(gxT), dp(gvT), dp(gx0), dp(gv0), dp(gm), C.byref(f),
                                     C.byref(b)))
         return gx0, gv0, gm, f.value, b.value
+
+    def set_tuning(self, key, value):
+        self._ck(self._L.nb_set_tuning(self.h, key.encode(), int(value)))
+
+    def tuning(self, key):
+        v = C.c_int64()
+        self._ck(self._L.nb_get_tuning(self.h, key.encode(), C.byref(v)))
+        return v.value
+
+    def autotune(self, method=LEAPFROG_KDK, cache_path=None):
+        self._ck(self._L.nb_autotune(self.h, method, None if cache_path is None else cache_path.encode()))
 
     def profile(self, on=True):
         self._ck(self._L.nb_profile_enable(self.h, int(on)))

@@ -23,13 +23,15 @@
 //     (first e >= s with c[e] < L) comes from a sparse-table range minimum;
 //     its skip pointer (next node outside its subtree) is the first node
 //     starting at e + 1;
-//  5. node mass and COM bottom up in FP64 (a leaf climbs while it is the last
-//     child to finish), children summed in reverse octant order as calc_com;
+//  5. node mass and COM bottom up in FP64, one launch per depth from the
+//     deepest level up (leaves first), children summed in reverse octant
+//     order as calc_com;
 //  6. the walk is stackless over (node, skip): descend = node + 1, accept or
 //     skip = skip[node].  Two walks: per lane with the reference's per-body
 //     MAC in FP64 (parity), and per wave with FP32 monopoles where a node is
 //     opened for all 64 lanes when any lane needs it (node data is
-//     wave-uniform, so it arrives through scalar loads).
+//     wave-uniform, so it arrives through scalar loads); a rejected node with
+//     at most `bucket` bodies is summed body by body instead of descended.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -68,7 +70,8 @@ struct BHWork {
     DBuf<double> mass, com, side;
     DBuf<uint32_t> ready;
     DBuf<float4> f4;        // com (FP32) and G*mass
-    DBuf<float> r2open;     // (side/theta)^2, -1 for leaves
+    DBuf<int4> i4;          // skip, first, count, r^2_open bits (walk record)
+    int bucket = 32;        // rejected nodes with at most this many bodies are summed directly
     DBuf<float4> pos32s;    // sorted body positions (FP32) and G*m
     DBuf<double> accs;      // sorted accelerations, SoA
     DBuf<unsigned long long> icount;
@@ -423,18 +426,24 @@ __global__ void k_com_level(int32_t total, int level, const uint8_t *__restrict_
     com[2 * (size_t)total + p] = sz / ms;
 }
 
+// Walk record per node: f4 = (com, G*mass) in FP32 and i4 = (skip, first
+// sorted body, body count, r^2_open as float bits; -1 for leaves).
 __global__ void k_walk_prep(int32_t total, const double *__restrict__ mass, const double *__restrict__ com,
-                            const double *__restrict__ side, const int32_t *__restrict__ body, double G, double theta,
-                            float4 *__restrict__ f4, float *__restrict__ r2open) {
+                            const double *__restrict__ side, const int32_t *__restrict__ body,
+                            const int32_t *__restrict__ skip, const int32_t *__restrict__ first,
+                            const int32_t *__restrict__ last, double G, double theta, float4 *__restrict__ f4,
+                            int4 *__restrict__ i4) {
     const int32_t x = blockIdx.x * kBlock + threadIdx.x;
     if (x >= total) return;
     f4[x] = make_float4((float)com[x], (float)com[(size_t)total + x], (float)com[2 * (size_t)total + x],
                         (float)(G * mass[x]));
-    if (body[x] >= 0) r2open[x] = -1.0f;
+    float r2o;
+    if (body[x] >= 0) r2o = -1.0f;
     else if (theta > 0) {
         double r = side[x] / theta;
-        r2open[x] = (float)(r * r);
-    } else r2open[x] = INFINITY;
+        r2o = (float)(r * r);
+    } else r2o = INFINITY;
+    i4[x] = make_int4(skip[x], first[x], last[x] - first[x] + 1, __float_as_int(r2o));
 }
 
 __global__ void k_sorted_pos(int64_t n, const uint32_t *__restrict__ idx, const double *__restrict__ x,
@@ -446,10 +455,28 @@ __global__ void k_sorted_pos(int64_t n, const uint32_t *__restrict__ idx, const 
 }
 
 // ---- 6a. per-wave walk, FP32 monopoles ----
+// The wave shares one node sequence.  A node is accepted when every lane's MAC
+// holds; a rejected node with at most `bucket` bodies is not descended but
+// summed body by body (its bodies are contiguous in sorted order), which
+// replaces up to 2*bucket-1 dependent node visits by a streamed loop (the
+// autotuner picks the bucket; 1 disables it).
+__device__ __forceinline__ void monopole(float4 q, float4 p, float eps2, bool soft, float &ax, float &ay,
+                                         float &az) {
+    const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+    const float s2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+    float inv = __builtin_amdgcn_rsqf(s2);
+    if (!soft) inv = s2 > 0.f ? inv : 0.f;  // own body at eps = 0
+    float w = q.w * inv;
+    w = w * (inv * inv);
+    ax = __builtin_fmaf(w, dx, ax);
+    ay = __builtin_fmaf(w, dy, ay);
+    az = __builtin_fmaf(w, dz, az);
+}
+
+template <bool SOFT>
 __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, const float4 *__restrict__ pos,
-                                                      const float4 *__restrict__ f4, const float *__restrict__ r2open,
-                                                      const int32_t *__restrict__ skip, float eps2, bool soft,
-                                                      double *__restrict__ accs,
+                                                      const float4 *__restrict__ f4, const int4 *__restrict__ i4,
+                                                      int bucket, float eps2, double *__restrict__ accs,
                                                       unsigned long long *__restrict__ icount) {
 #pragma clang fp contract(fast)
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -463,21 +490,18 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
     while (node < total) {
         node = __builtin_amdgcn_readfirstlane(node);
         const float4 q = f4[node];
-        const float r2o = r2open[node];
+        const int4 r = i4[node];
         const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
         const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        const bool acc_lane = d2 > r2o;
+        const bool acc_lane = d2 > __int_as_float(r.w);
         if ((__ballot(acc_lane) & active) == active) {
-            float s2 = d2 + eps2;
-            float inv = __builtin_amdgcn_rsqf(s2);
-            if (!soft) inv = s2 > 0.f ? inv : 0.f;  // own leaf at eps = 0
-            float w = q.w * inv;
-            w = w * (inv * inv);
-            ax = __builtin_fmaf(w, dx, ax);
-            ay = __builtin_fmaf(w, dy, ay);
-            az = __builtin_fmaf(w, dz, az);
+            monopole(q, p, eps2, SOFT, ax, ay, az);
             ++cnt;
-            node = skip[node];
+            node = r.x;
+        } else if (r.z <= bucket) {
+            for (int j = r.y; j < r.y + r.z; ++j) monopole(pos[j], p, eps2, SOFT, ax, ay, az);
+            cnt += (unsigned)r.z;
+            node = r.x;
         } else {
             node = node + 1;
         }
@@ -614,7 +638,7 @@ static void bh_build(nb_system *s) {
         size_t t = (size_t)total + total / 4;
         w.first.ensure(t); w.last.ensure(t); w.skip.ensure(t); w.parent.ensure(t); w.body.ensure(t);
         w.depth.ensure(t); w.nchild.ensure(t); w.octant.ensure(t); w.mass.ensure(t); w.com.ensure(3 * t);
-        w.side.ensure(t); w.ready.ensure(t); w.f4.ensure(t); w.r2open.ensure(t);
+        w.side.ensure(t); w.ready.ensure(t); w.f4.ensure(t); w.i4.ensure(t);
         w.cap_nodes = (int64_t)t;
     }
     w.nodes = total;
@@ -644,11 +668,16 @@ void bh_forces(nb_system *s, int64_t *interactions) {
         ScopedTimer tm(s->prof, "bh_walk", st);
         if (s->precision == NB_PREC_MIXED) {
             hipLaunchKernelGGL(k_walk_prep, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.mass.p, w.com.p,
-                               w.side.p, w.body.p, s->G, s->theta, w.f4.p, w.r2open.p);
+                               w.side.p, w.body.p, w.skip.p, w.first.p, w.last.p, s->G, s->theta, w.f4.p, w.i4.p);
             hipLaunchKernelGGL(k_sorted_pos, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.idx.p, s->x.p, s->m.p, s->G,
                                w.pos32s.p);
-            hipLaunchKernelGGL(k_walk_wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.f4.p,
-                               w.r2open.p, w.skip.p, (float)(s->eps * s->eps), s->eps > 0, w.accs.p, w.icount.p);
+            const int bucket = s->tune_bh_bucket > 0 ? s->tune_bh_bucket : w.bucket;
+            if (s->eps > 0)
+                hipLaunchKernelGGL(k_walk_wave<true>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p,
+                                   w.f4.p, w.i4.p, bucket, (float)(s->eps * s->eps), w.accs.p, w.icount.p);
+            else
+                hipLaunchKernelGGL(k_walk_wave<false>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p,
+                                   w.f4.p, w.i4.p, bucket, 0.0f, w.accs.p, w.icount.p);
         } else {
             hipLaunchKernelGGL(k_walk_exact, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.cnt.p, w.st.p,
                                s->x.p, w.mass.p, w.com.p, w.side.p, w.body.p, w.skip.p, s->G, s->eps * s->eps,

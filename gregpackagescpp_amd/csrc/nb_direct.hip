@@ -294,17 +294,16 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBl
 
 }  // namespace
 
-int choose_nsplit(int64_t n) {
-    // Aim for ~8 waves per SIMD on 256 CUs (4 SIMDs each) at IPL = 2:
-    // waves = ceil(n / 512) * 4 * nsplit.
-    int64_t wgs = (n + 2 * kBlock - 1) / (2 * kBlock);
-    int64_t want = 2048;  // workgroups
-    int64_t s = (want + wgs - 1) / wgs;
-    if (s < 1) s = 1;
-    if (s > 64) s = 64;
-    // keep at least one full Kahan tile of j per split
-    while (s > 1 && n / s < kTile) --s;
-    return (int)s;
+int choose_nsplit(int64_t n, int ipl, int waves_per_simd) {
+    // waves in flight = ceil(n / (256 ipl)) * 4 * nsplit; 256 CUs x 4 SIMDs
+    if (ipl <= 0) ipl = 2;
+    if (waves_per_simd <= 0) waves_per_simd = 8;
+    const int64_t wgs = (n + (int64_t)kBlock * ipl - 1) / ((int64_t)kBlock * ipl);
+    const int64_t want = (int64_t)waves_per_simd * 1024 / 4;  // workgroups of 4 waves
+    int64_t sp = (want + wgs - 1) / wgs;
+    sp = std::max<int64_t>(1, std::min<int64_t>(sp, 64));
+    while (sp > 1 && n / sp < kTile) --sp;  // at least one full Kahan tile of j per split
+    return (int)sp;
 }
 
 void launch_pack(hipStream_t s, int64_t n, const double *x, const double *m, double G, float4 *pos32) {
@@ -313,10 +312,9 @@ void launch_pack(hipStream_t s, int64_t n, const double *x, const double *m, dou
     NB_HIP(hipGetLastError());
 }
 
-void launch_force_mixed(hipStream_t s, int64_t n, const float4 *pos32, float eps2, bool soft, float *partial,
-                        int nsplit) {
-    if (n <= 0) return;
-    constexpr int IPL = 2;
+template <int IPL>
+static void launch_force_mixed_ipl(hipStream_t s, int64_t n, const float4 *pos32, float eps2, bool soft,
+                                   float *partial, int nsplit) {
     int jchunk = (int)((n + nsplit - 1) / nsplit);
     jchunk = (jchunk + kTile - 1) / kTile * kTile;
     dim3 grid((unsigned)((n + kBlock * IPL - 1) / (kBlock * IPL)), (unsigned)nsplit);
@@ -326,6 +324,14 @@ void launch_force_mixed(hipStream_t s, int64_t n, const float4 *pos32, float eps
     else
         hipLaunchKernelGGL((k_force_mixed<IPL, false>), grid, dim3(kBlock), 0, s, pos32, (int)n, jchunk, eps2,
                            partial);
+}
+
+void launch_force_mixed(hipStream_t s, int64_t n, const float4 *pos32, float eps2, bool soft, float *partial,
+                        int nsplit, int ipl) {
+    if (n <= 0) return;
+    if (ipl == 1) launch_force_mixed_ipl<1>(s, n, pos32, eps2, soft, partial, nsplit);
+    else if (ipl == 4) launch_force_mixed_ipl<4>(s, n, pos32, eps2, soft, partial, nsplit);
+    else launch_force_mixed_ipl<2>(s, n, pos32, eps2, soft, partial, nsplit);
     NB_HIP(hipGetLastError());
 }
 

@@ -94,6 +94,7 @@ struct DirectWork {
     DBuf<float4> pos32;    // x, y, z, G*m in FP32
     DBuf<float> partial;   // [split][6][n]: Kahan sum and compensation per component
     int nsplit = 1;
+    int ipl = 2;           // i-bodies per lane
     DBuf<double> partial64;  // [split][3][n] for the FP64 split kernels
     int nsplit64 = 1;
 };
@@ -101,7 +102,7 @@ struct DirectWork {
 // Launchers (stream-ordered, no sync).
 void launch_pack(hipStream_t s, int64_t n, const double *x, const double *m, double G, float4 *pos32);
 void launch_force_mixed(hipStream_t s, int64_t n, const float4 *pos32, float eps2, bool soft, float *partial,
-                        int nsplit);
+                        int nsplit, int ipl);
 // Reduce the split partials to FP64 accelerations, then the closing half kick
 // (v += half_dt*a); when `next` the next step's opening half kick and drift
 // (v += half_dt*a; x += dt*v) and the FP32 repack follow in the same pass.
@@ -114,7 +115,7 @@ void launch_kick_drift(hipStream_t s, int64_t n, const double *a, double *v, dou
 void launch_drift(hipStream_t s, int64_t n, const double *v, double *x, double dt);
 void launch_pe_fp64(hipStream_t s, int64_t n, const double *x, const double *m, double G, double eps2, int form,
                     double *pe);
-int choose_nsplit(int64_t n);
+int choose_nsplit(int64_t n, int ipl, int waves_per_simd);
 // FP64 j-split forces (deterministic split order, not the reference order)
 int choose_nsplit64(int64_t n);
 void launch_force_fp64_split(hipStream_t s, int64_t n, const double *x, const double *m, double eps2, bool soft,

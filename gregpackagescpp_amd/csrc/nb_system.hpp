@@ -37,6 +37,11 @@ struct nb_system {
     uint64_t step_count = 0;
     std::vector<int64_t> merge_log;  // 4 per record
 
+    // tuning knobs (0 = engine default); set by nb_set_tuning / nb_autotune
+    int tune_bh_bucket = 0;
+    int tune_direct_ipl = 0;
+    int tune_direct_waves = 0;  // target waves per SIMD for the direct split
+
     nbe::DirectWork dw;
     nbe::BHWork *bh = nullptr;
     nbe::CollWork *coll = nullptr;

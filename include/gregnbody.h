@@ -141,6 +141,19 @@ int nb_bh_accelerations(nb_system *sys, double *acc, int64_t *interactions);
 int nb_adjoint(nb_system *sys, uint64_t steps, uint64_t ckpt_every, const double *dl_dx_T, const double *dl_dv_T,
                double *dl_dx0, double *dl_dv0, double *dl_dm, int64_t *fwd_ns, int64_t *bwd_ns);
 
+/* Tuning knobs (engine extension; 0 restores the default):
+ *   "direct_ipl"   i-bodies per lane of the FP32 direct kernel (1, 2, 4)
+ *   "direct_waves" target waves per SIMD for the direct kernel's j split
+ *   "bh_bucket"    rejected tree nodes with at most this many bodies are summed
+ *                  body by body in the wave walk */
+int nb_set_tuning(nb_system *sys, const char *key, int64_t value);
+int nb_get_tuning(const nb_system *sys, const char *key, int64_t *value);
+/* Autotuner (engine extension): times the candidate settings of the knobs
+ * above on the current bodies with HIP events and keeps the fastest.  With a
+ * cache_path, results are read from / appended to that text file, keyed by
+ * (kernel, body count, precision). */
+int nb_autotune(nb_system *sys, int method, const char *cache_path);
+
 /* Kernel timing (engine extension): when enabled, every launch of the named
  * kernel class is bracketed by HIP events on the context's stream.
  * names: "force", "update", "energy", "bh_build", "bh_walk", "adjoint". */

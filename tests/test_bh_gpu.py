@@ -177,3 +177,31 @@ def test_bh_errors():
     with pytest.raises(capi.NBodyError) as e:
         s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)
     assert e.value.code == capi.NB_ERR_INVALID_PLACEMENT
+
+
+@pytest.mark.parametrize("bucket", [1, 8, 32])
+def test_wave_walk_bucket_sizes(bucket):
+    """Summing small rejected nodes body by body (bucket > 1) only adds exact
+    pair terms: the error against the direct sum does not grow."""
+    n = 16384
+    m, pos, _ = capi.ic_two_plummer(13, n)
+    s = system(m, pos, prec=capi.PREC_MIXED)
+    s.set_tuning("bh_bucket", bucket)
+    acc, cnt = s.bh_accelerations()
+    sample = np.arange(0, n, 16)
+    ref = direct_sample(m, pos, sample)
+    err = np.linalg.norm(acc[sample] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    exact, _ = system(m, pos).bh_accelerations()
+    e_exact = np.linalg.norm(exact[sample] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    print(f"bucket {bucket}: median err {np.median(err):.2e} (per-body MAC {np.median(e_exact):.2e}), "
+          f"interactions {cnt}")
+    assert np.median(err) <= 1.5 * np.median(e_exact) + 1e-6
+
+
+def test_bh_autotune():
+    m, pos, vel = capi.ic_two_plummer(14, 65536)
+    s = system(m, pos, vel, prec=capi.PREC_MIXED)
+    s.autotune(capi.LEAPFROG_KDK | capi.BARNES_HUT)
+    assert s.tuning("bh_bucket") in (1, 8, 16, 32, 64)
+    s.set_iterations(2)
+    s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)

@@ -172,3 +172,27 @@ def test_error_paths():
         s.set_units("M1:0,D1:1,T1:1")
     s.set_units("M1:1,D1:1,T1:1")
     assert s.G == float(LD(66743) / LD(10 ** 15))
+
+
+def test_tuning_knobs_keep_results():
+    """Every (i-bodies per lane, waves per SIMD) setting of the FP32 kernel and
+    the autotuner's pick give the same dynamics to FP32 summation accuracy."""
+    m, pos, vel = capi.ic_plummer(2, 8192)
+    ref = None
+    for ipl in (1, 2, 4):
+        for waves in (4, 16):
+            s = make_system(m, pos, vel, 1e-3, 5, 0.01, capi.PREC_MIXED, G=1.0)
+            s.set_tuning("direct_ipl", ipl)
+            s.set_tuning("direct_waves", waves)
+            s.evolve()
+            dv = s.state()["vel"] - vel
+            if ref is None:
+                ref = dv
+            assert np.abs(dv - ref).max() / np.abs(ref).max() < 1e-6, (ipl, waves)
+    s = make_system(m, pos, vel, 1e-3, 5, 0.01, capi.PREC_MIXED, G=1.0)
+    s.autotune(capi.LEAPFROG_KDK)
+    assert s.tuning("direct_ipl") in (1, 2, 4) and s.tuning("direct_waves") in (4, 8, 16)
+    s.evolve()
+    assert np.abs((s.state()["vel"] - vel) - ref).max() / np.abs(ref).max() < 1e-6
+    with pytest.raises(capi.NBodyError):
+        s.set_tuning("direct_ipl", 3)
