@@ -20,6 +20,7 @@
 #include <chrono>
 #include <vector>
 
+#include "nb_split64.hpp"
 #include "nb_system.hpp"
 
 namespace nbe {
@@ -27,26 +28,28 @@ namespace nbe {
 namespace {
 constexpr int kBlock = 256;
 
-__device__ __forceinline__ double rsqrt64(double s) {
-    double y = __builtin_amdgcn_rsq(s);
-    double h = 0.5 * s;
-    y = y * __builtin_fma(-h * y, y, 1.5);
-    y = y * __builtin_fma(-h * y, y, 1.5);
-    return y;
-}
-
+// One VJP pair pass: for the 64 k-bodies of a workgroup (one per lane) the
+// split's j range is divided over its kWaves64 waves; the waves' sums are
+// combined in LDS in wave order and one partial per (split, k) is written,
+// [split][4][n] = (J^T u)_xyz and (dA/dm)^T u.
 template <bool SOFT>
-__global__ __launch_bounds__(kBlock) void k_vjp_split(int64_t n, const double *__restrict__ x,
-                                                      const double *__restrict__ m, const double *__restrict__ u,
-                                                      int jchunk, double eps2, double *__restrict__ partial) {
-    const int split = blockIdx.y;
-    const int64_t j0 = (int64_t)split * jchunk;
-    const int64_t j1 = min(n, j0 + jchunk);
-    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(64 * kWaves64) void k_vjp_split(int64_t n, const double *__restrict__ x,
+                                                             const double *__restrict__ m,
+                                                             const double *__restrict__ u, int jchunk, double eps2,
+                                                             double *__restrict__ partial) {
+    __shared__ double red[kWaves64 - 1][4][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t c0 = (int64_t)blockIdx.y * jchunk;
+    const int64_t c1 = min(n, c0 + jchunk);
+    const int64_t wq = (jchunk + kWaves64 - 1) / kWaves64;
+    const int64_t j0 = c0 + wave * wq;
+    const int64_t j1 = min(c1, j0 + wq);
+    const int64_t k = (int64_t)blockIdx.x * 64 + lane;
     const int64_t kk = k < n ? k : 0;
     const double xk = x[kk], yk = x[n + kk], zk = x[2 * n + kk], mk = m[kk];
     const double ukx = u[kk], uky = u[n + kk], ukz = u[2 * n + kk];
-    double gx = 0.0, gy = 0.0, gz = 0.0, gm = 0.0;
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
     for (int64_t j = j0; j < j1; ++j) {
         const double dx = x[j] - xk, dy = x[n + j] - yk, dz = x[2 * n + j] - zk;
         const double mj = m[j];
@@ -62,29 +65,36 @@ __global__ __launch_bounds__(kBlock) void k_vjp_split(int64_t n, const double *_
         const double wz = __builtin_fma(mk, ujz, -mj * ukz);
         const double dw = __builtin_fma(dz, wz, __builtin_fma(dy, wy, dx * wx));
         const double t = inv5x3 * dw;
-        gx = __builtin_fma(inv3, wx, __builtin_fma(-t, dx, gx));
-        gy = __builtin_fma(inv3, wy, __builtin_fma(-t, dy, gy));
-        gz = __builtin_fma(inv3, wz, __builtin_fma(-t, dz, gz));
+        g[0] = __builtin_fma(inv3, wx, __builtin_fma(-t, dx, g[0]));
+        g[1] = __builtin_fma(inv3, wy, __builtin_fma(-t, dy, g[1]));
+        g[2] = __builtin_fma(inv3, wz, __builtin_fma(-t, dz, g[2]));
         const double ud = __builtin_fma(ujz, dz, __builtin_fma(ujy, dy, ujx * dx));
-        gm = __builtin_fma(-inv3, ud, gm);
+        g[3] = __builtin_fma(-inv3, ud, g[3]);
     }
-    if (k < n) {
-        double *pp = partial + (size_t)split * 4 * n + k;
-        pp[0] = gx;
-        pp[n] = gy;
-        pp[2 * n] = gz;
-        pp[3 * n] = gm;
+    if (wave > 0)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) red[wave - 1][c][lane] = g[c];
+    __syncthreads();
+    if (wave == 0 && k < n) {
+#pragma unroll
+        for (int w = 0; w < kWaves64 - 1; ++w)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) g[c] += red[w][c][lane];
+        double *pp = partial + (size_t)blockIdx.y * 4 * n + k;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) pp[(size_t)c * n] = g[c];
     }
 }
 
 // lx += scale * sum(partial xyz) ; lm += scale * sum(partial m), fixed split order
-__global__ void k_vjp_apply(int64_t n, const double *__restrict__ partial, int nsplit, double scale,
-                            double *__restrict__ lx, double *__restrict__ lm) {
-    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (k >= n) return;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int q = 0; q < nsplit; ++q)
-        for (int c = 0; c < 4; ++c) acc[c] += partial[(size_t)q * 4 * n + (size_t)c * n + k];
+__global__ __launch_bounds__(64 * kWaves64) void k_vjp_apply(int64_t n, const double *__restrict__ partial,
+                                                             int nsplit, double scale, double *__restrict__ lx,
+                                                             double *__restrict__ lm) {
+    __shared__ double red[kWaves64 - 1][4][64];
+    const int64_t k = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    double acc[4];
+    reduce_splits<4>(partial, n, nsplit, k, acc, red);
+    if (threadIdx.x >= 64 || k >= n) return;
     for (int c = 0; c < 3; ++c) lx[c * n + k] += scale * acc[c];
     lm[k] += scale * acc[3];
 }
@@ -140,7 +150,7 @@ void adjoint_run(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT,
     const int nsplit = choose_nsplit64(n);
     w.vpart.ensure((size_t)nsplit * 4 * (size_t)n);
     const int jchunk = (int)((n + nsplit - 1) / nsplit);
-    const dim3 vgrid(blocks(n), (unsigned)nsplit);
+    const dim3 vgrid((unsigned)((n + 63) / 64), (unsigned)nsplit);
     const double h = s->half_dt, dt = s->dt, eps2 = s->eps * s->eps;
     const bool soft = s->eps > 0;
 
@@ -179,13 +189,13 @@ void adjoint_run(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT,
     auto vjp = [&](const double *xs, double coef) {
         ScopedTimer tm(s->prof, "adjoint", st);
         if (soft)
-            hipLaunchKernelGGL(k_vjp_split<true>, vgrid, dim3(kBlock), 0, st, n, xs, s->m.p, w.lv.p, jchunk, eps2,
-                               w.vpart.p);
+            hipLaunchKernelGGL(k_vjp_split<true>, vgrid, dim3(64 * kWaves64), 0, st, n, xs, s->m.p, w.lv.p, jchunk,
+                               eps2, w.vpart.p);
         else
-            hipLaunchKernelGGL(k_vjp_split<false>, vgrid, dim3(kBlock), 0, st, n, xs, s->m.p, w.lv.p, jchunk, eps2,
-                               w.vpart.p);
-        hipLaunchKernelGGL(k_vjp_apply, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.vpart.p, nsplit, coef * s->G,
-                           w.lx.p, w.lm.p);
+            hipLaunchKernelGGL(k_vjp_split<false>, vgrid, dim3(64 * kWaves64), 0, st, n, xs, s->m.p, w.lv.p, jchunk,
+                               eps2, w.vpart.p);
+        hipLaunchKernelGGL(k_vjp_apply, dim3((unsigned)((n + 63) / 64)), dim3(64 * kWaves64), 0, st, n, w.vpart.p,
+                           nsplit, coef * s->G, w.lx.p, w.lm.p);
     };
 
     // ---- reverse sweep, segment by segment ----

@@ -118,13 +118,14 @@ void direct_forces(nb_system *s) {
     if (s->precision == NB_PREC_MIXED) {
         s->dw.pos32.ensure((size_t)n);
         s->dw.ipl = s->tune_direct_ipl > 0 ? s->tune_direct_ipl : 2;
+        s->dw.variant = s->tune_direct_kernel == 1 ? 0 : 1;
         s->dw.nsplit = choose_nsplit(n, s->dw.ipl, s->tune_direct_waves);
         s->dw.partial.ensure((size_t)s->dw.nsplit * 6 * (size_t)n);
         launch_pack(s->stream, n, s->x.p, s->m.p, s->G, s->dw.pos32.p);
         {
             ScopedTimer t(s->prof, "force", s->stream);
             launch_force_mixed(s->stream, n, s->dw.pos32.p, (float)(s->eps * s->eps), form_of(s) == 1,
-                               s->dw.partial.p, s->dw.nsplit, s->dw.ipl);
+                               s->dw.partial.p, s->dw.nsplit, s->dw.ipl, s->dw.variant);
         }
         ScopedTimer t(s->prof, "update", s->stream);
         launch_update_mixed(s->stream, n, s->dw.partial.p, s->dw.nsplit, s->a.p, s->v.p, s->x.p, s->m.p, s->G,
@@ -176,7 +177,7 @@ void direct_kdk_steps(nb_system *s, uint64_t steps) {
             {
                 ScopedTimer t(s->prof, "force", s->stream);
                 launch_force_mixed(s->stream, n, s->dw.pos32.p, (float)(s->eps * s->eps), form_of(s) == 1,
-                                   s->dw.partial.p, s->dw.nsplit, s->dw.ipl);
+                                   s->dw.partial.p, s->dw.nsplit, s->dw.ipl, s->dw.variant);
             }
             const bool last = st == steps || coll;
             ScopedTimer t(s->prof, "update", s->stream);

@@ -19,6 +19,7 @@
 #include <algorithm>
 
 #include "nb_internal.hpp"
+#include "nb_split64.hpp"
 
 #pragma clang fp contract(off)
 
@@ -88,6 +89,85 @@ __global__ __launch_bounds__(kBlock) void k_force_mixed(const float4 *__restrict
             kahan_add(sx[k], cx[k], px[k]);
             kahan_add(sy[k], cy[k], py[k]);
             kahan_add(sz[k], cz[k], pz[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        int i = ibase + k * kBlock;
+        if (i < n) {
+            float *pp = partial + (size_t)split * 6 * n + i;
+            pp[0] = sx[k];
+            pp[(size_t)n] = sy[k];
+            pp[2 * (size_t)n] = sz[k];
+            pp[3 * (size_t)n] = cx[k];
+            pp[4 * (size_t)n] = cy[k];
+            pp[5 * (size_t)n] = cz[k];
+        }
+    }
+}
+
+// The same pair sum with the j stream staged through LDS: each workgroup
+// copies kBlock j-bodies at a time into LDS (one coalesced float4 per lane)
+// and every wave reads them back with broadcast ds_read_b128, so all operands
+// of the 13 pair ops are VGPRs.  On gfx950 FP32 VALU ops whose operands are
+// all VGPRs issue at up to twice the rate of ops that take an SGPR operand,
+// which the scalar-load form above pays on 5 of its 13 ops (3 sub, the
+// eps^2 fma, the mass multiply).  The tile and Kahan-fold boundaries are those
+// of k_force_mixed, so both kernels give identical sums.
+template <int IPL, bool SOFT>
+__global__ __launch_bounds__(kBlock) void k_force_mixed_lds(const float4 *__restrict__ pos, int n, int jchunk,
+                                                            float eps2, float *__restrict__ partial) {
+#pragma clang fp contract(fast)
+    __shared__ float4 tile[kBlock];
+    const int split = blockIdx.y;
+    const int j0 = split * jchunk;
+    const int j1 = min(n, j0 + jchunk);
+    const int ibase = blockIdx.x * (kBlock * IPL) + threadIdx.x;
+    float e2 = eps2;
+    asm volatile("" : "+v"(e2));  // keep eps^2 in a VGPR
+    float xi[IPL], yi[IPL], zi[IPL];
+    float sx[IPL], sy[IPL], sz[IPL], cx[IPL], cy[IPL], cz[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        int i = ibase + k * kBlock;
+        float4 p = i < n ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z;
+        sx[k] = sy[k] = sz[k] = cx[k] = cy[k] = cz[k] = 0.f;
+    }
+    for (int jb = j0; jb < j1; jb += kBlock) {
+        const int cnt = min(kBlock, j1 - jb);
+        __syncthreads();  // the previous tile has been read by every wave
+        if ((int)threadIdx.x < cnt) tile[threadIdx.x] = pos[jb + threadIdx.x];
+        __syncthreads();
+        for (int t = 0; t < cnt; t += kTile) {
+            const int te = min(cnt, t + kTile);
+            float px[IPL], py[IPL], pz[IPL];
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) px[k] = py[k] = pz[k] = 0.f;
+#pragma unroll 8
+            for (int j = t; j < te; ++j) {
+                const float4 pj = tile[j];
+#pragma unroll
+                for (int k = 0; k < IPL; ++k) {
+                    float dx = pj.x - xi[k];
+                    float dy = pj.y - yi[k];
+                    float dz = pj.z - zi[k];
+                    float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, e2)));
+                    float inv = __builtin_amdgcn_rsqf(r2);
+                    if (!SOFT) inv = r2 > 0.f ? inv : 0.f;  // self pair at eps = 0
+                    float w = pj.w * inv;
+                    w = w * (inv * inv);
+                    px[k] = __builtin_fmaf(w, dx, px[k]);
+                    py[k] = __builtin_fmaf(w, dy, py[k]);
+                    pz[k] = __builtin_fmaf(w, dz, pz[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) {
+                kahan_add(sx[k], cx[k], px[k]);
+                kahan_add(sy[k], cy[k], py[k]);
+                kahan_add(sz[k], cz[k], pz[k]);
+            }
         }
     }
 #pragma unroll
@@ -177,28 +257,26 @@ __global__ __launch_bounds__(kBlock) void k_force_fp64(int64_t n, const double *
     }
 }
 
-// FP64 reciprocal square root: v_rsq_f64 seed and two Newton steps
-// y <- y (3 - s y^2)/2 (each step doubles the correct bits).
-__device__ __forceinline__ double rsqrt64(double s) {
-    double y = __builtin_amdgcn_rsq(s);
-    double h = 0.5 * s;
-    y = y * __builtin_fma(-h * y, y, 1.5);
-    y = y * __builtin_fma(-h * y, y, 1.5);
-    return y;
-}
-
-// FP64 pair forces with the j range split over gridDim.y (fills the GPU at
-// small N).  Partials [split][3][n] are combined in fixed split order, so the
-// result is deterministic (but not the reference's summation order).
+// FP64 pair forces with the j range split twice: over gridDim.y, and inside a
+// workgroup over its kWaves64 waves, which share the same 64 i-bodies (one per
+// lane).  The waves' sums are combined in LDS in wave order and one partial
+// per (split, i) is written, [split][3][n]; splits are combined in fixed order
+// by k_update_fp64_split, so the result is deterministic (but not the
+// reference's summation order).
 template <bool SOFT>
-__global__ __launch_bounds__(kBlock) void k_force_fp64_split(int64_t n, const double *__restrict__ x,
-                                                             const double *__restrict__ m, int jchunk, double eps2,
-                                                             double *__restrict__ partial) {
+__global__ __launch_bounds__(64 * kWaves64) void k_force_fp64_split(int64_t n, const double *__restrict__ x,
+                                                                    const double *__restrict__ m, int jchunk,
+                                                                    double eps2, double *__restrict__ partial) {
 #pragma clang fp contract(fast)
-    const int split = blockIdx.y;
-    const int64_t j0 = (int64_t)split * jchunk;
-    const int64_t j1 = min(n, j0 + jchunk);
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    __shared__ double red[kWaves64 - 1][3][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t c0 = (int64_t)blockIdx.y * jchunk;
+    const int64_t c1 = min(n, c0 + jchunk);
+    const int64_t wq = (jchunk + kWaves64 - 1) / kWaves64;
+    const int64_t j0 = c0 + wave * wq;
+    const int64_t j1 = min(c1, j0 + wq);
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
     const int64_t ii = i < n ? i : 0;
     const double xi = x[ii], yi = x[n + ii], zi = x[2 * n + ii];
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -212,25 +290,38 @@ __global__ __launch_bounds__(kBlock) void k_force_fp64_split(int64_t n, const do
         ay = __builtin_fma(w, dy, ay);
         az = __builtin_fma(w, dz, az);
     }
-    if (i < n) {
-        double *pp = partial + (size_t)split * 3 * n + i;
+    if (wave > 0) {
+        red[wave - 1][0][lane] = ax;
+        red[wave - 1][1][lane] = ay;
+        red[wave - 1][2][lane] = az;
+    }
+    __syncthreads();
+    if (wave == 0 && i < n) {
+#pragma unroll
+        for (int w = 0; w < kWaves64 - 1; ++w) {
+            ax += red[w][0][lane];
+            ay += red[w][1][lane];
+            az += red[w][2][lane];
+        }
+        double *pp = partial + (size_t)blockIdx.y * 3 * n + i;
         pp[0] = ax;
         pp[n] = ay;
         pp[2 * n] = az;
     }
 }
 
-__global__ void k_update_fp64_split(int64_t n, const double *__restrict__ partial, int nsplit, double G,
-                                    double *__restrict__ a, double *__restrict__ v, double *__restrict__ x,
-                                    double half_dt, double dt, int mode) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ __launch_bounds__(64 * kWaves64) void k_update_fp64_split(int64_t n, const double *__restrict__ partial,
+                                                                     int nsplit, double G, double *__restrict__ a,
+                                                                     double *__restrict__ v, double *__restrict__ x,
+                                                                     double half_dt, double dt, int mode) {
+    __shared__ double red[kWaves64 - 1][3][64];
+    const int64_t i = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
     double acc[3];
+    reduce_splits<3>(partial, n, nsplit, i, acc, red);
+    if (threadIdx.x >= 64 || i >= n) return;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        double s = 0.0;
-        for (int q = 0; q < nsplit; ++q) s += partial[(size_t)q * 3 * n + (size_t)c * n + i];
-        acc[c] = G * s;
+        acc[c] = G * acc[c];
         a[c * n + i] = acc[c];
     }
     if (mode == UPD_STORE) return;
@@ -314,24 +405,32 @@ void launch_pack(hipStream_t s, int64_t n, const double *x, const double *m, dou
 
 template <int IPL>
 static void launch_force_mixed_ipl(hipStream_t s, int64_t n, const float4 *pos32, float eps2, bool soft,
-                                   float *partial, int nsplit) {
+                                   float *partial, int nsplit, int variant) {
     int jchunk = (int)((n + nsplit - 1) / nsplit);
     jchunk = (jchunk + kTile - 1) / kTile * kTile;
     dim3 grid((unsigned)((n + kBlock * IPL - 1) / (kBlock * IPL)), (unsigned)nsplit);
-    if (soft)
+    if (variant == 1) {
+        if (soft)
+            hipLaunchKernelGGL((k_force_mixed_lds<IPL, true>), grid, dim3(kBlock), 0, s, pos32, (int)n, jchunk,
+                               eps2, partial);
+        else
+            hipLaunchKernelGGL((k_force_mixed_lds<IPL, false>), grid, dim3(kBlock), 0, s, pos32, (int)n, jchunk,
+                               eps2, partial);
+    } else if (soft) {
         hipLaunchKernelGGL((k_force_mixed<IPL, true>), grid, dim3(kBlock), 0, s, pos32, (int)n, jchunk, eps2,
                            partial);
-    else
+    } else {
         hipLaunchKernelGGL((k_force_mixed<IPL, false>), grid, dim3(kBlock), 0, s, pos32, (int)n, jchunk, eps2,
                            partial);
+    }
 }
 
 void launch_force_mixed(hipStream_t s, int64_t n, const float4 *pos32, float eps2, bool soft, float *partial,
-                        int nsplit, int ipl) {
+                        int nsplit, int ipl, int variant) {
     if (n <= 0) return;
-    if (ipl == 1) launch_force_mixed_ipl<1>(s, n, pos32, eps2, soft, partial, nsplit);
-    else if (ipl == 4) launch_force_mixed_ipl<4>(s, n, pos32, eps2, soft, partial, nsplit);
-    else launch_force_mixed_ipl<2>(s, n, pos32, eps2, soft, partial, nsplit);
+    if (ipl == 1) launch_force_mixed_ipl<1>(s, n, pos32, eps2, soft, partial, nsplit, variant);
+    else if (ipl == 4) launch_force_mixed_ipl<4>(s, n, pos32, eps2, soft, partial, nsplit, variant);
+    else launch_force_mixed_ipl<2>(s, n, pos32, eps2, soft, partial, nsplit, variant);
     NB_HIP(hipGetLastError());
 }
 
@@ -353,11 +452,13 @@ void launch_force_fp64(hipStream_t s, int64_t n, const double *x, const double *
     NB_HIP(hipGetLastError());
 }
 
+// Splits of the j range for the FP64 split kernels: enough workgroups (64
+// i-bodies x kWaves64 waves each) to fill the GPU, with at least 16 j per wave.
 int choose_nsplit64(int64_t n) {
-    int64_t wgs = (n + kBlock - 1) / kBlock;
-    int64_t sp = (2048 + wgs - 1) / wgs;
-    sp = std::max<int64_t>(1, std::min<int64_t>(sp, 128));
-    while (sp > 1 && n / sp < 32) --sp;
+    const int64_t groups = (n + 63) / 64;
+    int64_t sp = (2048 + groups - 1) / groups;
+    sp = std::max<int64_t>(1, std::min<int64_t>(sp, 64));
+    while (sp > 1 && n / (sp * kWaves64) < 16) --sp;
     return (int)sp;
 }
 
@@ -365,19 +466,21 @@ void launch_force_fp64_split(hipStream_t s, int64_t n, const double *x, const do
                              double *partial, int nsplit) {
     if (n <= 0) return;
     const int jchunk = (int)((n + nsplit - 1) / nsplit);
-    dim3 grid(blocks_for(n), (unsigned)nsplit);
+    dim3 grid((unsigned)((n + 63) / 64), (unsigned)nsplit);
     if (soft)
-        hipLaunchKernelGGL(k_force_fp64_split<true>, grid, dim3(kBlock), 0, s, n, x, m, jchunk, eps2, partial);
+        hipLaunchKernelGGL(k_force_fp64_split<true>, grid, dim3(64 * kWaves64), 0, s, n, x, m, jchunk, eps2,
+                           partial);
     else
-        hipLaunchKernelGGL(k_force_fp64_split<false>, grid, dim3(kBlock), 0, s, n, x, m, jchunk, eps2, partial);
+        hipLaunchKernelGGL(k_force_fp64_split<false>, grid, dim3(64 * kWaves64), 0, s, n, x, m, jchunk, eps2,
+                           partial);
     NB_HIP(hipGetLastError());
 }
 
 void launch_update_fp64_split(hipStream_t s, int64_t n, const double *partial, int nsplit, double G, double *a,
                               double *v, double *x, double half_dt, double dt, int mode) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_update_fp64_split, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, partial, nsplit, G, a, v, x,
-                       half_dt, dt, mode);
+    hipLaunchKernelGGL(k_update_fp64_split, dim3((unsigned)((n + 63) / 64)), dim3(64 * kWaves64), 0, s, n, partial,
+                       nsplit, G, a, v, x, half_dt, dt, mode);
     NB_HIP(hipGetLastError());
 }
 

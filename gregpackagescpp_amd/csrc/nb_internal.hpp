@@ -95,6 +95,7 @@ struct DirectWork {
     DBuf<float> partial;   // [split][6][n]: Kahan sum and compensation per component
     int nsplit = 1;
     int ipl = 2;           // i-bodies per lane
+    int variant = 1;       // 0: j stream through scalar loads, 1: j tiles through LDS
     DBuf<double> partial64;  // [split][3][n] for the FP64 split kernels
     int nsplit64 = 1;
 };
@@ -102,7 +103,7 @@ struct DirectWork {
 // Launchers (stream-ordered, no sync).
 void launch_pack(hipStream_t s, int64_t n, const double *x, const double *m, double G, float4 *pos32);
 void launch_force_mixed(hipStream_t s, int64_t n, const float4 *pos32, float eps2, bool soft, float *partial,
-                        int nsplit, int ipl);
+                        int nsplit, int ipl, int variant);
 // Reduce the split partials to FP64 accelerations, then the closing half kick
 // (v += half_dt*a); when `next` the next step's opening half kick and drift
 // (v += half_dt*a; x += dt*v) and the FP32 repack follow in the same pass.

@@ -40,6 +40,7 @@ struct nb_system {
     // tuning knobs (0 = engine default); set by nb_set_tuning / nb_autotune
     int tune_bh_bucket = 0;
     int tune_direct_ipl = 0;
+    int tune_direct_kernel = 0;  // 1: scalar-load j stream, 2: LDS j tiles (default)
     int tune_direct_waves = 0;  // target waves per SIMD for the direct split
 
     nbe::DirectWork dw;

@@ -19,6 +19,7 @@ namespace {
 int *knob(nb_system *s, const char *key) {
     if (!std::strcmp(key, "direct_ipl")) return &s->tune_direct_ipl;
     if (!std::strcmp(key, "direct_waves")) return &s->tune_direct_waves;
+    if (!std::strcmp(key, "direct_kernel")) return &s->tune_direct_kernel;
     if (!std::strcmp(key, "bh_bucket")) return &s->tune_bh_bucket;
     return nullptr;
 }
@@ -91,12 +92,17 @@ int nb_set_tuning(nb_system *s, const char *key, int64_t value) {
         s->err = "direct_ipl must be 1, 2 or 4";
         return NB_ERR_INVALID_ARGUMENT;
     }
+    if (!std::strcmp(key, "direct_kernel") && value != 0 && value != 1 && value != 2) {
+        s->err = "direct_kernel must be 1 (scalar-load j stream) or 2 (LDS j tiles)";
+        return NB_ERR_INVALID_ARGUMENT;
+    }
     if (value < 0 || value > 4096) {
         s->err = "tuning value out of range";
         return NB_ERR_INVALID_ARGUMENT;
     }
     *k = (int)value;
-    s->acc_valid = s->acc_valid && std::strcmp(key, "direct_ipl") && std::strcmp(key, "direct_waves");
+    s->acc_valid = s->acc_valid && std::strcmp(key, "direct_ipl") && std::strcmp(key, "direct_waves") &&
+                   std::strcmp(key, "direct_kernel");
     return NB_OK;
 }
 
@@ -139,28 +145,35 @@ int nb_autotune(nb_system *s, int method, const char *cache_path) {
                 }
             }
         } else if (s->precision == NB_PREC_MIXED) {
-            int vi, vw;
-            if (cache_lookup(cache_path, "direct", n, s->precision, "direct_ipl", &vi) &&
+            int vk, vi, vw;
+            if (cache_lookup(cache_path, "direct", n, s->precision, "direct_kernel", &vk) &&
+                cache_lookup(cache_path, "direct", n, s->precision, "direct_ipl", &vi) &&
                 cache_lookup(cache_path, "direct", n, s->precision, "direct_waves", &vw)) {
+                s->tune_direct_kernel = vk;
                 s->tune_direct_ipl = vi;
                 s->tune_direct_waves = vw;
             } else {
                 double best = 1e30;
-                int bi = 2, bw = 8;
+                int bk = 2, bi = 2, bw = 8;
                 const int reps = n >= 32768 ? 4 : 20;
-                for (int ipl : {1, 2, 4})
-                    for (int waves : {4, 8, 16}) {
-                        s->tune_direct_ipl = ipl;
-                        s->tune_direct_waves = waves;
-                        double ms = time_ms(st, reps, [&]() { direct_forces(s); });
-                        if (ms < best) {
-                            best = ms;
-                            bi = ipl;
-                            bw = waves;
+                for (int kern : {1, 2})
+                    for (int ipl : {1, 2, 4})
+                        for (int waves : {4, 8, 16}) {
+                            s->tune_direct_kernel = kern;
+                            s->tune_direct_ipl = ipl;
+                            s->tune_direct_waves = waves;
+                            double ms = time_ms(st, reps, [&]() { direct_forces(s); });
+                            if (ms < best) {
+                                best = ms;
+                                bk = kern;
+                                bi = ipl;
+                                bw = waves;
+                            }
                         }
-                    }
+                s->tune_direct_kernel = bk;
                 s->tune_direct_ipl = bi;
                 s->tune_direct_waves = bw;
+                cache_store(cache_path, "direct", n, s->precision, "direct_kernel", bk);
                 cache_store(cache_path, "direct", n, s->precision, "direct_ipl", bi);
                 cache_store(cache_path, "direct", n, s->precision, "direct_waves", bw);
             }

@@ -142,6 +142,8 @@ int nb_adjoint(nb_system *sys, uint64_t steps, uint64_t ckpt_every, const double
                double *dl_dx0, double *dl_dv0, double *dl_dm, int64_t *fwd_ns, int64_t *bwd_ns);
 
 /* Tuning knobs (engine extension; 0 restores the default):
+ *   "direct_kernel" FP32 direct kernel: 1 j stream through scalar loads,
+ *                  2 j tiles through LDS (identical sums)
  *   "direct_ipl"   i-bodies per lane of the FP32 direct kernel (1, 2, 4)
  *   "direct_waves" target waves per SIMD for the direct kernel's j split
  *   "bh_bucket"    rejected tree nodes with at most this many bodies are summed
