@@ -86,52 +86,60 @@ __global__ __launch_bounds__(64 * kWaves64) void k_vjp_split(int64_t n, const do
     }
 }
 
-// lx += scale * sum(partial xyz) ; lm += scale * sum(partial m), fixed split order
+// lx += scale * sum(partial xyz) ; lm += scale * sum(partial m), fixed split
+// order; then, when lv is set, lv += dt * lx (the next reverse step's kick).
 __global__ __launch_bounds__(64 * kWaves64) void k_vjp_apply(int64_t n, const double *__restrict__ partial,
                                                              int nsplit, double scale, double *__restrict__ lx,
-                                                             double *__restrict__ lm) {
+                                                             double *__restrict__ lm, double *__restrict__ lv,
+                                                             double dt) {
     __shared__ double red[kWaves64 - 1][4][64];
     const int64_t k = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
     double acc[4];
     reduce_splits<4>(partial, n, nsplit, k, acc, red);
     if (threadIdx.x >= 64 || k >= n) return;
-    for (int c = 0; c < 3; ++c) lx[c * n + k] += scale * acc[c];
+    for (int c = 0; c < 3; ++c) {
+        const double l = lx[c * n + k] + scale * acc[c];
+        lx[c * n + k] = l;
+        if (lv) lv[c * n + k] += dt * l;
+    }
     lm[k] += scale * acc[3];
 }
 
-__global__ void k_axpy(int64_t len, double a, const double *__restrict__ xv, double *__restrict__ y) {
+__global__ void k_copy(int64_t len, const double *__restrict__ src, double *__restrict__ dst) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < len) y[i] += a * xv[i];
+    if (i < len) dst[i] = src[i];
+}
+
+// Opening half kick and drift into another position array:
+// v += h a ; xout = xin + dt v   (3n entries).
+__global__ void k_kick_drift_to(int64_t len, const double *__restrict__ a, double *__restrict__ v,
+                                const double *__restrict__ xin, double *__restrict__ xout, double h, double dt) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= len) return;
+    const double vv = v[i] + h * a[i];
+    v[i] = vv;
+    xout[i] = xin[i] + dt * vv;
 }
 
 inline unsigned blocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 struct AdjWork {
-    DBuf<double> ck, traj, lx, lv, lm, vpart, xT, vT;
+    DBuf<double> ck, traj, lx, lv, lm, vpart, xT, vT, stage;
+    StepGraph fwd_seg, rev_seg;
 };
 
 }  // namespace
 
-// Forward KDK pieces on the FP64 split kernels.
-static void fast_forces(nb_system *s) {
-    const int64_t n = s->n;
-    s->dw.nsplit64 = choose_nsplit64(n);
-    s->dw.partial64.ensure((size_t)s->dw.nsplit64 * 3 * (size_t)n);
-    launch_force_fp64_split(s->stream, n, s->x.p, s->m.p, s->eps * s->eps, s->eps > 0, s->dw.partial64.p,
-                            s->dw.nsplit64);
-    launch_update_fp64_split(s->stream, n, s->dw.partial64.p, s->dw.nsplit64, s->G, s->a.p, s->v.p, s->x.p,
-                             s->half_dt, s->dt, UPD_STORE);
-}
-
-static void fast_step(nb_system *s) {
-    const int64_t n = s->n;
-    launch_kick_drift(s->stream, n, s->a.p, s->v.p, s->x.p, s->half_dt, s->dt);
-    launch_force_fp64_split(s->stream, n, s->x.p, s->m.p, s->eps * s->eps, s->eps > 0, s->dw.partial64.p,
-                            s->dw.nsplit64);
-    launch_update_fp64_split(s->stream, n, s->dw.partial64.p, s->dw.nsplit64, s->G, s->a.p, s->v.p, s->x.p,
-                             s->half_dt, s->dt, UPD_CLOSE);
-}
-
+// Adjoint driver.  Launch structure (all on the context's stream):
+//   forward:  a(x0); snapshot 0; opening kick+drift; then per step one force
+//             pass and one update that closes the step, snapshots the state
+//             at segment ends, and opens the next step (2 launches/step);
+//   reverse:  per segment, restore its snapshot, recompute its positions into
+//             traj[0..len] (2 launches/step), then per step one VJP pair pass
+//             and one apply that also performs the next step's lv += dt lx.
+// With graphs on, every full forward segment and every interior reverse
+// segment is one graph launch; segment snapshots pass through a fixed
+// staging buffer so that the captured pointers do not change.
 void adjoint_run(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT, const double *gvT, double *gx0,
                  double *gv0, double *gm, int64_t *fwd_ns, int64_t *bwd_ns) {
     const int64_t n = s->n;
@@ -139,6 +147,7 @@ void adjoint_run(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT,
     hipStream_t st = s->stream;
     if (ckpt > steps) ckpt = steps;
     const uint64_t nseg = (steps + ckpt - 1) / ckpt;
+    const bool graphs = s->tune_step_graphs != 2;
     AdjWork w;
     w.ck.ensure(nseg * 2 * n3);
     w.traj.ensure((ckpt + 1) * n3);
@@ -147,32 +156,67 @@ void adjoint_run(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT,
     w.lm.ensure((size_t)n);
     w.xT.ensure(n3);
     w.vT.ensure(n3);
+    w.stage.ensure(2 * n3);
+    s->dw.nsplit64 = choose_nsplit64(n);
+    s->dw.partial64.ensure((size_t)s->dw.nsplit64 * 3 * (size_t)n);
     const int nsplit = choose_nsplit64(n);
     w.vpart.ensure((size_t)nsplit * 4 * (size_t)n);
     const int jchunk = (int)((n + nsplit - 1) / nsplit);
     const dim3 vgrid((unsigned)((n + 63) / 64), (unsigned)nsplit);
     const double h = s->half_dt, dt = s->dt, eps2 = s->eps * s->eps;
     const bool soft = s->eps > 0;
+    const unsigned b3 = blocks((int64_t)n3);
+
+    auto force_at = [&](const double *xs) {
+        launch_force_fp64_split(st, n, xs, s->m.p, eps2, soft, s->dw.partial64.p, s->dw.nsplit64);
+    };
+    auto update = [&](const double *xin, double *xout, int mode, double *sx, double *sv) {
+        launch_update_fp64_split_ex(st, n, s->dw.partial64.p, s->dw.nsplit64, s->G, s->a.p, s->v.p, xin, xout, h,
+                                    dt, mode, sx, sv);
+    };
+    auto copy = [&](const double *src, double *dst, size_t len) {
+        hipLaunchKernelGGL(k_copy, dim3(blocks((int64_t)len)), dim3(kBlock), 0, st, (int64_t)len, src, dst);
+    };
+    // steps q0 .. q0+len-1 of the forward run (state opened for step q0);
+    // the last step of a segment that is not the run's last writes the
+    // synchronised state to the staging buffer
+    auto fwd_steps = [&](uint64_t q0, uint64_t len) {
+        for (uint64_t q = q0; q < q0 + len; ++q) {
+            force_at(s->x.p);
+            if (q + 1 == steps)
+                update(s->x.p, s->x.p, UPD_CLOSE, nullptr, nullptr);
+            else if (q + 1 == q0 + len)
+                update(s->x.p, s->x.p, UPD_CLOSE_OPEN, w.stage.p, w.stage.p + n3);
+            else
+                update(s->x.p, s->x.p, UPD_CLOSE_OPEN, nullptr, nullptr);
+        }
+    };
 
     // ---- forward with checkpoints ----
     NB_HIP(hipStreamSynchronize(st));
     auto t0 = std::chrono::high_resolution_clock::now();
-    fast_forces(s);
-    for (uint64_t q = 0; q <= steps; ++q) {
-        if (q % ckpt == 0 && q < steps) {
-            double *c = w.ck.p + (q / ckpt) * 2 * n3;
-            NB_HIP(hipMemcpyAsync(c, s->x.p, n3 * 8, hipMemcpyDeviceToDevice, st));
-            NB_HIP(hipMemcpyAsync(c + n3, s->v.p, n3 * 8, hipMemcpyDeviceToDevice, st));
+    force_at(s->x.p);
+    update(s->x.p, s->x.p, UPD_STORE, nullptr, nullptr);
+    copy(s->x.p, w.ck.p, n3);
+    copy(s->v.p, w.ck.p + n3, n3);
+    hipLaunchKernelGGL(k_kick_drift_to, dim3(b3), dim3(kBlock), 0, st, (int64_t)n3, s->a.p, s->v.p, s->x.p, s->x.p,
+                       h, dt);
+    for (uint64_t seg = 0; seg < nseg; ++seg) {
+        const uint64_t q0 = seg * ckpt, len = std::min<uint64_t>(ckpt, steps - q0);
+        const bool last = seg + 1 == nseg;
+        if (graphs && !last && len == ckpt) {
+            if (!w.fwd_seg.valid()) w.fwd_seg.capture(st, [&]() { fwd_steps(q0, len); });
+            w.fwd_seg.launch(st);
+        } else {
+            fwd_steps(q0, len);
         }
-        if (q < steps) {
-            ScopedTimer tm(s->prof, "force", st);
-            fast_step(s);
-        }
+        if (!last) copy(w.stage.p, w.ck.p + (seg + 1) * 2 * n3, 2 * n3);
     }
+    NB_HIP(hipGetLastError());
     NB_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::high_resolution_clock::now();
-    NB_HIP(hipMemcpyAsync(w.xT.p, s->x.p, n3 * 8, hipMemcpyDeviceToDevice, st));
-    NB_HIP(hipMemcpyAsync(w.vT.p, s->v.p, n3 * 8, hipMemcpyDeviceToDevice, st));
+    copy(s->x.p, w.xT.p, n3);
+    copy(s->v.p, w.vT.p, n3);
 
     // ---- cotangents in ----
     std::vector<double> tmp(n3);
@@ -186,8 +230,9 @@ void adjoint_run(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT,
     NB_HIP(hipMemsetAsync(w.lm.p, 0, (size_t)n * 8, st));
     NB_HIP(hipStreamSynchronize(st));
 
-    auto vjp = [&](const double *xs, double coef) {
-        ScopedTimer tm(s->prof, "adjoint", st);
+    // lx += coef G J(xs)^T lv, lm += coef G (dA/dm)^T lv; then, when `open`,
+    // lv += dt lx for the next reverse step
+    auto vjp = [&](const double *xs, double coef, bool open) {
         if (soft)
             hipLaunchKernelGGL(k_vjp_split<true>, vgrid, dim3(64 * kWaves64), 0, st, n, xs, s->m.p, w.lv.p, jchunk,
                                eps2, w.vpart.p);
@@ -195,34 +240,44 @@ void adjoint_run(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT,
             hipLaunchKernelGGL(k_vjp_split<false>, vgrid, dim3(64 * kWaves64), 0, st, n, xs, s->m.p, w.lv.p, jchunk,
                                eps2, w.vpart.p);
         hipLaunchKernelGGL(k_vjp_apply, dim3((unsigned)((n + 63) / 64)), dim3(64 * kWaves64), 0, st, n, w.vpart.p,
-                           nsplit, coef * s->G, w.lx.p, w.lm.p);
+                           nsplit, coef * s->G, w.lx.p, w.lm.p, open ? w.lv.p : nullptr, dt);
+    };
+    // recompute of a segment from its restored state (traj[0], v) and its
+    // reverse sweep; `first` adds the closing VJP at x_T
+    auto rev_segment = [&](uint64_t start, uint64_t len, bool first) {
+        double *tr = w.traj.p;
+        force_at(tr);
+        update(tr, tr, UPD_STORE, nullptr, nullptr);
+        hipLaunchKernelGGL(k_kick_drift_to, dim3(b3), dim3(kBlock), 0, st, (int64_t)n3, s->a.p, s->v.p, tr, tr + n3,
+                           h, dt);
+        for (uint64_t q = 1; q <= len; ++q) {
+            force_at(tr + q * n3);
+            if (q < len)
+                update(tr + q * n3, tr + (q + 1) * n3, UPD_CLOSE_OPEN, nullptr, nullptr);
+            else
+                update(tr + q * n3, tr + q * n3, UPD_CLOSE, nullptr, nullptr);
+        }
+        if (first) vjp(tr + len * n3, h, true);  // closing kick of the last step, at x_T
+        for (uint64_t q = len; q-- > 0;) {
+            const uint64_t sg = start + q;
+            vjp(tr + q * n3, sg > 0 ? 2.0 * h : h, sg > 0);  // opening kick of sg (+ closing kick of sg-1)
+        }
     };
 
     // ---- reverse sweep, segment by segment ----
     auto t2 = std::chrono::high_resolution_clock::now();
-    bool first = true;
     for (uint64_t seg = nseg; seg-- > 0;) {
         const uint64_t start = seg * ckpt;
         const uint64_t len = std::min<uint64_t>(ckpt, steps - start);
         const double *c = w.ck.p + seg * 2 * n3;
-        NB_HIP(hipMemcpyAsync(s->x.p, c, n3 * 8, hipMemcpyDeviceToDevice, st));
-        NB_HIP(hipMemcpyAsync(s->v.p, c + n3, n3 * 8, hipMemcpyDeviceToDevice, st));
-        fast_forces(s);
-        NB_HIP(hipMemcpyAsync(w.traj.p, s->x.p, n3 * 8, hipMemcpyDeviceToDevice, st));
-        for (uint64_t q = 1; q <= len; ++q) {
-            ScopedTimer tm(s->prof, "force", st);
-            fast_step(s);
-            NB_HIP(hipMemcpyAsync(w.traj.p + q * n3, s->x.p, n3 * 8, hipMemcpyDeviceToDevice, st));
-        }
-        for (uint64_t q = len; q-- > 0;) {
-            const uint64_t sg = start + q;
-            if (first) {
-                vjp(w.traj.p + len * n3, h);  // closing kick of the last step, at x_T
-                first = false;
-            }
-            hipLaunchKernelGGL(k_axpy, dim3(blocks((int64_t)n3)), dim3(kBlock), 0, st, (int64_t)n3, dt, w.lx.p,
-                               w.lv.p);
-            vjp(w.traj.p + q * n3, sg > 0 ? 2.0 * h : h);  // opening kick of sg (+ closing kick of sg-1)
+        copy(c, w.traj.p, n3);
+        copy(c + n3, s->v.p, n3);
+        const bool first = seg + 1 == nseg;
+        if (graphs && !first && seg > 0 && len == ckpt) {
+            if (!w.rev_seg.valid()) w.rev_seg.capture(st, [&]() { rev_segment(start, len, false); });
+            w.rev_seg.launch(st);
+        } else {
+            rev_segment(start, len, first);
         }
     }
     NB_HIP(hipGetLastError());
@@ -237,11 +292,11 @@ void adjoint_run(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT,
     for (int64_t i = 0; i < n; ++i)
         for (int c = 0; c < 3; ++c) gv0[3 * i + c] = tmp[(size_t)(c * n + i)];
     NB_HIP(hipMemcpy(gm, w.lm.p, (size_t)n * 8, hipMemcpyDeviceToHost));
-    NB_HIP(hipMemcpyAsync(s->x.p, w.xT.p, n3 * 8, hipMemcpyDeviceToDevice, st));
-    NB_HIP(hipMemcpyAsync(s->v.p, w.vT.p, n3 * 8, hipMemcpyDeviceToDevice, st));
-    fast_forces(s);
+    copy(w.xT.p, s->x.p, n3);
+    copy(w.vT.p, s->v.p, n3);
+    force_at(s->x.p);
+    update(s->x.p, s->x.p, UPD_STORE, nullptr, nullptr);
     NB_HIP(hipStreamSynchronize(st));
-    s->prof.collect();
     s->acc_valid = s->precision == NB_PREC_FP64_FAST;
     s->evolved = true;
     s->acc_method = NB_LEAPFROG_KDK;

@@ -149,49 +149,84 @@ void direct_forces(nb_system *s) {
 
 // KDK steps from valid accelerations (evolve's KDK branch :2949-2968 with
 // leapfrog_kdk_acc_e_and_step :514-526 and s_coll when collisions are on).
+// Everything a captured step graph depends on: buffer addresses, launch
+// shapes and the scalars baked into its kernel arguments.
+GraphKey graph_key(const nb_system *s) {
+    GraphKey k;
+    k.n = s->n;
+    k.precision = s->precision;
+    k.ptrs = {s->x.p, s->v.p, s->a.p, s->m.p, s->dw.pos32.p, s->dw.partial.p, s->dw.partial64.p};
+    k.shape = {s->dw.nsplit, s->dw.ipl, s->dw.variant, s->dw.nsplit64, form_of(s)};
+    k.scalars = {s->G, s->dt, s->half_dt, s->eps};
+    return k;
+}
+
 void direct_kdk_steps(nb_system *s, uint64_t steps) {
     const bool mixed = s->precision == NB_PREC_MIXED;
     const bool fast64 = s->precision == NB_PREC_FP64_FAST;
     const bool coll = s->coll_mode == NB_OVERLAP_COLL_CHECK;
-    bool need_open = true;
-    for (uint64_t st = 1; st <= steps; ++st) {
+    // one step's launches; `open` = the opening half kick and drift are still
+    // due; returns whether the next step must open
+    auto issue_step = [&](bool open, bool last) -> bool {
         const int64_t n = s->n;
-        if (n == 0) break;
-        if (need_open) {
+        if (open) {
             launch_kick_drift(s->stream, n, s->a.p, s->v.p, s->x.p, s->half_dt, s->dt);
             if (mixed) launch_pack(s->stream, n, s->x.p, s->m.p, s->G, s->dw.pos32.p);
         }
         if (fast64) {
-            if (s->dw.partial64.n < (size_t)s->dw.nsplit64 * 3 * (size_t)n) direct_forces(s);
             {
                 ScopedTimer t(s->prof, "force", s->stream);
                 launch_force_fp64_split(s->stream, n, s->x.p, s->m.p, s->eps * s->eps, form_of(s) == 1,
                                         s->dw.partial64.p, s->dw.nsplit64);
             }
-            const bool last = st == steps || coll;
             launch_update_fp64_split(s->stream, n, s->dw.partial64.p, s->dw.nsplit64, s->G, s->a.p, s->v.p, s->x.p,
                                      s->half_dt, s->dt, last ? UPD_CLOSE : UPD_CLOSE_OPEN);
-            need_open = last;
-        } else if (mixed) {
-            if (s->dw.pos32.n < (size_t)n) direct_forces(s);  // defensive: sizes follow n
+            return last;
+        }
+        if (mixed) {
             {
                 ScopedTimer t(s->prof, "force", s->stream);
                 launch_force_mixed(s->stream, n, s->dw.pos32.p, (float)(s->eps * s->eps), form_of(s) == 1,
                                    s->dw.partial.p, s->dw.nsplit, s->dw.ipl, s->dw.variant);
             }
-            const bool last = st == steps || coll;
             ScopedTimer t(s->prof, "update", s->stream);
             launch_update_mixed(s->stream, n, s->dw.partial.p, s->dw.nsplit, s->a.p, s->v.p, s->x.p, s->m.p, s->G,
                                 s->half_dt, s->dt, last ? UPD_CLOSE : UPD_CLOSE_OPEN, s->dw.pos32.p);
-            need_open = last;
-        } else {
-            {
-                ScopedTimer t(s->prof, "force", s->stream);
-                launch_force_fp64(s->stream, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->a.p);
-            }
-            launch_kick(s->stream, n, s->a.p, s->v.p, s->half_dt);
-            need_open = true;
+            return last;
         }
+        {
+            ScopedTimer t(s->prof, "force", s->stream);
+            launch_force_fp64(s->stream, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->a.p);
+        }
+        launch_kick(s->stream, n, s->a.p, s->v.p, s->half_dt);
+        return true;
+    };
+    // Interior steps of small systems replay a captured graph of kGraphSteps
+    // steps (no collisions, no per-launch timing).
+    const bool graphs = !coll && !s->prof.on &&
+                        (s->tune_step_graphs == 1 || (s->tune_step_graphs == 0 && s->n <= kGraphMaxBodies));
+    bool need_open = true;
+    for (uint64_t st = 1; st <= steps;) {
+        const int64_t n = s->n;
+        if (n == 0) break;
+        if (graphs && st > 1 && steps - st >= (uint64_t)kGraphSteps) {
+            const GraphKey key = graph_key(s);
+            if (!s->step_graph.valid() || !(s->step_graph_key == key)) {
+                s->step_graph.capture(s->stream, [&]() {
+                    bool open = need_open;
+                    for (int q = 0; q < kGraphSteps; ++q) open = issue_step(open, false);
+                });
+                s->step_graph_key = key;
+            }
+            s->step_graph.launch(s->stream);
+            st += kGraphSteps;
+            s->step_count += kGraphSteps;
+            continue;
+        }
+        if (fast64 && s->dw.partial64.n < (size_t)s->dw.nsplit64 * 3 * (size_t)n) direct_forces(s);
+        if (mixed && s->dw.pos32.n < (size_t)n) direct_forces(s);  // defensive: sizes follow n
+        need_open = issue_step(need_open, st == steps || coll);
+        ++st;
         ++s->step_count;
         if (coll) {
             collision_pass(s, s->step_count);

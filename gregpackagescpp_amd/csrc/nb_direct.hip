@@ -310,10 +310,16 @@ __global__ __launch_bounds__(64 * kWaves64) void k_force_fp64_split(int64_t n, c
     }
 }
 
+// Split reduction, then the KDK update for UPD_CLOSE / UPD_CLOSE_OPEN.  The
+// drift reads xin and writes xout (the same array for an in-place step); when
+// snap_x is set, the synchronised state (x before the drift, v after the
+// closing kick) is also written there and to snap_v.
 __global__ __launch_bounds__(64 * kWaves64) void k_update_fp64_split(int64_t n, const double *__restrict__ partial,
                                                                      int nsplit, double G, double *__restrict__ a,
-                                                                     double *__restrict__ v, double *__restrict__ x,
-                                                                     double half_dt, double dt, int mode) {
+                                                                     double *__restrict__ v, const double *xin,
+                                                                     double *xout, double half_dt, double dt,
+                                                                     int mode, double *__restrict__ snap_x,
+                                                                     double *__restrict__ snap_v) {
     __shared__ double red[kWaves64 - 1][3][64];
     const int64_t i = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
     double acc[3];
@@ -329,8 +335,13 @@ __global__ __launch_bounds__(64 * kWaves64) void k_update_fp64_split(int64_t n, 
     for (int c = 0; c < 3; ++c) {
         double vv = v[c * n + i] + half_dt * acc[c];
         if (mode == UPD_CLOSE_OPEN) {
+            const double xx = xin[c * n + i];
+            if (snap_x) {
+                snap_x[c * n + i] = xx;
+                snap_v[c * n + i] = vv;
+            }
             vv += half_dt * acc[c];
-            x[c * n + i] += dt * vv;
+            xout[c * n + i] = xx + dt * vv;
         }
         v[c * n + i] = vv;
     }
@@ -478,9 +489,15 @@ void launch_force_fp64_split(hipStream_t s, int64_t n, const double *x, const do
 
 void launch_update_fp64_split(hipStream_t s, int64_t n, const double *partial, int nsplit, double G, double *a,
                               double *v, double *x, double half_dt, double dt, int mode) {
+    launch_update_fp64_split_ex(s, n, partial, nsplit, G, a, v, x, x, half_dt, dt, mode, nullptr, nullptr);
+}
+
+void launch_update_fp64_split_ex(hipStream_t s, int64_t n, const double *partial, int nsplit, double G, double *a,
+                                 double *v, const double *xin, double *xout, double half_dt, double dt, int mode,
+                                 double *snap_x, double *snap_v) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_update_fp64_split, dim3((unsigned)((n + 63) / 64)), dim3(64 * kWaves64), 0, s, n, partial,
-                       nsplit, G, a, v, x, half_dt, dt, mode);
+                       nsplit, G, a, v, xin, xout, half_dt, dt, mode, snap_x, snap_v);
     NB_HIP(hipGetLastError());
 }
 

@@ -1,6 +1,7 @@
 // nb_internal.hpp -- shared host/device declarations of the gfx950 N-body engine.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <array>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -89,6 +90,59 @@ struct ScopedTimer {
     }
 };
 
+// A sequence of launches captured from one stream and replayed with one
+// graph launch (used for step loops of small systems, where launch latency
+// dominates).  The captured code must not record events or memsets.
+struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    StepGraph() = default;
+    StepGraph(const StepGraph &) = delete;
+    StepGraph &operator=(const StepGraph &) = delete;
+    ~StepGraph() { reset(); }
+    bool valid() const { return exec != nullptr; }
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+    }
+    template <typename F>
+    void capture(hipStream_t st, F &&f) {
+        reset();
+        NB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            f();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        NB_HIP(hipStreamEndCapture(st, &g));
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            exec = nullptr;
+            throw DeviceError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    }
+    void launch(hipStream_t st) { NB_HIP(hipGraphLaunch(exec, st)); }
+};
+
+// Direct step loops of systems up to kGraphMaxBodies bodies replay a graph of
+// kGraphSteps steps.
+constexpr int kGraphSteps = 32;
+constexpr int64_t kGraphMaxBodies = 16384;
+
+struct GraphKey {
+    int64_t n = -1;
+    int precision = -1;
+    std::array<const void *, 7> ptrs{};
+    std::array<int, 5> shape{};
+    std::array<double, 4> scalars{};
+    bool operator==(const GraphKey &o) const {
+        return n == o.n && precision == o.precision && ptrs == o.ptrs && shape == o.shape && scalars == o.scalars;
+    }
+};
+
 // ---- direct summation (nb_direct.hip) ----
 struct DirectWork {
     DBuf<float4> pos32;    // x, y, z, G*m in FP32
@@ -123,6 +177,11 @@ void launch_force_fp64_split(hipStream_t s, int64_t n, const double *x, const do
                              double *partial, int nsplit);
 void launch_update_fp64_split(hipStream_t s, int64_t n, const double *partial, int nsplit, double G, double *a,
                               double *v, double *x, double half_dt, double dt, int mode);
+// As above with the drift reading xin and writing xout, and (when snap_x is
+// set) the synchronised state x_q, v_q written to snap_x / snap_v on the way.
+void launch_update_fp64_split_ex(hipStream_t s, int64_t n, const double *partial, int nsplit, double G, double *a,
+                                 double *v, const double *xin, double *xout, double half_dt, double dt, int mode,
+                                 double *snap_x, double *snap_v);
 
 // update modes for launch_update_mixed
 enum { UPD_STORE = 0, UPD_CLOSE = 1, UPD_CLOSE_OPEN = 2 };

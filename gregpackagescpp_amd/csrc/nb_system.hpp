@@ -41,7 +41,12 @@ struct nb_system {
     int tune_bh_bucket = 0;
     int tune_direct_ipl = 0;
     int tune_direct_kernel = 0;  // 1: scalar-load j stream, 2: LDS j tiles (default)
-    int tune_direct_waves = 0;  // target waves per SIMD for the direct split
+    int tune_direct_waves = 0;   // target waves per SIMD for the direct split
+    int tune_step_graphs = 0;    // 0: graphs for n <= kGraphMaxBodies, 1: always, 2: never
+
+    // captured direct step loop (kGraphSteps steps) and what it was captured for
+    nbe::StepGraph step_graph;
+    nbe::GraphKey step_graph_key;
 
     nbe::DirectWork dw;
     nbe::BHWork *bh = nullptr;
@@ -56,6 +61,7 @@ namespace nbe {
 void direct_forces(nb_system *s);                    // a(x) into s->a (and pos32 for the mixed path)
 void direct_kdk_steps(nb_system *s, uint64_t steps); // steps of KDK from valid a
 int form_of(const nb_system *s);                     // 0 = unsoftened reference form, 1 = softened
+GraphKey graph_key(const nb_system *s);
 // collisions after a step's closing kick (nb_coll.hip)
 void collision_pass(nb_system *s, uint64_t step);
 // Barnes-Hut (nb_bh.hip)

@@ -20,6 +20,7 @@ int *knob(nb_system *s, const char *key) {
     if (!std::strcmp(key, "direct_ipl")) return &s->tune_direct_ipl;
     if (!std::strcmp(key, "direct_waves")) return &s->tune_direct_waves;
     if (!std::strcmp(key, "direct_kernel")) return &s->tune_direct_kernel;
+    if (!std::strcmp(key, "step_graphs")) return &s->tune_step_graphs;
     if (!std::strcmp(key, "bh_bucket")) return &s->tune_bh_bucket;
     return nullptr;
 }
@@ -94,6 +95,10 @@ int nb_set_tuning(nb_system *s, const char *key, int64_t value) {
     }
     if (!std::strcmp(key, "direct_kernel") && value != 0 && value != 1 && value != 2) {
         s->err = "direct_kernel must be 1 (scalar-load j stream) or 2 (LDS j tiles)";
+        return NB_ERR_INVALID_ARGUMENT;
+    }
+    if (!std::strcmp(key, "step_graphs") && value != 0 && value != 1 && value != 2) {
+        s->err = "step_graphs must be 1 (on) or 2 (off)";
         return NB_ERR_INVALID_ARGUMENT;
     }
     if (value < 0 || value > 4096) {

@@ -147,7 +147,8 @@ int nb_adjoint(nb_system *sys, uint64_t steps, uint64_t ckpt_every, const double
  *   "direct_ipl"   i-bodies per lane of the FP32 direct kernel (1, 2, 4)
  *   "direct_waves" target waves per SIMD for the direct kernel's j split
  *   "bh_bucket"    rejected tree nodes with at most this many bodies are summed
- *                  body by body in the wave walk */
+ *                  body by body in the wave walk
+ *   "step_graphs"  1 capture step loops of small systems in HIP graphs, 2 off */
 int nb_set_tuning(nb_system *sys, const char *key, int64_t value);
 int nb_get_tuning(const nb_system *sys, const char *key, int64_t *value);
 /* Autotuner (engine extension): times the candidate settings of the knobs
