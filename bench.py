@@ -10,7 +10,12 @@ N(N-1)*K / t over the K timed steps.  The JSON line also carries:
                average launch duration, timed with HIP events on the
                library's stream;
   cpu_baseline the compiled reference's own evolve(leapfrog_kdk) (oracle/_ref,
-               long double, one thread) on a bounded sample, else the oracle.
+               long double, one thread) on a bounded sample, else the oracle;
+  cpu_baseline_allcore  an FP64 restatement over all host threads (OpenMP);
+  bh           Barnes-Hut KDK at 1,048,576 bodies (I_alg interactions/s), with
+               its own single-thread and all-core CPU baselines;
+  adjoint      backward/forward wall-time ratio of the adjoint at N = 4096;
+  batch        4096-system comet ensemble with merging (interactions/s).
 """
 import argparse
 import json
@@ -29,8 +34,19 @@ N_BODIES = 65536
 EPS, DT = 0.01, 1e-3
 FLOP_PER_INTERACTION = 20
 PEAK_FP32_TFLOPS = 157.3
-PROFILED_FORCE_TRAFFIC = {"bodies": 65536, "ipl": 2, "waves": 16, "fetch_kib": 7709.8, "write_kib": 49152.0,
-                          "fetch_factor": 2.0, "write_factor": 1.0}
+# rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the FP32 force kernel
+# at 65,536 bodies per launch configuration (direct_kernel, direct_ipl,
+# direct_waves), from `bench.py --profile-direct --kernel K --ipl I --waves W`;
+# the counters are scaled by what they read for a copy of known size
+# (`bench.py --hbm-copy`): FETCH_SIZE x 2.0, WRITE_SIZE x 1.0.
+PROFILED_FORCE_TRAFFIC = {
+    (2, 1, 16): (4115.8, 24576.0),
+    (2, 2, 16): (4123.2, 49152.0),
+    (2, 4, 16): (4145.3, 98304.0),
+    (1, 2, 16): (7715.4, 49152.0),
+    (1, 4, 16): (7827.2, 98304.0),
+}
+FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
 
 
 def cpu_baseline(m, pos, vel):
@@ -157,6 +173,78 @@ def cpu_bh_baseline(m, pos):
                       f"one force evaluation ({cnt} interactions, {t:.1f} s)"}
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    return O
+
+
+def cpu_direct_allcore(m, pos):
+    """All-core FP64 restatement (OpenMP over i, every i summing all j) on the
+    full 65,536-body IC: one force evaluation."""
+    O = _oracle()
+    threads = O.max_threads()
+    t0 = time.perf_counter()
+    O.direct_acc_mt(m, pos, 1.0, EPS, threads)
+    t = time.perf_counter() - t0
+    n = len(m)
+    return {"value": n * (n - 1) / t, "unit": "interactions/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 softened direct sum, OpenMP over {threads} host threads, full N={n} Plummer IC, "
+                      f"one force evaluation ({t:.2f} s)"}
+
+
+def cpu_bh_allcore(m, pos):
+    """All-core corrected BH: serial insertion tree, OpenMP walk, full 1M bodies."""
+    O = _oracle()
+    threads = O.max_threads()
+    t0 = time.perf_counter()
+    _, cnt = O.bh_acc_mt(m, pos, 1.0, EPS, BH_THETA, threads)
+    t = time.perf_counter() - t0
+    return {"value": cnt / t, "unit": "interactions/s", "cores": threads, "kind": "port",
+            "sample": f"oracle insertion octree (serial build) + per-body MAC walk over {threads} host threads, "
+                      f"full {len(m)}-body two-Plummer IC, one force evaluation ({cnt} interactions, {t:.1f} s)"}
+
+
+def bench_batch(capi, steps):
+    """Batched ensemble: 4096 comet-string systems (central body of 1.9e27 kg
+    plus n ~ U{8..512} fragments, seed 3), dt = 5 s, merging on, mixed
+    precision, one persistent launch per 1000 steps.  Interactions are the
+    force pairs actually evaluated, sum over steps and systems of n(n-1) with
+    n reduced by each merge from the step after it; the collision scan's
+    n(n-1)/2 overlap tests per step are reported beside them."""
+    G_SI = 6.6743e-11
+    offsets, m, r, pos, vel, e = capi.ic_comet_ensemble(3, 4096, 8, 512)
+    sizes = np.diff(offsets).astype(np.int64)
+    b = capi.Batch(offsets, m, r, pos, vel, e)
+    b.set_params(G_SI, 5.0, 0.0, capi.OVERLAP_COLL_CHECK, True, 1e10, capi.PREC_MIXED)
+    ns = b.evolve(steps)
+    log = b.merge_log()
+    b.close()
+    t = ns * 1e-9
+    # n_s(t) for the force pass of step t: merges at steps < t have removed bodies
+    inter, tests = 0, 0
+    counts = np.zeros(len(sizes), np.int64)
+    if len(log):
+        np.add.at(counts, log[:, 0], 1)
+    for sys_id in range(len(sizes)):
+        n0 = int(sizes[sys_id])
+        msteps = np.sort(log[log[:, 0] == sys_id, 1]) if counts[sys_id] else np.zeros(0, np.int64)
+        # piecewise-constant n over [1, steps]
+        bounds = np.concatenate([[1], msteps + 1, [steps + 1]])
+        for k in range(len(bounds) - 1):
+            nk = n0 - k
+            span = int(bounds[k + 1] - bounds[k])
+            inter += span * nk * (nk - 1)
+            tests += span * nk * (nk - 1) // 2
+    return {"value": inter / t, "unit": "interactions/s", "systems": len(sizes), "bodies": int(offsets[-1]),
+            "steps": steps, "wall_s": t, "interactions": inter, "overlap_tests_per_s": tests / t,
+            "merges": int(len(log)),
+            "workload": "4096 comet-string systems (seed 3): 1.9e27 kg central body + U{8..512} fragments, "
+                        "dt=5 s, inelastic merging on, FP32 pair forces from LDS, FP64 state, exact FP64 "
+                        "collision replay; persistent launch, one workgroup per system"}
+
+
 def profile_run(capi, args):
     """Fixed-configuration runs for counter collection (no autotuning, no HIP
     events): the kernels seen are exactly those of the timed step."""
@@ -168,8 +256,9 @@ def profile_run(capi, args):
     if args.profile_direct:
         m, pos, vel = capi.ic_plummer(1, N_BODIES)
         s.set_bodies(m, pos, vel)
-        s.set_tuning("direct_ipl", 2)
-        s.set_tuning("direct_waves", 16)
+        s.set_tuning("direct_kernel", args.kernel)
+        s.set_tuning("direct_ipl", args.ipl)
+        s.set_tuning("direct_waves", args.waves)
         s.set_iterations(args.steps)
         s.evolve(capi.LEAPFROG_KDK)
     else:
@@ -191,9 +280,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-bh", action="store_true")
     ap.add_argument("--bh-steps", type=int, default=5)
+    ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--hbm-copy", action="store_true", help="run only the HBM copy calibration kernel")
     ap.add_argument("--profile-direct", action="store_true",
                     help="for rocprofv3: the 65,536-body mixed direct step with fixed tuning, no events, no JSON")
+    ap.add_argument("--kernel", type=int, default=1, help="--profile-direct: direct_kernel knob")
+    ap.add_argument("--ipl", type=int, default=2, help="--profile-direct: direct_ipl knob")
+    ap.add_argument("--waves", type=int, default=16, help="--profile-direct: direct_waves knob")
     ap.add_argument("--profile-bh", action="store_true",
                     help="for rocprofv3: the 1M-body BH step (mixed, bucket 32), no events, no JSON")
     args = ap.parse_args()
@@ -220,7 +313,7 @@ def main():
     s.set_precision(capi.PREC_MIXED)
     s.set_bodies(m, pos, vel)
     s.autotune(capi.LEAPFROG_KDK)  # project autotuner: i-bodies per lane, j-split width
-    tuned = {"direct_ipl": s.tuning("direct_ipl"), "direct_waves": s.tuning("direct_waves")}
+    tuned = {k: s.tuning(k) for k in ("direct_kernel", "direct_ipl", "direct_waves")}
     s.set_iterations(max(1, args.warmup))
     s.evolve(capi.LEAPFROG_KDK)  # warm-up: a(x0) + W steps
 
@@ -245,9 +338,10 @@ def main():
     # reads for a copy kernel of known size; valid for the configuration it was
     # measured at (its write traffic scales with the j-split count).
     traffic = None
-    prof = PROFILED_FORCE_TRAFFIC
-    if n == prof["bodies"] and tuned["direct_ipl"] == prof["ipl"] and tuned["direct_waves"] == prof["waves"]:
-        traffic = (prof["fetch_kib"] * prof["fetch_factor"] + prof["write_kib"] * prof["write_factor"]) * 1024
+    key = (tuned["direct_kernel"], tuned["direct_ipl"], tuned["direct_waves"])
+    if n == N_BODIES and key in PROFILED_FORCE_TRAFFIC:
+        fetch_kib, write_kib = PROFILED_FORCE_TRAFFIC[key]
+        traffic = (fetch_kib * FETCH_FACTOR + write_kib * WRITE_FACTOR) * 1024
 
     line = {
         "metric": "interactions/s at N=64K direct & 1M BH; adjoint/forward time ratio",
@@ -264,7 +358,8 @@ def main():
         "data": "synthetic: seeded Plummer sphere (G=M=a=1, equal masses), no external data",
         "config": {"workload": "direct-sum leapfrog KDK, Plummer N=65536, eps=0.01, dt=1e-3, mixed precision",
                    "bodies": n, "global_batch": 1, "parallelism": "single GPU"},
-        "roofline": {"bound": "valu", "kernel": "k_force_mixed", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "kernel": "k_force_mixed_lds" if tuned["direct_kernel"] == 2 else "k_force_mixed",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "avg_launch_ms": avg_force_s * 1e3, "update_kernel_avg_ms": upd_ms / max(launches, 1),
                      "flop_per_interaction": FLOP_PER_INTERACTION},
@@ -274,11 +369,15 @@ def main():
     if not args.no_bh:
         line["bh"] = bench_bh(capi, args.bh_steps)
         line["adjoint"] = bench_adjoint(capi)
+        line["batch"] = bench_batch(capi, args.batch_steps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(m, pos, vel)
+        line["cpu_baseline_allcore"] = cpu_direct_allcore(m, pos)
         if not args.no_bh:
             bm, bp, _ = capi.ic_two_plummer(2, 65536)
             line["bh"]["cpu_baseline"] = cpu_bh_baseline(bm, bp)
+            bm, bp, _ = capi.ic_two_plummer(2, BH_BODIES)
+            line["bh"]["cpu_baseline_allcore"] = cpu_bh_allcore(bm, bp)
     print(json.dumps(line), flush=True)
 
 

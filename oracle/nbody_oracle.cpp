@@ -30,6 +30,9 @@
  *     mass conservation; equal-mass elastic swap closed form.
  *   adjoint: central finite differences on tiny systems (tests/test_oracle.py).
  */
+#include <omp.h>
+
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstddef>
@@ -670,6 +673,79 @@ int orc_bh_acc_d(int64_t n, double G, double eps, double theta, const double *m,
     if (interactions) *interactions = total;
     return 0;
 }
+
+/* ---------------- all-core CPU baselines (timing only) ----------------
+ * FP64 restatements spread over host threads with OpenMP, for the all-core
+ * CPU figure printed beside the GPU throughput.  They are not parity
+ * references: the direct form sums all j for every i (no pair symmetry, the
+ * GPU's work), softened pair force a_i += G m_j d / (s sqrt(s)),
+ * s = |d|^2 + eps^2; the BH form builds the same insertion tree serially and
+ * walks the bodies in parallel. */
+}  // extern "C"
+
+namespace {
+__attribute__((target_clones("arch=haswell", "default")))
+void direct_rows_mt(int64_t i0, int64_t i1, int64_t n, double G, double eps2, const double *m, const double *x,
+                    const double *y, const double *z, double *acc) {
+    for (int64_t i = i0; i < i1; ++i) {
+        const double xi = x[i], yi = y[i], zi = z[i];
+        double ax = 0, ay = 0, az = 0;
+#pragma omp simd reduction(+ : ax, ay, az)
+        for (int64_t j = 0; j < n; ++j) {
+            const double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
+            const double s = dx * dx + dy * dy + dz * dz + eps2;
+            const double w = m[j] / (s * std::sqrt(s));
+            ax += w * dx;
+            ay += w * dy;
+            az += w * dz;
+        }
+        acc[3 * i] = G * ax;
+        acc[3 * i + 1] = G * ay;
+        acc[3 * i + 2] = G * az;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int orc_direct_acc_mt(int64_t n, double G, double eps, const double *m, const double *pos, double *acc,
+                      int threads) {
+    std::vector<double> x(n), y(n), z(n);
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = pos[3 * i];
+        y[i] = pos[3 * i + 1];
+        z[i] = pos[3 * i + 2];
+    }
+    const double eps2 = eps * eps;
+    const int64_t rows = 64;
+#pragma omp parallel for schedule(dynamic) num_threads(threads > 0 ? threads : omp_get_max_threads())
+    for (int64_t i0 = 0; i0 < n; i0 += rows)
+        direct_rows_mt(i0, std::min(n, i0 + rows), n, G, eps2, m, x.data(), y.data(), z.data(), acc);
+    return 0;
+}
+
+int orc_bh_acc_mt(int64_t n, double G, double eps, double theta, const double *m, const double *pos, double *acc,
+                  int64_t *interactions, int threads) {
+    Tree t;
+    int rc = build_tree(t, n, m, pos);
+    if (rc) return rc;
+    std::vector<int64_t> leaf_of(n, -1);
+    for (size_t c = 0; c < t.cells.size(); ++c)
+        if (t.cells[c].body >= 0) leaf_of[t.cells[c].body] = (int64_t)c;
+    int64_t total = 0;
+    const double eps2 = eps * eps;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : total) \
+    num_threads(threads > 0 ? threads : omp_get_max_threads())
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t cnt = 0;
+        walk_body(t, leaf_of[i], pos + 3 * i, G, eps2, (long double)theta, acc + 3 * i, &cnt, nullptr);
+        total += cnt;
+    }
+    if (interactions) *interactions = total;
+    return 0;
+}
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
 
 /* Accepted-cell list of one body, as pre-order node indices (see orc_bh_tree_d). */
 int64_t orc_bh_accepted_d(int64_t n, double theta, const double *m, const double *pos, int64_t body_idx,

@@ -58,6 +58,9 @@ def lib():
         L.orc_adjoint_d.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64, _D, _D, _D, _D, _D, _D,
                                     _D, _D]
         L.orc_forward_d.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64, _D, _D, _D]
+        L.orc_direct_acc_mt.argtypes = [C.c_int64, C.c_double, C.c_double, _D, _D, _D, C.c_int]
+        L.orc_bh_acc_mt.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, _D, _D, _D, _I64, C.c_int]
+        L.orc_max_threads.restype = C.c_int
         L.orc_trace.argtypes = [C.c_int]
         L.orc_trace_len.restype = C.c_int64
         L.orc_trace_copy.argtypes = [_I64, _LDP]
@@ -209,6 +212,31 @@ def bh_acc(m, pos, G, eps, theta, per_body=False):
     if rc:
         raise RuntimeError(f"oracle bh_acc failed: {rc}")
     return (acc, cnt.value, pb) if per_body else (acc, cnt.value)
+
+
+def direct_acc_mt(m, pos, G, eps, threads=0):
+    """All-core FP64 softened direct sum (timing baseline, not a parity reference)."""
+    n = len(m)
+    m, pos = _d(m), _d(pos)
+    acc = np.zeros((n, 3))
+    lib().orc_direct_acc_mt(n, G, eps, _dp(m), _dp(pos), _dp(acc), threads)
+    return acc
+
+
+def bh_acc_mt(m, pos, G, eps, theta, threads=0):
+    """All-core corrected BH (serial insertion tree, parallel walk); timing baseline."""
+    n = len(m)
+    m, pos = _d(m), _d(pos)
+    acc = np.zeros((n, 3))
+    cnt = C.c_int64()
+    rc = lib().orc_bh_acc_mt(n, G, eps, theta, _dp(m), _dp(pos), _dp(acc), C.byref(cnt), threads)
+    if rc:
+        raise RuntimeError(f"oracle bh_acc_mt failed: {rc}")
+    return acc, cnt.value
+
+
+def max_threads():
+    return lib().orc_max_threads()
 
 
 def bh_accepted(m, pos, theta, body):

@@ -337,3 +337,19 @@ def test_adjoint_vs_central_differences():
         mm[i] -= h
         fd = (loss(mp, x0, v0) - loss(mm, x0, v0)) / (2 * h)
         assert abs(fd - gm[i]) < 1e-6 * max(1, abs(fd))
+
+
+def test_allcore_baselines_agree_with_restatement():
+    """The all-core timing baselines compute the same quantities: the direct
+    sum to FP64 rounding of the softened restatement (different summation
+    order), the BH walk bit for bit with the same interaction count (same
+    tree, same per-body walk, bodies spread over threads)."""
+    m, pos, _ = capi.ic_uniform_ball(31, 500, 1e20, 1e22, 1e9, 1e3)
+    G, eps = 6.6743e-11, 1e6
+    ref = O.direct_acc(m, pos, G, eps, O.FORM_SOFT, dtype=np.float64)
+    mt = O.direct_acc_mt(m, pos, G, eps, threads=4)
+    assert np.abs(mt - ref).max() <= 1e-12 * np.abs(ref).max()
+    a1, c1 = O.bh_acc(m, pos, G, eps, 0.5)
+    a2, c2 = O.bh_acc_mt(m, pos, G, eps, 0.5, threads=4)
+    assert c1 == c2 and np.array_equal(a1, a2)
+    assert O.max_threads() >= 1
