@@ -141,8 +141,9 @@ void direct_forces(nb_system *s) {
         launch_update_fp64_split(s->stream, n, s->dw.partial64.p, s->dw.nsplit64, s->G, s->a.p, s->v.p, s->x.p,
                                  s->half_dt, s->dt, UPD_STORE);
     } else {
+        s->dw.terms.ensure((size_t)(fp64_terms_block(n) * 3 * n));
         ScopedTimer t(s->prof, "force", s->stream);
-        launch_force_fp64(s->stream, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->a.p);
+        launch_force_fp64(s->stream, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->a.p, s->dw.terms.p);
     }
     s->acc_valid = true;
 }
@@ -155,7 +156,7 @@ GraphKey graph_key(const nb_system *s) {
     GraphKey k;
     k.n = s->n;
     k.precision = s->precision;
-    k.ptrs = {s->x.p, s->v.p, s->a.p, s->m.p, s->dw.pos32.p, s->dw.partial.p, s->dw.partial64.p};
+    k.ptrs = {s->x.p, s->v.p, s->a.p, s->m.p, s->dw.pos32.p, s->dw.partial.p, s->dw.partial64.p, s->dw.terms.p};
     k.shape = {s->dw.nsplit, s->dw.ipl, s->dw.variant, s->dw.nsplit64, form_of(s)};
     k.scalars = {s->G, s->dt, s->half_dt, s->eps};
     return k;
@@ -196,7 +197,8 @@ void direct_kdk_steps(nb_system *s, uint64_t steps) {
         }
         {
             ScopedTimer t(s->prof, "force", s->stream);
-            launch_force_fp64(s->stream, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->a.p);
+            launch_force_fp64(s->stream, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->a.p,
+                              s->dw.terms.n >= (size_t)(fp64_terms_block(n) * 3 * n) ? s->dw.terms.p : nullptr);
         }
         launch_kick(s->stream, n, s->a.p, s->v.p, s->half_dt);
         return true;

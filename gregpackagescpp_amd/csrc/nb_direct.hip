@@ -15,7 +15,9 @@
 //    the split partials are reduced in FP64 by k_update_mixed.
 //  * k_force_fp64: FP64, one lane per i, j ascending, FP contraction off, the
 //    exact operation order of the reference's pairwise update (so results are
-//    bit-identical to an IEEE-double restatement of it).
+//    bit-identical to an IEEE-double restatement of it).  For small n the
+//    same sums run in two phases (k_pair_terms, k_ordered_sum) so that the
+//    pair terms use the whole GPU.
 #include <algorithm>
 
 #include "nb_internal.hpp"
@@ -29,6 +31,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kTile = 64;  // j-bodies summed in plain FP32 before each Kahan fold
+constexpr int kTermJ = 8;  // j per lane in k_pair_terms
 
 __global__ void k_pack(int64_t n, const double *__restrict__ x, const double *__restrict__ m, double G,
                        float4 *__restrict__ pos32) {
@@ -222,6 +225,36 @@ __global__ void k_update_mixed(int64_t n, const float *__restrict__ partial, int
 // FP64 pairwise force, reference operation order: for body i the contributions
 // of j = 0..n-1 (j != i) are added in ascending j, which is the order in which
 // the reference's i<j double loop delivers them (gregsys.hpp:477-497).
+// Contribution of body j to a_i in the reference's pairwise update, before it
+// is added to the running sum (the reference forms the term, then adds it).
+template <int FORM>
+__device__ __forceinline__ void ref_pair_term(double xi, double yi, double zi, double xj, double yj, double zj,
+                                              double mj, double G, double eps2, double &tx, double &ty, double &tz) {
+    const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+    if (FORM == 0) {
+        // cumulative_acc: r^3 = |(d.d) d|, k = (G m_j) / r^3
+        const double s = dx * dx + dy * dy + dz * dz;
+        const double wx = s * dx, wy = s * dy, wz = s * dz;
+        const double r3 = __builtin_sqrt(wx * wx + wy * wy + wz * wz);
+        const double k = (G * mj) / r3;
+        tx = k * dx;
+        ty = k * dy;
+        tz = k * dz;
+    } else {
+        // bh_acc form: G*(m/s) * (d/sqrt(s)), s = d.d + eps^2
+        const double s = dx * dx + dy * dy + dz * dz + eps2;
+        const double sq = __builtin_sqrt(s);
+        const double k = G * (mj / s);
+        tx = k * (dx / sq);
+        ty = k * (dy / sq);
+        tz = k * (dz / sq);
+    }
+}
+
+// FP64 pairwise force, reference operation order: for body i the contributions
+// of j = 0..n-1 (j != i) are added in ascending j, which is the order in which
+// the reference's i<j double loop delivers them (gregsys.hpp:477-497).  One
+// lane per i; used for large n, where n lanes fill the GPU.
 template <int FORM>
 __global__ __launch_bounds__(kBlock) void k_force_fp64(int64_t n, const double *__restrict__ x,
                                                        const double *__restrict__ m, double G, double eps2,
@@ -232,29 +265,55 @@ __global__ __launch_bounds__(kBlock) void k_force_fp64(int64_t n, const double *
     const double xi = x[ii], yi = x[n + ii], zi = x[2 * n + ii];
     double ax = 0.0, ay = 0.0, az = 0.0;
     for (int64_t j = 0; j < n; ++j) {
-        const double xj = x[j], yj = x[n + j], zj = x[2 * n + j], mj = m[j];
         if (j == ii) continue;
-        double dx = xj - xi, dy = yj - yi, dz = zj - zi;
-        if (FORM == 0) {
-            // cumulative_acc: r^3 = |(d.d) d|, k = (G m_j) / r^3
-            double s = dx * dx + dy * dy + dz * dz;
-            double wx = s * dx, wy = s * dy, wz = s * dz;
-            double r3 = __builtin_sqrt(wx * wx + wy * wy + wz * wz);
-            double k = (G * mj) / r3;
-            ax += k * dx; ay += k * dy; az += k * dz;
-        } else {
-            // bh_acc form: G*(m/s) * (d/sqrt(s)), s = d.d + eps^2
-            double s = dx * dx + dy * dy + dz * dz + eps2;
-            double sq = __builtin_sqrt(s);
-            double k = G * (mj / s);
-            ax += k * (dx / sq); ay += k * (dy / sq); az += k * (dz / sq);
-        }
+        double tx, ty, tz;
+        ref_pair_term<FORM>(xi, yi, zi, x[j], x[n + j], x[2 * n + j], m[j], G, eps2, tx, ty, tz);
+        ax += tx;
+        ay += ty;
+        az += tz;
     }
     if (live) {
         a[i] = ax;
         a[n + i] = ay;
         a[2 * n + i] = az;
     }
+}
+
+// The same sums in two phases, for small n (where one lane per i leaves the
+// GPU idle): k_pair_terms evaluates the terms of a block of j for all i in
+// parallel into terms[j - j0][3][n] (0 for j == i), then k_ordered_sum adds
+// them to the running sums in ascending j, one lane per (component, i).  The
+// additions happen in the same order on the same values, so the result is
+// bit-identical to k_force_fp64 (adding the +0 of j == i leaves a sum that
+// started at +0 unchanged).
+template <int FORM>
+__global__ __launch_bounds__(kBlock) void k_pair_terms(int64_t n, const double *__restrict__ x,
+                                                       const double *__restrict__ m, double G, double eps2,
+                                                       int64_t j0, int64_t jcnt, double *__restrict__ terms) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int64_t jb = j0 + (int64_t)blockIdx.y * kTermJ;
+    const int64_t je = min(j0 + jcnt, jb + kTermJ);
+    const double xi = x[i], yi = x[n + i], zi = x[2 * n + i];
+    for (int64_t j = jb; j < je; ++j) {
+        double tx = 0.0, ty = 0.0, tz = 0.0;
+        if (j != i) ref_pair_term<FORM>(xi, yi, zi, x[j], x[n + j], x[2 * n + j], m[j], G, eps2, tx, ty, tz);
+        double *t = terms + (size_t)(j - j0) * 3 * n + i;
+        t[0] = tx;
+        t[n] = ty;
+        t[2 * n] = tz;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ordered_sum(int64_t n3, int64_t jcnt, const double *__restrict__ terms,
+                                                        double *__restrict__ a, int first) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n3) return;
+    double s = first ? 0.0 : a[t];
+    const double *p = terms + t;
+#pragma unroll 16
+    for (int64_t j = 0; j < jcnt; ++j) s += p[(size_t)j * n3];
+    a[t] = s;
 }
 
 // FP64 pair forces with the j range split twice: over gridDim.y, and inside a
@@ -453,13 +512,32 @@ void launch_update_mixed(hipStream_t s, int64_t n, const float *partial, int nsp
     NB_HIP(hipGetLastError());
 }
 
+int64_t fp64_terms_block(int64_t n) {
+    if (n <= 1 || n > kTermsMaxBodies) return 0;
+    return std::min<int64_t>(n, std::max<int64_t>(kTermJ, kTermsBudget / (3 * n)));
+}
+
 void launch_force_fp64(hipStream_t s, int64_t n, const double *x, const double *m, double G, double eps2, int form,
-                       double *a) {
+                       double *a, double *terms) {
     if (n <= 0) return;
-    if (form == 0)
+    const int64_t jb = fp64_terms_block(n);
+    if (jb > 0 && terms) {
+        const int64_t n3 = 3 * n;
+        for (int64_t j0 = 0; j0 < n; j0 += jb) {
+            const int64_t cnt = std::min<int64_t>(jb, n - j0);
+            const dim3 grid(blocks_for(n), (unsigned)((cnt + kTermJ - 1) / kTermJ));
+            if (form == 0)
+                hipLaunchKernelGGL(k_pair_terms<0>, grid, dim3(kBlock), 0, s, n, x, m, G, eps2, j0, cnt, terms);
+            else
+                hipLaunchKernelGGL(k_pair_terms<1>, grid, dim3(kBlock), 0, s, n, x, m, G, eps2, j0, cnt, terms);
+            hipLaunchKernelGGL(k_ordered_sum, dim3(blocks_for(n3)), dim3(kBlock), 0, s, n3, cnt, terms, a,
+                               j0 == 0 ? 1 : 0);
+        }
+    } else if (form == 0) {
         hipLaunchKernelGGL(k_force_fp64<0>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, x, m, G, eps2, a);
-    else
+    } else {
         hipLaunchKernelGGL(k_force_fp64<1>, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, x, m, G, eps2, a);
+    }
     NB_HIP(hipGetLastError());
 }
 

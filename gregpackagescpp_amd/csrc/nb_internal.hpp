@@ -135,7 +135,7 @@ constexpr int64_t kGraphMaxBodies = 16384;
 struct GraphKey {
     int64_t n = -1;
     int precision = -1;
-    std::array<const void *, 7> ptrs{};
+    std::array<const void *, 8> ptrs{};
     std::array<int, 5> shape{};
     std::array<double, 4> scalars{};
     bool operator==(const GraphKey &o) const {
@@ -151,6 +151,7 @@ struct DirectWork {
     int ipl = 2;           // i-bodies per lane
     int variant = 1;       // 0: j stream through scalar loads, 1: j tiles through LDS
     DBuf<double> partial64;  // [split][3][n] for the FP64 split kernels
+    DBuf<double> terms;      // [j block][3][n] pair terms of the FP64 reference-order path
     int nsplit64 = 1;
 };
 
@@ -163,8 +164,14 @@ void launch_force_mixed(hipStream_t s, int64_t n, const float4 *pos32, float eps
 // (v += half_dt*a; x += dt*v) and the FP32 repack follow in the same pass.
 void launch_update_mixed(hipStream_t s, int64_t n, const float *partial, int nsplit, double *a, double *v, double *x,
                          const double *m, double G, double half_dt, double dt, int mode, float4 *pos32);
+// FP64 reference-order forces.  For n <= kTermsMaxBodies (and a terms buffer of
+// fp64_terms_block(n) * 3n doubles) the pair terms are evaluated in parallel
+// and summed in order by a second kernel; otherwise one lane per i.
+constexpr int64_t kTermsMaxBodies = 16384;
+constexpr int64_t kTermsBudget = int64_t(32) << 20;  // doubles (256 MB)
+int64_t fp64_terms_block(int64_t n);  // j per term block, 0 = one-lane-per-i kernel
 void launch_force_fp64(hipStream_t s, int64_t n, const double *x, const double *m, double G, double eps2, int form,
-                       double *a);
+                       double *a, double *terms);
 void launch_kick(hipStream_t s, int64_t n, const double *a, double *v, double coef);
 void launch_kick_drift(hipStream_t s, int64_t n, const double *a, double *v, double *x, double half_dt, double dt);
 void launch_drift(hipStream_t s, int64_t n, const double *v, double *x, double dt);

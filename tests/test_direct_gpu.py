@@ -222,3 +222,18 @@ def test_step_graphs_match_eager(prec):
         ref = O.kdk(m, pos, vel, G_SI, 10.0, 100, eps=1e6, form=O.FORM_SOFT, dtype=np.float64)
         assert np.array_equal(s.state()["pos"], ref["pos"])
         assert np.array_equal(s.state()["vel"], ref["vel"])
+
+
+def test_fp64_one_lane_kernel_above_term_path():
+    """Above 16,384 bodies the FP64 reference-order path runs one lane per i
+    (below, the pair terms are evaluated in parallel and summed in order by a
+    second kernel); both must equal the double restatement bit for bit."""
+    n = 16400
+    print(f"case fp64 one-lane n={n}", flush=True)
+    m, pos, vel = capi.ic_uniform_ball(14, n, 1e20, 1e22, 1e9, 1e3)
+    s = make_system(m, pos, vel, 10.0, 1, 1e6)
+    s.evolve(capi.LEAPFROG_KDK)
+    st = s.state()
+    ref = O.kdk(m, pos, vel, G_SI, 10.0, 1, eps=1e6, form=O.FORM_SOFT, dtype=np.float64)
+    assert np.array_equal(st["pos"], ref["pos"])
+    assert np.array_equal(st["acc"], ref["acc"])
