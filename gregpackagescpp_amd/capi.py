@@ -28,6 +28,8 @@ NB_ERR_NO_EVOLUTION = -9
 NB_ERR_NOT_SUPPORTED = -10
 NB_ERR_CAPACITY = -11
 NB_ERR_DEVICE = -12
+NB_ERR_NSYS_LOAD = -13
+NB_ERR_NSYS_FORMAT = -14
 
 TWO_BODY = 1
 EULER = 2
@@ -44,6 +46,16 @@ _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _I64 = C.POINTER(C.c_int64)
 _I32 = C.POINTER(C.c_int32)
+
+
+class NsysInfo(C.Structure):
+    """nb_nsys_info: the .nsys header as the engine reports it."""
+    _fields_ = [("d_types", C.c_int32), ("time_size", C.c_int32), ("m_size", C.c_int32), ("r_size", C.c_int32),
+                ("t_size", C.c_int32), ("rest_size", C.c_int32), ("g_vals", C.c_uint64 * 6),
+                ("num_bodies", C.c_uint64), ("time_elapsed", C.c_double), ("dt", C.c_double)]
+
+
+_NI = C.POINTER(NsysInfo)
 
 # name -> (restype, argtypes); every symbol declared in include/gregnbody.h
 SIGNATURES = {
@@ -70,6 +82,11 @@ SIGNATURES = {
     "nb_tree_topology": (C.c_int, [_P, C.c_int64, _I32, _I32, _I64, _I64, _D, _D, _D, _I64]),
     "nb_bh_accelerations": (C.c_int, [_P, _D, _I64]),
     "nb_adjoint": (C.c_int, [_P, C.c_uint64, C.c_uint64, _D, _D, _D, _D, _D, _I64, _I64]),
+    "nb_nsys_read": (C.c_int, [C.c_char_p, C.c_int, _NI, C.c_int64, _D, _D, _D, _D, _D]),
+    "nb_nsys_write": (C.c_int, [C.c_char_p, _NI, C.c_int64, _D, _D, _D, _D, _D, C.c_int, _I64]),
+    "nb_load_nsys": (C.c_int, [_P, C.c_char_p, C.c_int, C.c_int]),
+    "nb_save_nsys": (C.c_int, [_P, C.c_char_p, C.c_int, _I64]),
+    "nb_time_elapsed": (C.c_int, [_P, _D]),
     "nb_set_tuning": (C.c_int, [_P, C.c_char_p, C.c_int64]),
     "nb_get_tuning": (C.c_int, [_P, C.c_char_p, _I64]),
     "nb_autotune": (C.c_int, [_P, C.c_int, C.c_char_p]),
@@ -286,6 +303,21 @@ class System:
                                     C.byref(b)))
         return gx0, gv0, gm, f.value, b.value
 
+    def load_nsys(self, path, check_nan_inf=True, allow_overlapping=True):
+        """Replace bodies, dt, time elapsed and G ratios from an .nsys snapshot."""
+        self._ck(self._L.nb_load_nsys(self.h, os.fsencode(path), int(check_nan_inf), int(allow_overlapping)))
+
+    def save_nsys(self, path, value_size=16) -> int:
+        """Write the current state as an .nsys snapshot; returns the file size."""
+        b = C.c_int64()
+        self._ck(self._L.nb_save_nsys(self.h, os.fsencode(path), int(value_size), C.byref(b)))
+        return b.value
+
+    def time_elapsed(self) -> float:
+        t = C.c_double()
+        self._ck(self._L.nb_time_elapsed(self.h, C.byref(t)))
+        return t.value
+
     def set_tuning(self, key, value):
         self._ck(self._L.nb_set_tuning(self.h, key.encode(), int(value)))
 
@@ -363,6 +395,41 @@ class Batch:
         rec = np.zeros((max(cnt.value, 1), 5), dtype=np.int64)
         self._ck(self._L.nb_batch_merge_log(self.h, cnt.value, ip(rec), C.byref(cnt)))
         return rec[: cnt.value]
+
+
+# ---- .nsys snapshots without a context (host only) ----
+def _static_ck(rc):
+    if rc != NB_OK:
+        raise NBodyError(rc, lib().nb_last_error(None).decode())
+
+
+def nsys_read(path, check_nan_inf=True):
+    """Parse an .nsys file: dict(info=NsysInfo, m, r, pos, vel, rest_c)."""
+    L = lib()
+    info = NsysInfo()
+    rc = L.nb_nsys_read(os.fsencode(path), int(check_nan_inf), C.byref(info), 0, None, None, None, None, None)
+    if rc not in (NB_OK, NB_ERR_CAPACITY):
+        _static_ck(rc)
+    n = int(info.num_bodies)
+    m, r, rest = np.zeros(n), np.zeros(n), np.zeros(n)
+    pos, vel = np.zeros((n, 3)), np.zeros((n, 3))
+    _static_ck(L.nb_nsys_read(os.fsencode(path), int(check_nan_inf), C.byref(info), n, dp(m), dp(r), dp(pos),
+                              dp(vel), dp(rest)))
+    return dict(info=info, m=m, r=r, pos=pos, vel=vel, rest_c=rest)
+
+
+def nsys_write(path, m, r, pos, vel, rest_c, time_elapsed=0.0, dt=1.0, g_vals=(1, 1, 1, 1, 1, 1),
+               value_size=16) -> int:
+    """Write an .nsys file (M = R = T of value_size bytes); returns its size."""
+    m, r, pos, vel, rest_c = map(_c64, (m, r, pos, vel, rest_c))
+    info = NsysInfo()
+    info.time_elapsed, info.dt = time_elapsed, dt
+    for k in range(6):
+        info.g_vals[k] = int(g_vals[k])
+    b = C.c_int64()
+    _static_ck(lib().nb_nsys_write(os.fsencode(path), C.byref(info), len(m), dp(m), dp(r), dp(pos), dp(vel),
+                                   dp(rest_c), int(value_size), C.byref(b)))
+    return b.value
 
 
 # ---- seeded initial conditions ----

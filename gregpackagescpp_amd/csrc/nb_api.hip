@@ -24,6 +24,15 @@ namespace {
 
 thread_local std::string g_static_err;
 
+}  // namespace
+
+namespace nbe {
+// message for nb_last_error(NULL) from host-only entry points in other files
+void set_static_error(const std::string &msg) { g_static_err = msg; }
+}  // namespace nbe
+
+namespace {
+
 int fail(nb_system *s, int code, const std::string &msg) {
     if (s) s->err = msg;
     else g_static_err = msg;
@@ -328,6 +337,7 @@ int nb_set_bodies(nb_system *s, int64_t n, const double *m, const double *r, con
         s->acc_valid = false;
         s->step_count = 0;
         s->merge_log.clear();
+        s->time_elapsed = 0;
         return NB_OK;
     });
 }
@@ -480,6 +490,7 @@ int nb_evolve(nb_system *s, int method, int64_t *elapsed_ns) {
         NB_HIP(hipStreamSynchronize(s->stream));
         s->prof.collect();
         s->evolved = true;
+        s->time_elapsed = (long double)s->iterations * (long double)s->dt;  // evolve :3069
         auto t1 = std::chrono::high_resolution_clock::now();
         if (elapsed_ns) *elapsed_ns = (int64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
         return NB_OK;
@@ -495,6 +506,7 @@ int nb_advance(nb_system *s, uint64_t steps, int64_t *elapsed_ns) {
         auto t0 = std::chrono::high_resolution_clock::now();
         if (s->acc_method == NB_BARNES_HUT) bh_steps(s, steps);
         else direct_kdk_steps(s, steps);
+        s->time_elapsed += (long double)steps * (long double)s->dt;
         NB_HIP(hipStreamSynchronize(s->stream));
         s->prof.collect();
         auto t1 = std::chrono::high_resolution_clock::now();

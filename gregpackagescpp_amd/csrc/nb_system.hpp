@@ -35,6 +35,7 @@ struct nb_system {
     bool acc_valid = false;
     int acc_method = 0;  // method whose accelerations are cached
     uint64_t step_count = 0;
+    long double time_elapsed = 0;  // system::time_elapsed (gregsys.hpp:206)
     std::vector<int64_t> merge_log;  // 4 per record
 
     // tuning knobs (0 = engine default); set by nb_set_tuning / nb_autotune
@@ -62,6 +63,7 @@ void direct_forces(nb_system *s);                    // a(x) into s->a (and pos3
 void direct_kdk_steps(nb_system *s, uint64_t steps); // steps of KDK from valid a
 int form_of(const nb_system *s);                     // 0 = unsoftened reference form, 1 = softened
 GraphKey graph_key(const nb_system *s);
+void set_static_error(const std::string &msg);  // message for nb_last_error(NULL)
 // collisions after a step's closing kick (nb_coll.hip)
 void collision_pass(nb_system *s, uint64_t step);
 // Barnes-Hut (nb_bh.hip)

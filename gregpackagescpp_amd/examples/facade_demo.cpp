@@ -57,6 +57,26 @@ int main() {
     } catch (const gtd::overlapping_bodies_error &e) {
         std::printf("overlapping bodies rejected: %s\n", e.what());
     }
+    // .nsys snapshot: to_nsys, then the .nsys constructor (the sl9.cpp frame
+    // pattern: write a frame, reload it later)
+    const char *snap = "facade_demo_snapshot.nsys";
+    const std::streamoff bytes = sys.to_nsys(snap);
+    sys_t again{snap};
+    std::remove(snap);
+    if (bytes != 96 + 2 * 80 || again.num_bodies() != 2 || again.elapsed_time() != sys.elapsed_time()) return 5;
+    for (int k = 0; k < 2; ++k)
+        if (again[k].pos().get_x() != sys[k].pos().get_x() || again[k].vel().get_y() != sys[k].vel().get_y() ||
+            again[k].mass() != sys[k].mass())
+            return 6;
+    std::printf(".nsys snapshot: %lld bytes, time elapsed %.6e s, reloaded identically\n", (long long)bytes,
+                (double)again.elapsed_time());
+    try {
+        sys_t missing{"no_such_file.nsys"};
+        return 7;
+    } catch (const gtd::nsys_load_error &e) {
+        std::printf("missing snapshot rejected: %s", e.what());
+    }
+
     std::printf("facade demo OK\n");
     return 0;
 }

@@ -39,6 +39,8 @@ extern "C" {
 #define NB_ERR_NOT_SUPPORTED (-10)        /* method outside this engine's path     */
 #define NB_ERR_CAPACITY (-11)             /* caller buffer or engine limit too small */
 #define NB_ERR_DEVICE (-12)               /* HIP runtime failure                   */
+#define NB_ERR_NSYS_LOAD (-13)            /* nsys_load_error                       */
+#define NB_ERR_NSYS_FORMAT (-14)          /* nsys_format_error                     */
 
 /* ---- integration method bit flags: gtd::system constants gregsys.hpp:156-165 ---- */
 #define NB_TWO_BODY 1
@@ -140,6 +142,49 @@ int nb_bh_accelerations(nb_system *sys, double *acc, int64_t *interactions);
  * recomputing each segment forward.  The state is left at time T. */
 int nb_adjoint(nb_system *sys, uint64_t steps, uint64_t ckpt_every, const double *dl_dx_T, const double *dl_dv_T,
                double *dl_dx0, double *dl_dv0, double *dl_dm, int64_t *fwd_ns, int64_t *bwd_ns);
+
+/* ---- .nsys snapshots (system::to_nsys gregsys.hpp:3214-3258, the .nsys
+ *      constructor :1479-1490 and load_from_nsys :943-1400) ----
+ * Layout: a 96-byte header -- "NS", d_types (bit 0 float time, bits 1-3 float
+ * M/R/T, bits 4-6 signed integral M/R/T), time_size, time elapsed and dt as
+ * time_size-byte values in 16-byte fields, the sizes of M, R, T and of the
+ * restitution coefficient, the six G-ratio integers (m_num, m_denom, d_num,
+ * d_denom, t_num, t_denom) and the body count -- then one chunk per body:
+ * restitution (16-byte field), mass, radius, x, y, z, vx, vy, vz in their
+ * declared types and natural alignment; the file is zero-padded to a multiple
+ * of 4 bytes.  Values travel as FP64 through this API (x87 long double fields
+ * are converted on the host). */
+typedef struct nb_nsys_info {
+    int32_t d_types, time_size, m_size, r_size, t_size, rest_size;
+    uint64_t g_vals[6];
+    uint64_t num_bodies;
+    double time_elapsed;
+    double dt;
+} nb_nsys_info;
+
+/* Parse an .nsys file without a context (host only).  Fills *info, then, if
+ * cap >= num_bodies, the bodies (pos/vel interleaved xyz).  Errors mirror
+ * load_from_nsys: NB_ERR_NSYS_LOAD / NB_ERR_NSYS_FORMAT with the reference's
+ * messages (nb_last_error(NULL)), NB_ERR_INVALID_ARGUMENT for dt <= 0
+ * (check_dt :1401-1405), NB_ERR_CAPACITY when cap is too small (info is
+ * still filled). */
+int nb_nsys_read(const char *path, int check_nan_inf, nb_nsys_info *info, int64_t cap, double *m, double *r,
+                 double *pos, double *vel, double *rest_c);
+/* Write an .nsys file (host only) with mass, radius, position and velocity
+ * all of value_size bytes: 8 (double, 80-byte chunks) or 16 (x87 long double,
+ * 144-byte chunks); time and restitution as long double.  info supplies the
+ * time elapsed, dt and G ratios (other fields ignored).  *bytes = file size. */
+int nb_nsys_write(const char *path, const nb_nsys_info *info, int64_t n, const double *m, const double *r,
+                  const double *pos, const double *vel, const double *rest_c, int value_size, int64_t *bytes);
+/* The context forms: load replaces bodies, dt, time elapsed and the G ratios
+ * (G recomputed as parse_units_format does); allow_overlapping = 0 runs the
+ * construction overlap check.  save writes the current state (to_nsys). */
+int nb_load_nsys(nb_system *sys, const char *path, int check_nan_inf, int allow_overlapping);
+int nb_save_nsys(const nb_system *sys, const char *path, int value_size, int64_t *bytes);
+/* Time elapsed (system::elapsed_time :1630-1632): set to iterations*dt by
+ * nb_evolve, advanced by steps*dt by nb_advance, 0 after nb_set_bodies, read
+ * from the file by nb_load_nsys. */
+int nb_time_elapsed(const nb_system *sys, double *t);
 
 /* Tuning knobs (engine extension; 0 restores the default):
  *   "direct_kernel" FP32 direct kernel: 1 j stream through scalar loads,

@@ -21,6 +21,8 @@
 
 #include <chrono>
 #include <cstdint>
+#include <cstdio>
+#include <ios>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -45,6 +47,8 @@ class no_evolution_error : public nbody_error { using nbody_error::nbody_error; 
 class barnes_hut_error : public nbody_error { using nbody_error::nbody_error; };
 class invalid_body_placement : public barnes_hut_error { using barnes_hut_error::barnes_hut_error; };
 class indistinguishable_bodies : public barnes_hut_error { using barnes_hut_error::barnes_hut_error; };
+class nsys_load_error : public nbody_error { using nbody_error::nbody_error; };
+class nsys_format_error : public nsys_load_error { using nsys_load_error::nsys_load_error; };
 class device_error : public std::runtime_error { using std::runtime_error::runtime_error; };
 
 namespace detail {
@@ -60,6 +64,8 @@ inline void raise(int code, const char *msg) {
         case NB_ERR_INVALID_PLACEMENT: throw invalid_body_placement(msg);
         case NB_ERR_INDISTINGUISHABLE: throw indistinguishable_bodies(msg);
         case NB_ERR_NO_EVOLUTION: throw no_evolution_error(msg);
+        case NB_ERR_NSYS_LOAD: throw nsys_load_error(msg);
+        case NB_ERR_NSYS_FORMAT: throw nsys_format_error(msg);
         case NB_ERR_DEVICE: throw device_error(msg);
         default: throw nbody_error(msg);
     }
@@ -132,6 +138,25 @@ class system {
         }
         stale_ = false;
     }
+    static void type_check(int is_fp, int is_signed, int size, const char *type, const char *what) {
+        (void)is_signed;
+        char msg[256];
+        if (!is_fp) {
+            // the reference's wording for this case (NSYS_TYPE_CHECK, floating template parameter)
+            snprintf(msg, sizeof msg,
+                     "Invalid .nsys format: d_types field states that %s values are stored in floating-point format, "
+                     "but template parameter %s is integral.\n",
+                     what, type);
+            throw nsys_load_error(msg);
+        }
+        if (size != (int)sizeof(double)) {
+            snprintf(msg, sizeof msg,
+                     "Size mismatch error: size of %s template parameter (%zu bytes) does not match its reported size "
+                     "(%d bytes) in the .nsys file.\n",
+                     type, sizeof(double), size);
+            throw nsys_load_error(msg);
+        }
+    }
     void upload(const std::vector<body> &bods, bool allow_overlapping) {
         const size_t n = bods.size();
         std::vector<double> m(n), r(n), e(n), p(3 * n), v(3 * n);
@@ -175,6 +200,24 @@ public:
         set_iterations(num_iterations);
         set_G_units(units_format);
         upload(bodies, allow_overlapping_bodies);
+    }
+    /* .nsys constructor (gregsys.hpp:1479-1490).  The stored types must be this
+     * system's (floating, 8 bytes), checked in load_from_nsys's order and words
+     * (NSYS_TYPE_CHECK :989-1046). */
+    explicit system(const char *nsys_path, bool alloc_chunks = true, bool check_nan_inf = true,
+                    bool allow_overlapping_bodies = true, int device = 0)
+        : system(device) {
+        (void)alloc_chunks;  // the engine reads the chunks in one pass either way
+        if (nsys_path == nullptr) throw std::invalid_argument{"nullptr passed as .nsys file path.\n"};
+        nb_nsys_info info{};
+        const int rc = nb_nsys_read(nsys_path, check_nan_inf, &info, 0, nullptr, nullptr, nullptr, nullptr, nullptr);
+        if (rc != NB_OK && rc != NB_ERR_CAPACITY) detail::raise(rc, nb_last_error(nullptr));
+        type_check(info.d_types & 0b10, info.d_types & 0b10000, info.m_size, "M", "mass");
+        type_check(info.d_types & 0b100, info.d_types & 0b100000, info.r_size, "R", "radius");
+        type_check(info.d_types & 0b1000, info.d_types & 0b1000000, info.t_size, "T", "position and velocity");
+        check(nb_load_nsys(h_, nsys_path, check_nan_inf, allow_overlapping_bodies));
+        check(nb_set_iterations(h_, iterations_));
+        stale_ = true;
     }
     system(const system &) = delete;
     system &operator=(const system &) = delete;
@@ -238,6 +281,20 @@ public:
         }
         check(nb_set_iterations(h_, saved));
         return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::high_resolution_clock::now() - t0);
+    }
+
+    /* to_nsys (gregsys.hpp:3214-3258): the file size, or 0 if it could not be
+     * written. */
+    std::streamoff to_nsys(const char *path) const {
+        if (path == nullptr) return 0;
+        int64_t bytes = 0;
+        if (nb_save_nsys(h_, path, (int)sizeof(T), &bytes) != NB_OK) return 0;
+        return (std::streamoff)bytes;
+    }
+    long double elapsed_time() const {
+        double t = 0;
+        check(nb_time_elapsed(h_, &t));
+        return t;
     }
 
     /* energies of the current state (FP64) */

@@ -323,3 +323,53 @@ def read_ref_tree(path):
     rec = np.frombuffer(raw[8:], dtype=np.dtype([("d", "<i8"), ("o", "<i8"), ("b", "<i8"), ("m", "<f8")]))
     assert rec.shape[0] == cnt
     return rec
+
+
+# ---------------- .nsys snapshot layout (restatement) ----------------
+# nsys_file_header / nsys_file_chunk (gregsys.hpp:1444-1477), written by
+# system::to_nsys (:3214-3258): 96-byte header, then per body a 16-byte
+# restitution field and mass, radius, x, y, z, vx, vy, vz in their declared
+# types with natural alignment, file padded to a multiple of 4 bytes.  Only
+# floating-point M/R/T (4, 8, 16 bytes) are restated here.
+_NSYS_FLOAT = {4: np.float32, 8: np.float64, 16: LD}
+
+
+def parse_nsys(path):
+    raw = open(path, "rb").read()
+    h = raw[:96]
+    out = {"signature": h[:2], "d_types": h[2], "time_size": h[3],
+           "m_size": h[36], "r_size": h[37], "t_size": h[38], "rest_size": h[39]}
+    out["g_vals"] = np.frombuffer(h[40:88], np.uint64).copy()
+    n = int(np.frombuffer(h[88:96], np.uint64)[0])
+    out["num_bodies"] = n
+    tt = _NSYS_FLOAT[h[3]]
+    out["time_elapsed"] = LD(np.frombuffer(h[4:4 + h[3]], tt)[0])
+    out["dt"] = LD(np.frombuffer(h[20:20 + h[3]], tt)[0])
+    sizes = [h[36], h[37], h[38]]
+    offs, off = [], 16
+    for sz in sizes:
+        off = (off + sz - 1) // sz * sz
+        offs.append(off)
+        off += sz if len(offs) < 3 else 6 * sz
+    chunk = (off + max(sizes) - 1) // max(sizes) * max(sizes)
+    body = n * chunk
+    out["expected_size"] = 96 + body + ((4 - body % 4) % 4)
+    out["file_size"] = len(raw)
+    rest, m, r = np.zeros(n, LD), np.zeros(n, LD), np.zeros(n, LD)
+    pos, vel = np.zeros((n, 3), LD), np.zeros((n, 3), LD)
+    rt = _NSYS_FLOAT[h[39]]
+    for i in range(n):
+        c = raw[96 + i * chunk: 96 + (i + 1) * chunk]
+        rest[i] = np.frombuffer(c[:h[39]], rt)[0]
+        m[i] = np.frombuffer(c[offs[0]:offs[0] + sizes[0]], _NSYS_FLOAT[sizes[0]])[0]
+        r[i] = np.frombuffer(c[offs[1]:offs[1] + sizes[1]], _NSYS_FLOAT[sizes[1]])[0]
+        t = np.frombuffer(c[offs[2]:offs[2] + 6 * sizes[2]], _NSYS_FLOAT[sizes[2]])
+        pos[i], vel[i] = t[:3], t[3:]
+    out.update(rest_c=rest, m=m, r=r, pos=pos, vel=vel, chunk=chunk)
+    return out
+
+
+def ref_nsys_copy(src, dst, value_type="ld"):
+    """The compiled reference reads src with its .nsys constructor and writes
+    it back with to_nsys (system<T,T,T>, T = long double or double)."""
+    run_ref("nsys_copy" if value_type == "ld" else "nsys_copy_d", src, dst)

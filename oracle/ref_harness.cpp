@@ -15,6 +15,12 @@
  *   mode tree : bh_tree<double,double,double,0>::add_bodies (greg8tree.hpp:1066)
  *               and its operator<< dump (:975-1001) parsed to DFS pre-order
  *               records (depth, octant, body index, mass).
+ *   mode nsys_ld / nsys_d : system<T,T,T> (T = long double / double) built
+ *               from in.bin with units "M3:2,D5:7,T11:13", evolved `steps`
+ *               KDK steps when steps > 0, then to_nsys(out) (gregsys.hpp:3214).
+ *   mode nsys_copy / nsys_copy_d : the .nsys constructor (gregsys.hpp:1479)
+ *               on the file given in place of in.bin, then to_nsys(out): the
+ *               reference reads a snapshot and writes it back.
  * Input (little endian): int64 n, int64 steps, double dt, then n records of
  * 9 doubles (m, r, e, x, y, z, vx, vy, vz).
  */
@@ -177,15 +183,54 @@ static int run_tree(const In &in, FILE *out) {
     return 0;
 }
 
+template <typename T>
+static int run_nsys_write(const In &in, const char *out_path) {
+    using bod_t = gtd::body<T, T, T, 0>;
+    std::vector<bod_t> bods;
+    bods.reserve((size_t)in.n);
+    for (int64_t i = 0; i < in.n; ++i) {
+        const double *r = &in.rec[(size_t)i * 9];
+        bods.emplace_back((T)r[0], (T)r[1], gtd::vector3D<T>{(T)r[3], (T)r[4], (T)r[5]},
+                          gtd::vector3D<T>{(T)r[6], (T)r[7], (T)r[8]}, (ld)r[2]);
+    }
+    gtd::system<T, T, T, false, false, 0, 0, 0, false> sys{bods, (ld)in.dt, (uint64_t)std::max<int64_t>(in.steps, 1),
+                                                           "M3:2,D5:7,T11:13", true};
+    if (in.steps > 0) sys.evolve(gtd::system<T, T, T, false, false, 0, 0, 0, false>::leapfrog_kdk);
+    return sys.to_nsys(out_path) > 0 ? 0 : 3;
+}
+
+template <typename T>
+static int run_nsys_copy(const char *in_path, const char *out_path) {
+    gtd::system<T, T, T, false, false, 0, 0, 0, false> sys{in_path};
+    return sys.to_nsys(out_path) > 0 ? 0 : 3;
+}
+
 int main(int argc, char **argv) {
     if (argc != 4) {
-        fprintf(stderr, "usage: %s kdk|coll|tree in.bin out.bin\n", argv[0]);
+        fprintf(stderr, "usage: %s kdk|coll|tree|nsys_ld|nsys_d|nsys_copy|nsys_copy_d in out\n", argv[0]);
         return 2;
+    }
+    const std::string m0 = argv[1];
+    if (m0 == "nsys_copy" || m0 == "nsys_copy_d") {
+        try {
+            return m0 == "nsys_copy" ? run_nsys_copy<ld>(argv[2], argv[3]) : run_nsys_copy<double>(argv[2], argv[3]);
+        } catch (const std::exception &e) {
+            fprintf(stderr, "reference threw: %s\n", e.what());
+            return 4;
+        }
     }
     In in;
     if (!read_in(argv[2], in)) {
         fprintf(stderr, "cannot read %s\n", argv[2]);
         return 2;
+    }
+    if (m0 == "nsys_ld" || m0 == "nsys_d") {
+        try {
+            return m0 == "nsys_ld" ? run_nsys_write<ld>(in, argv[3]) : run_nsys_write<double>(in, argv[3]);
+        } catch (const std::exception &e) {
+            fprintf(stderr, "reference threw: %s\n", e.what());
+            return 4;
+        }
     }
     FILE *out = fopen(argv[3], "wb");
     if (!out) return 2;

@@ -16,6 +16,7 @@ Usage: python tests/golden/make_fixtures.py
 """
 import os
 import sys
+import tempfile
 
 import numpy as np
 
@@ -27,7 +28,6 @@ import oracle as O  # noqa: E402
 from gregpackagescpp_amd import capi  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
-TMP = os.path.join(ROOT, "_work", "fixtures_tmp")
 
 KDK_CASES = [(2, 1), (2, 1000), (3, 10), (64, 10), (64, 1000), (1024, 10), (1024, 1000)]
 TREE_CASES = [2, 3, 17, 1000, 16384]
@@ -48,8 +48,8 @@ def coll_ics(n, seed):
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref is not built (needs /root/reference): nothing generated")
-    os.makedirs(TMP, exist_ok=True)
-    inp, out = os.path.join(TMP, "in.bin"), os.path.join(TMP, "out.bin")
+    tmp = tempfile.mkdtemp()
+    inp, out = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
     for n, steps in KDK_CASES:
         m, pos, vel = capi.ic_uniform_ball(0, n, 1e20, 1e22, 1e9, 1e3)
         O.write_ref_input(inp, m, pos, vel, steps=steps, dt=10.0)
