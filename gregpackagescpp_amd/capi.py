@@ -33,6 +33,8 @@ NB_ERR_NSYS_FORMAT = -14
 
 TWO_BODY = 1
 EULER = 2
+MODIFIED_EULER = 4
+MIDPOINT = 8
 LEAPFROG_KDK = 16
 LEAPFROG_DKD = 32
 BARNES_HUT = 65536

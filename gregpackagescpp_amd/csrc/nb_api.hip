@@ -159,6 +159,70 @@ void direct_forces(nb_system *s) {
 
 // KDK steps from valid accelerations (evolve's KDK branch :2949-2968 with
 // leapfrog_kdk_acc_e_and_step :514-526 and s_coll when collisions are on).
+// a(x) for arbitrary positions (SoA, 3n) into `a`, in the context's precision
+// (the predicted-state forces of modified Euler and midpoint).
+void forces_at(nb_system *s, const double *x, double *a) {
+    const int64_t n = s->n;
+    ScopedTimer t(s->prof, "force", s->stream);
+    if (s->precision == NB_PREC_MIXED) {
+        s->dw.pos32.ensure((size_t)n);
+        s->dw.ipl = s->tune_direct_ipl > 0 ? s->tune_direct_ipl : 2;
+        s->dw.variant = s->tune_direct_kernel == 1 ? 0 : 1;
+        s->dw.nsplit = choose_nsplit(n, s->dw.ipl, s->tune_direct_waves);
+        s->dw.partial.ensure((size_t)s->dw.nsplit * 6 * (size_t)n);
+        launch_pack(s->stream, n, x, s->m.p, s->G, s->dw.pos32.p);
+        launch_force_mixed(s->stream, n, s->dw.pos32.p, (float)(s->eps * s->eps), form_of(s) == 1, s->dw.partial.p,
+                           s->dw.nsplit, s->dw.ipl, s->dw.variant);
+        launch_update_mixed(s->stream, n, s->dw.partial.p, s->dw.nsplit, a, s->v.p, s->x.p, s->m.p, s->G,
+                            s->half_dt, s->dt, UPD_STORE, s->dw.pos32.p);
+    } else if (s->precision == NB_PREC_FP64_FAST) {
+        s->dw.nsplit64 = choose_nsplit64(n);
+        s->dw.partial64.ensure((size_t)s->dw.nsplit64 * 3 * (size_t)n);
+        launch_force_fp64_split(s->stream, n, x, s->m.p, s->eps * s->eps, form_of(s) == 1, s->dw.partial64.p,
+                                s->dw.nsplit64);
+        launch_update_fp64_split(s->stream, n, s->dw.partial64.p, s->dw.nsplit64, s->G, a, s->v.p, s->x.p,
+                                 s->half_dt, s->dt, UPD_STORE);
+    } else {
+        s->dw.terms.ensure((size_t)(fp64_terms_block(n) * 3 * n));
+        launch_force_fp64(s->stream, n, x, s->m.p, s->G, s->eps * s->eps, form_of(s), a, s->dw.terms.p);
+    }
+}
+
+// Steps of evolve's other direct-sum integrators (gregsys.hpp:2666-2891 for
+// Euler, modified Euler and midpoint, :2993-3039 for leapfrog DKD), each step
+// followed by s_coll when collisions are on.  Every step evaluates its own
+// forces, so no cached accelerations carry over between calls.
+void direct_other_steps(nb_system *s, int method, uint64_t steps) {
+    const bool coll = s->coll_mode == NB_OVERLAP_COLL_CHECK;
+    for (uint64_t st = 1; st <= steps; ++st) {
+        const int64_t n = s->n;
+        if (n == 0) break;
+        const size_t n3 = 3 * (size_t)n;
+        if (method == NB_EULER) {
+            forces_at(s, s->x.p, s->a.p);
+            launch_euler(s->stream, n, s->a.p, s->v.p, s->x.p, s->dt);
+        } else if (method == NB_MODIFIED_EULER || method == NB_MIDPOINT) {
+            s->dw.px.ensure(n3);
+            s->dw.pv.ensure(n3);
+            s->dw.pa.ensure(n3);
+            forces_at(s, s->x.p, s->a.p);
+            const bool mid = method == NB_MIDPOINT;
+            launch_predict(s->stream, n, s->x.p, s->v.p, s->a.p, mid ? s->half_dt : s->dt, s->dw.px.p, s->dw.pv.p);
+            forces_at(s, s->dw.px.p, s->dw.pa.p);
+            if (mid)
+                launch_midpoint(s->stream, n, s->dw.pv.p, s->dw.pa.p, s->v.p, s->x.p, s->dt);
+            else
+                launch_modeuler(s->stream, n, s->a.p, s->dw.pv.p, s->dw.pa.p, s->v.p, s->x.p, s->half_dt);
+        } else {  // NB_LEAPFROG_DKD: drift half, forces, kick full, drift half
+            launch_drift(s->stream, n, s->v.p, s->x.p, s->half_dt);
+            forces_at(s, s->x.p, s->a.p);
+            launch_kick_drift(s->stream, n, s->a.p, s->v.p, s->x.p, s->dt, s->half_dt);
+        }
+        ++s->step_count;
+        if (coll) collision_pass(s, s->step_count);
+    }
+}
+
 // Everything a captured step graph depends on: buffer addresses, launch
 // shapes and the scalars baked into its kernel arguments.
 GraphKey graph_key(const nb_system *s) {
@@ -472,8 +536,33 @@ int nb_evolve(nb_system *s, int method, int64_t *elapsed_ns) {
             if (s->n != 2) throw ApiError(NB_ERR_TWO_BODY, "A two-body integration can only be performed if there are two bodies present.");
             throw ApiError(NB_ERR_NOT_SUPPORTED, "two_body integration is empty in the reference");
         }
-        if ((method & NB_LEAPFROG_KDK) != NB_LEAPFROG_KDK || (method & (NB_EULER | NB_MODIFIED_EULER | NB_MIDPOINT)))
-            throw ApiError(NB_ERR_NOT_SUPPORTED, "only leapfrog_kdk (optionally | barnes_hut) is on this engine's path");
+        // the reference's dispatch order (evolve :2626-3055): the first matching flag wins
+        const int direct_other = (method & NB_EULER) == NB_EULER                   ? NB_EULER
+                                 : (method & NB_MODIFIED_EULER) == NB_MODIFIED_EULER ? NB_MODIFIED_EULER
+                                 : (method & NB_MIDPOINT) == NB_MIDPOINT             ? NB_MIDPOINT
+                                 : (method & NB_LEAPFROG_KDK) == NB_LEAPFROG_KDK     ? 0
+                                 : (method & NB_LEAPFROG_DKD) == NB_LEAPFROG_DKD     ? NB_LEAPFROG_DKD
+                                                                                     : -1;
+        if (direct_other < 0)
+            throw ApiError(NB_ERR_NOT_SUPPORTED, "rk4 / rk3_8 are empty in the reference");
+        if (direct_other > 0) {
+            if ((method & NB_BARNES_HUT) == NB_BARNES_HUT)
+                throw ApiError(NB_ERR_NOT_SUPPORTED,
+                               "Barnes-Hut is built with leapfrog_kdk only (the reference's other BH branches are "
+                               "empty or do not terminate)");
+            s->step_count = 0;
+            direct_other_steps(s, direct_other, s->iterations);
+            s->acc_method = direct_other;
+            s->acc_valid = false;
+            NB_HIP(hipStreamSynchronize(s->stream));
+            s->prof.collect();
+            s->evolved = true;
+            s->time_elapsed = (long double)s->iterations * (long double)s->dt;  // evolve :3069
+            auto t1 = std::chrono::high_resolution_clock::now();
+            if (elapsed_ns)
+                *elapsed_ns = (int64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+            return NB_OK;
+        }
         s->step_count = 0;
         if ((method & NB_BARNES_HUT) == NB_BARNES_HUT) {
             if (s->coll_mode != NB_NO_COLL_CHECK)
@@ -501,10 +590,13 @@ int nb_advance(nb_system *s, uint64_t steps, int64_t *elapsed_ns) {
     if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
     if (elapsed_ns) *elapsed_ns = 0;
     return guarded(s, [&]() {
-        if (!s->evolved || !s->acc_valid)
+        const bool other = s->evolved && (s->acc_method == NB_EULER || s->acc_method == NB_MODIFIED_EULER ||
+                                           s->acc_method == NB_MIDPOINT || s->acc_method == NB_LEAPFROG_DKD);
+        if (!s->evolved || (!other && !s->acc_valid))
             throw ApiError(NB_ERR_NO_EVOLUTION, "nb_advance needs a previous nb_evolve with unchanged parameters");
         auto t0 = std::chrono::high_resolution_clock::now();
-        if (s->acc_method == NB_BARNES_HUT) bh_steps(s, steps);
+        if (other) direct_other_steps(s, s->acc_method, steps);
+        else if (s->acc_method == NB_BARNES_HUT) bh_steps(s, steps);
         else direct_kdk_steps(s, steps);
         s->time_elapsed += (long double)steps * (long double)s->dt;
         NB_HIP(hipStreamSynchronize(s->stream));

@@ -451,6 +451,48 @@ __global__ void k_drift(int64_t n, const double *__restrict__ v, double *__restr
     x[i] += dt * v[i];
 }
 
+// The other integrators of evolve, per component (3n entries), in the
+// reference's operation order (gregsys.hpp:380-419).
+// take_euler_step: x += v*dt ; v += a*dt
+__global__ void k_euler(int64_t n, const double *__restrict__ a, double *__restrict__ v, double *__restrict__ x,
+                        double dt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    const double vv = v[i];
+    x[i] += vv * dt;
+    v[i] = vv + a[i] * dt;
+}
+
+// predicted state of modified Euler (h = dt) and midpoint (h = dt/2):
+// px = x + h*v ; pv = v + h*a
+__global__ void k_predict(int64_t n, const double *__restrict__ x, const double *__restrict__ v,
+                          const double *__restrict__ a, double h, double *__restrict__ px, double *__restrict__ pv) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    px[i] = x[i] + h * v[i];
+    pv[i] = v[i] + h * a[i];
+}
+
+// take_modified_euler_step: x += half_dt*(v + pv) ; v += half_dt*(a + pa)
+__global__ void k_modeuler(int64_t n, const double *__restrict__ a, const double *__restrict__ pv,
+                           const double *__restrict__ pa, double *__restrict__ v, double *__restrict__ x,
+                           double half_dt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    const double vv = v[i];
+    x[i] += half_dt * (vv + pv[i]);
+    v[i] = vv + half_dt * (a[i] + pa[i]);
+}
+
+// take_midpoint_step: x += dt*pv ; v += dt*pa
+__global__ void k_midpoint(int64_t n, const double *__restrict__ pv, const double *__restrict__ pa,
+                           double *__restrict__ v, double *__restrict__ x, double dt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    x[i] += dt * pv[i];
+    v[i] += dt * pa[i];
+}
+
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -598,6 +640,32 @@ void launch_kick(hipStream_t s, int64_t n, const double *a, double *v, double co
 void launch_kick_drift(hipStream_t s, int64_t n, const double *a, double *v, double *x, double half_dt, double dt) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_kick_drift, dim3(blocks_for(3 * n)), dim3(kBlock), 0, s, n, a, v, x, half_dt, dt);
+    NB_HIP(hipGetLastError());
+}
+
+void launch_euler(hipStream_t s, int64_t n, const double *a, double *v, double *x, double dt) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_euler, dim3(blocks_for(3 * n)), dim3(kBlock), 0, s, n, a, v, x, dt);
+    NB_HIP(hipGetLastError());
+}
+
+void launch_predict(hipStream_t s, int64_t n, const double *x, const double *v, const double *a, double h,
+                    double *px, double *pv) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_predict, dim3(blocks_for(3 * n)), dim3(kBlock), 0, s, n, x, v, a, h, px, pv);
+    NB_HIP(hipGetLastError());
+}
+
+void launch_modeuler(hipStream_t s, int64_t n, const double *a, const double *pv, const double *pa, double *v,
+                     double *x, double half_dt) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_modeuler, dim3(blocks_for(3 * n)), dim3(kBlock), 0, s, n, a, pv, pa, v, x, half_dt);
+    NB_HIP(hipGetLastError());
+}
+
+void launch_midpoint(hipStream_t s, int64_t n, const double *pv, const double *pa, double *v, double *x, double dt) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_midpoint, dim3(blocks_for(3 * n)), dim3(kBlock), 0, s, n, pv, pa, v, x, dt);
     NB_HIP(hipGetLastError());
 }
 

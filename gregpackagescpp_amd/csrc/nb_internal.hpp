@@ -152,6 +152,7 @@ struct DirectWork {
     int variant = 1;       // 0: j stream through scalar loads, 1: j tiles through LDS
     DBuf<double> partial64;  // [split][3][n] for the FP64 split kernels
     DBuf<double> terms;      // [j block][3][n] pair terms of the FP64 reference-order path
+    DBuf<double> px, pv, pa; // predicted state of modified Euler / midpoint (SoA, 3n)
     int nsplit64 = 1;
 };
 
@@ -175,6 +176,13 @@ void launch_force_fp64(hipStream_t s, int64_t n, const double *x, const double *
 void launch_kick(hipStream_t s, int64_t n, const double *a, double *v, double coef);
 void launch_kick_drift(hipStream_t s, int64_t n, const double *a, double *v, double *x, double half_dt, double dt);
 void launch_drift(hipStream_t s, int64_t n, const double *v, double *x, double dt);
+// the other integrators' update kernels (reference operation order)
+void launch_euler(hipStream_t s, int64_t n, const double *a, double *v, double *x, double dt);
+void launch_predict(hipStream_t s, int64_t n, const double *x, const double *v, const double *a, double h,
+                    double *px, double *pv);
+void launch_modeuler(hipStream_t s, int64_t n, const double *a, const double *pv, const double *pa, double *v,
+                     double *x, double half_dt);
+void launch_midpoint(hipStream_t s, int64_t n, const double *pv, const double *pa, double *v, double *x, double dt);
 void launch_pe_fp64(hipStream_t s, int64_t n, const double *x, const double *m, double G, double eps2, int form,
                     double *pe);
 int choose_nsplit(int64_t n, int ipl, int waves_per_simd);

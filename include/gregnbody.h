@@ -109,13 +109,19 @@ int nb_set_collisions(nb_system *sys, int mode, int mergers, double min_tot_com_
 int nb_set_precision(nb_system *sys, int precision);
 
 /* evolve(int integration_method) (gregsys.hpp:2593-2607): runs `iterations`
- * steps of NB_LEAPFROG_KDK (direct summation) or NB_LEAPFROG_KDK|NB_BARNES_HUT.
- * Like the reference, each call first recomputes a(x0).  elapsed_ns (may be
- * NULL) receives the host wall time of the call (evolve's return value). */
+ * steps of NB_LEAPFROG_KDK (direct summation, or | NB_BARNES_HUT), or of the
+ * reference's other direct-sum integrators NB_EULER, NB_MODIFIED_EULER,
+ * NB_MIDPOINT (:2666-2891) and NB_LEAPFROG_DKD (:2993-3039), with s_coll
+ * after every step when collisions are on; the first matching flag wins, as
+ * in the reference's dispatch.  Like the reference, a KDK call first
+ * recomputes a(x0).  elapsed_ns (may be NULL) receives the host wall time of
+ * the call (evolve's return value).  NB_ERR_NOT_SUPPORTED: rk4 / rk3_8 (empty
+ * in the reference) and Barnes-Hut with any integrator but KDK. */
 int nb_evolve(nb_system *sys, int method, int64_t *elapsed_ns);
-/* Engine extension: continue the previous evolve for `steps` more KDK steps
- * without recomputing a(x0) (the cached accelerations are those of the
- * current positions).  Returns NB_ERR_NO_EVOLUTION if nothing was evolved. */
+/* Engine extension: continue the previous evolve for `steps` more steps of
+ * the same method, without recomputing a(x0) for KDK (the cached
+ * accelerations are those of the current positions).  Returns
+ * NB_ERR_NO_EVOLUTION if nothing was evolved. */
 int nb_advance(nb_system *sys, uint64_t steps, int64_t *elapsed_ns);
 /* Total kinetic and potential energy of the current state in FP64, reference
  * formulae (set_ke gregbod.hpp:917-920, cumulative_pe gregsys.hpp:358-363,

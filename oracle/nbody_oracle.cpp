@@ -78,6 +78,9 @@ enum { FORM_NEWTON = 0, FORM_SOFT = 1 };
  *   r3 = sqrt((wx*wx + wy*wy) + wz*wz) ; a_i += (G*m_j)/r3 * d ; a_j -= (G*m_i)/r3 * d
  * form 1 is the reference's softened kernel bh_acc (gregsys.hpp:371-375) applied
  * pairwise:  s = d.d + eps^2 ; a_i += G*(m_j/s) * (d/sqrt(s)).                      */
+/* integration methods (the reference's bit flags, gregsys.hpp:156-161) */
+enum { METHOD_EULER = 2, METHOD_MODIFIED_EULER = 4, METHOD_MIDPOINT = 8, METHOD_KDK = 16, METHOD_DKD = 32 };
+
 template <typename T>
 inline void pair_acc(int form, T G, T eps2, const T *pi, const T *pj, T mi, T mj, T *ai, T *aj) {
     T dx = pj[0] - pi[0], dy = pj[1] - pi[1], dz = pj[2] - pi[2];
@@ -298,8 +301,8 @@ std::vector<long double> g_trace_vals;
  * v += half_dt*a ; x += dt*v ; recompute a ; v += half_dt*a ; then s_coll when
  * collisions are on (FUNC_TEMPL_SELECT :16-57). */
 template <typename T>
-int kdk(int64_t *n_io, int form, T G, T dt, T eps, int64_t steps, int coll, T min_mom, T *m, T *r, T *e, T *pos,
-        T *vel, T *acc, int64_t *labels, int64_t *log_out, int64_t log_cap, int64_t *log_len) {
+int integrate(int method, int64_t *n_io, int form, T G, T dt, T eps, int64_t steps, int coll, T min_mom, T *m, T *r,
+              T *e, T *pos, T *vel, T *acc, int64_t *labels, int64_t *log_out, int64_t log_cap, int64_t *log_len) {
     int64_t n = *n_io;
     T half_dt = dt / 2;
     std::vector<CBody<T>> bods(n);
@@ -320,16 +323,71 @@ int kdk(int64_t *n_io, int form, T G, T dt, T eps, int64_t steps, int coll, T mi
             for (size_t j = i + 1; j < nb; ++j)
                 pair_acc<T>(form, G, eps2, bods[i].p, bods[j].p, bods[i].m, bods[j].m, bods[i].a, bods[j].a);
     };
-    force();
-    for (int64_t s = 1; s <= steps; ++s) {
-        for (auto &b : bods)
+    // predicted positions, velocities and accelerations of modified Euler and
+    // midpoint (the reference's `predicted` tuples)
+    std::vector<T> pp, pv, pa;
+    auto predict = [&](T h) {
+        size_t nb = bods.size();
+        pp.assign(3 * nb, T(0));
+        pv.assign(3 * nb, T(0));
+        pa.assign(3 * nb, T(0));
+        for (size_t i = 0; i < nb; ++i)
             for (int k = 0; k < 3; ++k) {
-                b.v[k] += half_dt * b.a[k];
-                b.p[k] += dt * b.v[k];
+                pp[3 * i + k] = bods[i].p[k] + h * bods[i].v[k];
+                pv[3 * i + k] = bods[i].v[k] + h * bods[i].a[k];
             }
-        force();
-        for (auto &b : bods)
-            for (int k = 0; k < 3; ++k) b.v[k] += half_dt * b.a[k];
+        for (size_t i = 0; i < nb; ++i)
+            for (size_t j = i + 1; j < nb; ++j)
+                pair_acc<T>(form, G, eps2, &pp[3 * i], &pp[3 * j], bods[i].m, bods[j].m, &pa[3 * i], &pa[3 * j]);
+    };
+    if (method == METHOD_KDK) force();
+    for (int64_t s = 1; s <= steps; ++s) {
+        if (method == METHOD_KDK) {
+            for (auto &b : bods)
+                for (int k = 0; k < 3; ++k) {
+                    b.v[k] += half_dt * b.a[k];
+                    b.p[k] += dt * b.v[k];
+                }
+            force();
+            for (auto &b : bods)
+                for (int k = 0; k < 3; ++k) b.v[k] += half_dt * b.a[k];
+        } else if (method == METHOD_EULER) {
+            // calc_acc_and_e, then take_euler_step (gregsys.hpp:380-391, 2666-2712)
+            force();
+            for (auto &b : bods)
+                for (int k = 0; k < 3; ++k) {
+                    b.p[k] += b.v[k] * dt;
+                    b.v[k] += b.a[k] * dt;
+                }
+        } else if (method == METHOD_MODIFIED_EULER) {
+            // predictor with dt, corrector with the mean slopes (:392-405, 2714-2803)
+            force();
+            predict(dt);
+            for (size_t i = 0; i < bods.size(); ++i)
+                for (int k = 0; k < 3; ++k) {
+                    bods[i].p[k] += half_dt * (bods[i].v[k] + pv[3 * i + k]);
+                    bods[i].v[k] += half_dt * (bods[i].a[k] + pa[3 * i + k]);
+                }
+        } else if (method == METHOD_MIDPOINT) {
+            // slopes at the half-step prediction (:406-419, 2804-2891)
+            force();
+            predict(half_dt);
+            for (size_t i = 0; i < bods.size(); ++i)
+                for (int k = 0; k < 3; ++k) {
+                    bods[i].p[k] += dt * pv[3 * i + k];
+                    bods[i].v[k] += dt * pa[3 * i + k];
+                }
+        } else {
+            // leapfrog DKD: drift half, forces, kick full, drift half (:540-607, 2993-3039)
+            for (auto &b : bods)
+                for (int k = 0; k < 3; ++k) b.p[k] += half_dt * b.v[k];
+            force();
+            for (auto &b : bods)
+                for (int k = 0; k < 3; ++k) {
+                    b.v[k] += dt * b.a[k];
+                    b.p[k] += half_dt * b.v[k];
+                }
+        }
         if (coll) s_coll<T>(bods, coll == 2, min_mom, s, next_label, log);
         if (g_trace_on)
             for (auto &b : bods) {
@@ -577,14 +635,30 @@ int orc_kdk_ld(int64_t *n, int form, const long double *G, const long double *dt
                int64_t steps, int coll, const long double *min_mom, long double *m, long double *r, long double *e,
                long double *pos, long double *vel, long double *acc, int64_t *labels, int64_t *log, int64_t log_cap,
                int64_t *log_len) {
-    return kdk<long double>(n, form, *G, *dt, *eps, steps, coll, *min_mom, m, r, e, pos, vel, acc, labels, log,
-                            log_cap, log_len);
+    return integrate<long double>(METHOD_KDK, n, form, *G, *dt, *eps, steps, coll, *min_mom, m, r, e, pos, vel, acc,
+                                  labels, log, log_cap, log_len);
 }
 int orc_kdk_d(int64_t *n, int form, double G, double dt, double eps, int64_t steps, int coll, double min_mom,
               double *m, double *r, double *e, double *pos, double *vel, double *acc, int64_t *labels, int64_t *log,
               int64_t log_cap, int64_t *log_len) {
-    return kdk<double>(n, form, G, dt, eps, steps, coll, min_mom, m, r, e, pos, vel, acc, labels, log, log_cap,
-                       log_len);
+    return integrate<double>(METHOD_KDK, n, form, G, dt, eps, steps, coll, min_mom, m, r, e, pos, vel, acc, labels,
+                             log, log_cap, log_len);
+}
+/* The other direct-sum integrators of evolve (method = the reference's flag:
+ * 2 Euler, 4 modified Euler, 8 midpoint, 16 leapfrog KDK, 32 leapfrog DKD),
+ * with s_coll after every step when coll != 0. */
+int orc_integrate_ld(int method, int64_t *n, int form, const long double *G, const long double *dt,
+                     const long double *eps, int64_t steps, int coll, const long double *min_mom, long double *m,
+                     long double *r, long double *e, long double *pos, long double *vel, long double *acc,
+                     int64_t *labels, int64_t *log, int64_t log_cap, int64_t *log_len) {
+    return integrate<long double>(method, n, form, *G, *dt, *eps, steps, coll, *min_mom, m, r, e, pos, vel, acc,
+                                  labels, log, log_cap, log_len);
+}
+int orc_integrate_d(int method, int64_t *n, int form, double G, double dt, double eps, int64_t steps, int coll,
+                    double min_mom, double *m, double *r, double *e, double *pos, double *vel, double *acc,
+                    int64_t *labels, int64_t *log, int64_t log_cap, int64_t *log_len) {
+    return integrate<double>(method, n, form, G, dt, eps, steps, coll, min_mom, m, r, e, pos, vel, acc, labels, log,
+                             log_cap, log_len);
 }
 /* One collision pass only, on a given state (no force, no kick). */
 int orc_s_coll_d(int64_t *n_io, int merge, double min_mom, int64_t step, double *m, double *r, double *e,
@@ -877,8 +951,8 @@ void orc_trace_copy(int64_t *hdr, long double *vals) {
 int orc_forward_d(int64_t n, double G, double dt, double eps, int64_t steps, const double *m, double *x, double *v) {
     int64_t nn = n;
     std::vector<double> acc(3 * n);
-    return kdk<double>(&nn, FORM_SOFT, G, dt, eps, steps, 0, 0.0, const_cast<double *>(m), nullptr, nullptr, x, v,
-                       acc.data(), nullptr, nullptr, 0, nullptr);
+    return integrate<double>(METHOD_KDK, &nn, FORM_SOFT, G, dt, eps, steps, 0, 0.0, const_cast<double *>(m), nullptr,
+                             nullptr, x, v, acc.data(), nullptr, nullptr, 0, nullptr);
 }
 
 } // extern "C"

@@ -61,6 +61,8 @@ def lib():
         L.orc_direct_acc_mt.argtypes = [C.c_int64, C.c_double, C.c_double, _D, _D, _D, C.c_int]
         L.orc_bh_acc_mt.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, _D, _D, _D, _I64, C.c_int]
         L.orc_max_threads.restype = C.c_int
+        L.orc_integrate_ld.argtypes = [C.c_int] + L.orc_kdk_ld.argtypes
+        L.orc_integrate_d.argtypes = [C.c_int] + L.orc_kdk_d.argtypes
         L.orc_trace.argtypes = [C.c_int]
         L.orc_trace_len.restype = C.c_int64
         L.orc_trace_copy.argtypes = [_I64, _LDP]
@@ -125,9 +127,13 @@ def energy(m, pos, vel, G, eps=0.0, form=FORM_NEWTON, dtype=LD):
     return ke.value, pe.value
 
 
+METHOD_EULER, METHOD_MODIFIED_EULER, METHOD_MIDPOINT, METHOD_KDK, METHOD_DKD = 2, 4, 8, 16, 32
+
+
 def kdk(m, pos, vel, G, dt, steps, eps=0.0, form=FORM_NEWTON, dtype=LD, coll=0, r=None, e=None, min_mom=1e10,
-        labels=None, log_cap=100000):
-    """Leapfrog KDK (+ optional s_coll: coll=1 bounce, coll=2 mergers). Returns dict."""
+        labels=None, log_cap=100000, method=METHOD_KDK):
+    """Leapfrog KDK, or another direct integrator of evolve via `method` (the
+    reference's flag), + optional s_coll: coll=1 bounce, coll=2 mergers."""
     n = len(m)
     conv = _ld if dtype == LD else _d
     ptr = _lp if dtype == LD else _dp
@@ -140,11 +146,13 @@ def kdk(m, pos, vel, G, dt, steps, eps=0.0, form=FORM_NEWTON, dtype=LD, coll=0, 
     nn = C.c_int64(n)
     loglen = C.c_int64()
     if dtype == LD:
-        lib().orc_kdk_ld(C.byref(nn), form, _sc(G), _sc(dt), _sc(eps), steps, coll, _sc(min_mom), ptr(m), ptr(r),
-                         ptr(e), ptr(pos), ptr(vel), ptr(acc), _ip(lab), _ip(log), log_cap, C.byref(loglen))
+        lib().orc_integrate_ld(method, C.byref(nn), form, _sc(G), _sc(dt), _sc(eps), steps, coll, _sc(min_mom),
+                               ptr(m), ptr(r), ptr(e), ptr(pos), ptr(vel), ptr(acc), _ip(lab), _ip(log), log_cap,
+                               C.byref(loglen))
     else:
-        lib().orc_kdk_d(C.byref(nn), form, float(G), float(dt), float(eps), steps, coll, float(min_mom), ptr(m),
-                        ptr(r), ptr(e), ptr(pos), ptr(vel), ptr(acc), _ip(lab), _ip(log), log_cap, C.byref(loglen))
+        lib().orc_integrate_d(method, C.byref(nn), form, float(G), float(dt), float(eps), steps, coll,
+                              float(min_mom), ptr(m), ptr(r), ptr(e), ptr(pos), ptr(vel), ptr(acc), _ip(lab), _ip(log),
+                              log_cap, C.byref(loglen))
     k = nn.value
     return dict(n=k, m=m[:k], r=r[:k], e=e[:k], pos=pos[:k], vel=vel[:k], acc=acc[:k], labels=lab[:k],
                 log=log[: min(loglen.value, log_cap)])

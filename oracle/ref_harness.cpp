@@ -9,7 +9,11 @@
  * Usage:  ref_harness <mode> <in.bin> <out.bin>
  *   mode kdk  : gtd::system<ld,ld,ld,false,false,0,0,0,false>::evolve(leapfrog_kdk)
  *               (gregsys.hpp:2946-2990); output = final pos, vel (long double).
- *   mode coll : system<ld,ld,ld,false,false,3,0,0,false> (built with and without
+ *   mode euler / modeuler / midpoint / dkd : the same with evolve(euler),
+ *               evolve(modified_euler), evolve(midpoint), evolve(leapfrog_dkd)
+ *               (:2666-2891, :2993-3039).
+ *   mode coll, coll_euler, coll_modeuler, coll_midpoint, coll_dkd :
+ *               system<ld,ld,ld,false,false,3,0,0,false> (built with and without
  *               -DGREGSYS_MERGERS); evolve(leapfrog_kdk, nullptr, per_body,
  *               per_step) dumps every body after each step's s_coll.
  *   mode tree : bh_tree<double,double,double,0>::add_bodies (greg8tree.hpp:1066)
@@ -52,7 +56,7 @@ static bool read_in(const char *path, In &in) {
     return ok;
 }
 
-static int run_kdk(const In &in, FILE *out) {
+static int run_kdk(const In &in, FILE *out, int method) {
     using bod_t = gtd::body<ld, ld, ld, 0>;
     std::vector<bod_t> bods;
     bods.reserve((size_t)in.n);
@@ -64,7 +68,7 @@ static int run_kdk(const In &in, FILE *out) {
     gtd::system<ld, ld, ld, false, false, 0, 0, 0, false> sys{bods, (ld)in.dt, (uint64_t)in.steps,
                                                               "M1:1,D1:1,T1:1", true};
     /* evolve's return value is the reference's own wall-clock timer of the step loop (:2604, :3061-3065) */
-    auto ns = sys.evolve(gtd::system<ld, ld, ld, false, false, 0, 0, 0, false>::leapfrog_kdk);
+    auto ns = sys.evolve(method);
     printf("evolve_ns %lld\n", (long long)ns.count());
     for (const auto &b : sys) {
         ld v[6] = {b.pos().get_x(), b.pos().get_y(), b.pos().get_z(), b.vel().get_x(), b.vel().get_y(), b.vel().get_z()};
@@ -73,7 +77,7 @@ static int run_kdk(const In &in, FILE *out) {
     return 0;
 }
 
-static int run_coll(const In &in, FILE *out) {
+static int run_coll(const In &in, FILE *out, int method) {
     using bod_t = gtd::body<ld, ld, ld, 0>;
     using sys_t = gtd::system<ld, ld, ld, false, false, 3, 0, 0, false>;
     std::vector<bod_t> bods;
@@ -99,7 +103,7 @@ static int run_coll(const In &in, FILE *out) {
         ld v[8] = {b.mass(), b.rad(), b.pos().get_x(), b.pos().get_y(), b.pos().get_z(), b.vel().get_x(), b.vel().get_y(), b.vel().get_z()};
         fwrite(v, sizeof(ld), 8, out);
     };
-    sys.evolve(sys_t::leapfrog_kdk, nullptr, per_body, per_step);
+    sys.evolve(method, nullptr, per_body, per_step);
     return 0;
 }
 
@@ -207,7 +211,7 @@ static int run_nsys_copy(const char *in_path, const char *out_path) {
 
 int main(int argc, char **argv) {
     if (argc != 4) {
-        fprintf(stderr, "usage: %s kdk|coll|tree|nsys_ld|nsys_d|nsys_copy|nsys_copy_d in out\n", argv[0]);
+        fprintf(stderr, "usage: %s <mode> in out (modes: see the file header)\n", argv[0]);
         return 2;
     }
     const std::string m0 = argv[1];
@@ -237,9 +241,17 @@ int main(int argc, char **argv) {
     int rc = 2;
     std::string mode = argv[1];
     try {
-        if (mode == "kdk") rc = run_kdk(in, out);
-        else if (mode == "coll") rc = run_coll(in, out);
-        else if (mode == "tree") rc = run_tree(in, out);
+        using S = gtd::system<ld, ld, ld, false, false, 0, 0, 0, false>;
+        const std::pair<const char *, int> methods[] = {{"kdk", S::leapfrog_kdk}, {"euler", S::euler},
+                                                        {"modeuler", S::modified_euler}, {"midpoint", S::midpoint},
+                                                        {"dkd", S::leapfrog_dkd}};
+        for (const auto &mm : methods) {
+            if (mode == mm.first) rc = run_kdk(in, out, mm.second);
+            if (mode == std::string(mm.first == std::string("kdk") ? "coll" : "coll_") +
+                           (mm.first == std::string("kdk") ? "" : mm.first))
+                rc = run_coll(in, out, mm.second);
+        }
+        if (mode == "tree") rc = run_tree(in, out);
     } catch (const std::exception &e) {
         fprintf(stderr, "reference threw: %s\n", e.what());
         rc = 4;

@@ -11,8 +11,14 @@ generators; outputs are the reference's own results, stored as .npz:
                          DFS pre-order depth, octant, body index, node mass.
   ref_coll_{bounce,merge}_seed{K}.npz  system<ld,...,collisions=3> without /
                          with -DGREGSYS_MERGERS: every body after every step.
+  ref_integ_{method}_n64_s100.npz  evolve(euler | modified_euler | midpoint |
+                         leapfrog_dkd), uniform ball (seed 0), dt = 10 s.
+  ref_icoll_{method}_{bounce,merge}_seed1.npz  the same integrators with
+                         collisions = 3, rubble cluster: every body after
+                         every 25th step, and the final body count.
 
-Usage: python tests/golden/make_fixtures.py
+Usage: python tests/golden/make_fixtures.py [integ]   (integ: only the
+       integrator fixtures)
 """
 import os
 import sys
@@ -32,6 +38,7 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 KDK_CASES = [(2, 1), (2, 1000), (3, 10), (64, 10), (64, 1000), (1024, 10), (1024, 1000)]
 TREE_CASES = [2, 3, 17, 1000, 16384]
 COLL_CASES = [(24, 300, s) for s in range(4)]
+INTEG_METHODS = ["euler", "modeuler", "midpoint", "dkd"]
 
 
 def coll_ics(n, seed):
@@ -45,11 +52,37 @@ def coll_ics(n, seed):
     return m, r, e, pos, vel
 
 
+def integ_fixtures(inp, out):
+    for meth in INTEG_METHODS:
+        m, pos, vel = capi.ic_uniform_ball(0, 64, 1e20, 1e22, 1e9, 1e3)
+        O.write_ref_input(inp, m, pos, vel, steps=100, dt=10.0)
+        O.run_ref(meth, inp, out)
+        res = O.read_ref_kdk(out, 64)
+        np.savez_compressed(os.path.join(OUT, f"ref_integ_{meth}_n64_s100.npz"), m=m, pos=pos, vel=vel, dt=10.0,
+                            steps=100, pos_T=res[:, :3], vel_T=res[:, 3:])
+        print("integ", meth)
+        m, r, e, pos, vel = coll_ics(24, 1)  # seed 1: bounces and three mergers under KDK
+        for merge in (False, True):
+            O.write_ref_input(inp, m, pos, vel, r=r, e=e, steps=300, dt=5.0)
+            O.run_ref("coll_" + meth, inp, out, merge=merge)
+            ids0, st, ids, vals = O.read_ref_coll(out)
+            tag = "merge" if merge else "bounce"
+            keep = st % 25 == 0  # every 25th step's bodies (the full trace is 7200 records)
+            np.savez_compressed(os.path.join(OUT, f"ref_icoll_{meth}_{tag}_seed1.npz"), m=m, r=r, e=e, pos=pos,
+                                vel=vel, dt=5.0, steps=300, ids0=ids0, step=st[keep], ids=ids[keep],
+                                vals=vals[keep], n_final=int((st == st.max()).sum()))
+            print("icoll", meth, tag, len(st))
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref is not built (needs /root/reference): nothing generated")
     tmp = tempfile.mkdtemp()
     inp, out = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+    if sys.argv[1:] == ["integ"]:
+        integ_fixtures(inp, out)
+        return
+    integ_fixtures(inp, out)
     for n, steps in KDK_CASES:
         m, pos, vel = capi.ic_uniform_ball(0, n, 1e20, 1e22, 1e9, 1e3)
         O.write_ref_input(inp, m, pos, vel, steps=steps, dt=10.0)

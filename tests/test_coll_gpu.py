@@ -38,7 +38,7 @@ def rubble(n, seed):
     return m, r, e, pos, vel
 
 
-def run_single(m, r, e, pos, vel, steps, merge, prec=capi.PREC_FP64, dt=5.0):
+def run_single(m, r, e, pos, vel, steps, merge, prec=capi.PREC_FP64, dt=5.0, method=capi.LEAPFROG_KDK):
     s = capi.System(0)
     s.set_G(G_SI)
     s.set_timestep(dt)
@@ -46,7 +46,7 @@ def run_single(m, r, e, pos, vel, steps, merge, prec=capi.PREC_FP64, dt=5.0):
     s.set_precision(prec)
     s.set_collisions(capi.OVERLAP_COLL_CHECK, mergers=merge, min_tot_com_mom=1e10)
     s.set_bodies(m, pos, vel, r=r, rest_c=e)
-    s.evolve(capi.LEAPFROG_KDK)
+    s.evolve(method)
     return s.state(), s.ids(), s.merge_log()
 
 
@@ -162,3 +162,21 @@ def test_comet_ensemble_mixed_merge_list_matches_fp64():
         k0, k1 = starts[s], starts[s + 1]
         assert k1 - k0 == ref["n"]
         assert np.array_equal(fp64["pos"][k0:k1], ref["pos"])
+
+
+@pytest.mark.parametrize("method", [capi.EULER, capi.MODIFIED_EULER, capi.MIDPOINT, capi.LEAPFROG_DKD])
+@pytest.mark.parametrize("merge", [False, True])
+def test_other_integrators_collisions_bit_exact(method, merge):
+    """Euler, modified Euler, midpoint and leapfrog DKD with s_coll after every
+    step (rubble cluster, seed 1: bounces and, with mergers, three merges)
+    against the oracle's double restatement (pinned to the reference's
+    ref_icoll_* fixtures in long double)."""
+    print(f"case other integrators coll method={method} merge={merge}", flush=True)
+    m, r, e, pos, vel = rubble(24, 1)
+    st, ids, log = run_single(m, r, e, pos, vel, 300, merge, method=method)
+    ref = O.kdk(m, pos, vel, G_SI, 5.0, 300, coll=2 if merge else 1, r=r, e=e, dtype=np.float64, method=method)
+    assert len(ids) == ref["n"]
+    assert np.array_equal(ids, ref["labels"])
+    for k in ("m", "r", "pos", "vel"):
+        assert np.array_equal(st[k], ref[k]), k
+    assert np.array_equal(log, ref["log"])

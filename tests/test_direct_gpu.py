@@ -237,3 +237,47 @@ def test_fp64_one_lane_kernel_above_term_path():
     ref = O.kdk(m, pos, vel, G_SI, 10.0, 1, eps=1e6, form=O.FORM_SOFT, dtype=np.float64)
     assert np.array_equal(st["pos"], ref["pos"])
     assert np.array_equal(st["acc"], ref["acc"])
+
+
+OTHER_METHODS = [capi.EULER, capi.MODIFIED_EULER, capi.MIDPOINT, capi.LEAPFROG_DKD]
+
+
+@pytest.mark.parametrize("method", OTHER_METHODS)
+@pytest.mark.parametrize("eps", [0.0, 1e6])
+def test_other_integrators_fp64_bit_exact(method, eps):
+    """evolve(euler | modified_euler | midpoint | leapfrog_dkd) in FP64: the
+    reference's operation order, bit-identical to the double restatement
+    (whose long-double form is pinned to the compiled reference); nb_advance
+    continues the same sequence."""
+    print(f"case other integrator method={method} eps={eps}", flush=True)
+    m, pos, vel = capi.ic_uniform_ball(15, 257, 1e20, 1e22, 1e9, 1e3)
+    s = make_system(m, pos, vel, 10.0, 12, eps)
+    s.evolve(method)
+    s.advance(8)
+    st = s.state()
+    form = O.FORM_SOFT if eps > 0 else O.FORM_NEWTON
+    ref = O.kdk(m, pos, vel, G_SI, 10.0, 20, eps=eps, form=form, dtype=np.float64, method=method)
+    assert np.array_equal(st["pos"], ref["pos"])
+    assert np.array_equal(st["vel"], ref["vel"])
+    assert np.array_equal(st["acc"], ref["acc"])
+    assert s.time_elapsed() == 200.0
+
+
+@pytest.mark.parametrize("method", OTHER_METHODS)
+@pytest.mark.parametrize("prec", [capi.PREC_MIXED, capi.PREC_FP64_FAST])
+def test_other_integrators_fast_precisions(method, prec):
+    """The same integrators with FP32 pair forces (Kahan sums) or the FP64
+    split kernels: velocity increments within 1e-5 (mixed) / 1e-10 (FP64
+    split: rsq + 2 Newton steps and a split summation order, whose ~1e-16
+    per-term errors are amplified by cancellation among the 1023 terms of
+    each acceleration) relative of the long-double oracle."""
+    print(f"case other integrator method={method} prec={prec}", flush=True)
+    m, pos, vel = capi.ic_uniform_ball(16, 1024, 1e20, 1e22, 1e9, 1e3)
+    s = make_system(m, pos, vel, 10.0, 10, 1e6, prec)
+    s.evolve(method)
+    st = s.state()
+    ref = O.kdk(m, pos, vel, G_SI, 10.0, 10, eps=1e6, form=O.FORM_SOFT, method=method)
+    dv = st["vel"] - vel
+    dv_ref = (ref["vel"] - vel.astype(LD)).astype(np.float64)
+    tol = 1e-5 if prec == capi.PREC_MIXED else 1e-10
+    assert np.abs(dv - dv_ref).max() <= tol * np.abs(dv_ref).max()

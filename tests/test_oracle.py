@@ -54,6 +54,38 @@ def test_collisions_bit_exact_vs_reference(path):
         assert len(res["log"]) == len(f["m"]) - res["n"]
 
 
+METHODS = {"euler": O.METHOD_EULER, "modeuler": O.METHOD_MODIFIED_EULER, "midpoint": O.METHOD_MIDPOINT,
+           "dkd": O.METHOD_DKD}
+
+
+@pytest.mark.parametrize("path", gold("ref_integ_*.npz"), ids=os.path.basename)
+def test_other_integrators_bit_exact_vs_reference(path):
+    """evolve(euler | modified_euler | midpoint | leapfrog_dkd) of the compiled
+    reference, 64 bodies x 100 steps, against the oracle's long-double
+    restatement."""
+    f = np.load(path)
+    meth = METHODS[os.path.basename(path).split("_")[2]]
+    res = O.kdk(f["m"], f["pos"], f["vel"], G_SI, LD(float(f["dt"])), int(f["steps"]), method=meth)
+    assert np.array_equal(res["pos"], f["pos_T"].astype(LD))
+    assert np.array_equal(res["vel"], f["vel_T"].astype(LD))
+
+
+@pytest.mark.parametrize("path", gold("ref_icoll_*.npz"), ids=os.path.basename)
+def test_other_integrators_with_collisions_bit_exact(path):
+    """The same integrators with collisions = 3 (bounces; mergers with
+    -DGREGSYS_MERGERS): every body after every 25th step."""
+    f = np.load(path)
+    name = os.path.basename(path)
+    meth = METHODS[name.split("_")[2]]
+    merge = "_merge_" in name
+    res, st, lab, vals = O.kdk_trace(f["m"], f["pos"], f["vel"], G_SI, LD(float(f["dt"])), int(f["steps"]),
+                                     coll=2 if merge else 1, r=f["r"], e=f["e"], min_mom=1e10, method=meth)
+    keep = st % 25 == 0
+    assert np.array_equal(st[keep], f["step"])
+    assert np.array_equal(vals[keep], f["vals"])
+    assert res["n"] == int(f["n_final"])
+
+
 def test_collision_fixtures_exercise_bounces_and_merges():
     """The fixtures must contain both bounce-changed velocities and merges."""
     bounced = merged = False
