@@ -52,6 +52,14 @@ constexpr int kMaxLog = 32;
 enum { BH_ERR_NONE = 0, BH_ERR_PLACEMENT = 1, BH_ERR_INDIST = 2, BH_ERR_DEPTH = 3 };
 }  // namespace
 
+// Walk record of a node: FP32 COM and G*mass, then skip pointer, first sorted
+// body, body count and the FP32 bits of the squared opening radius, in one
+// 32-byte record so that a node visit reads one cache line.
+struct __align__(32) WalkNode {
+    float4 q;
+    int4 r;
+};
+
 struct BHWork {
     int64_t cap_n = 0, cap_nodes = 0;
     DBuf<double> part;      // bbox partials
@@ -69,8 +77,7 @@ struct BHWork {
     DBuf<int8_t> octant;
     DBuf<double> mass, com, side;
     DBuf<uint32_t> ready;
-    DBuf<float4> f4;        // com (FP32) and G*mass
-    DBuf<int4> i4;          // skip, first, count, r^2_open bits (walk record)
+    DBuf<WalkNode> wnode;   // walk record per node (one 32-byte line)
     int bucket = 32;        // rejected nodes with at most this many bodies are summed directly
     DBuf<float4> pos32s;    // sorted body positions (FP32) and G*m
     DBuf<double> accs;      // sorted accelerations, SoA
@@ -426,24 +433,26 @@ __global__ void k_com_level(int32_t total, int level, const uint8_t *__restrict_
     com[2 * (size_t)total + p] = sz / ms;
 }
 
-// Walk record per node: f4 = (com, G*mass) in FP32 and i4 = (skip, first
+// Walk record per node: q = (com, G*mass) in FP32 and r = (skip, first
 // sorted body, body count, r^2_open as float bits; -1 for leaves).
 __global__ void k_walk_prep(int32_t total, const double *__restrict__ mass, const double *__restrict__ com,
                             const double *__restrict__ side, const int32_t *__restrict__ body,
                             const int32_t *__restrict__ skip, const int32_t *__restrict__ first,
-                            const int32_t *__restrict__ last, double G, double theta, float4 *__restrict__ f4,
-                            int4 *__restrict__ i4) {
+                            const int32_t *__restrict__ last, double G, double theta,
+                            WalkNode *__restrict__ wnode) {
     const int32_t x = blockIdx.x * kBlock + threadIdx.x;
     if (x >= total) return;
-    f4[x] = make_float4((float)com[x], (float)com[(size_t)total + x], (float)com[2 * (size_t)total + x],
-                        (float)(G * mass[x]));
+    WalkNode w;
+    w.q = make_float4((float)com[x], (float)com[(size_t)total + x], (float)com[2 * (size_t)total + x],
+                      (float)(G * mass[x]));
     float r2o;
     if (body[x] >= 0) r2o = -1.0f;
     else if (theta > 0) {
         double r = side[x] / theta;
         r2o = (float)(r * r);
     } else r2o = INFINITY;
-    i4[x] = make_int4(skip[x], first[x], last[x] - first[x] + 1, __float_as_int(r2o));
+    w.r = make_int4(skip[x], first[x], last[x] - first[x] + 1, __float_as_int(r2o));
+    wnode[x] = w;
 }
 
 __global__ void k_sorted_pos(int64_t n, const uint32_t *__restrict__ idx, const double *__restrict__ x,
@@ -475,8 +484,8 @@ __device__ __forceinline__ void monopole(float4 q, float4 p, float eps2, bool so
 
 template <bool SOFT>
 __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, const float4 *__restrict__ pos,
-                                                      const float4 *__restrict__ f4, const int4 *__restrict__ i4,
-                                                      int bucket, float eps2, double *__restrict__ accs,
+                                                      const WalkNode *__restrict__ wnode, int bucket, float eps2,
+                                                      double *__restrict__ accs,
                                                       unsigned long long *__restrict__ icount) {
 #pragma clang fp contract(fast)
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -489,13 +498,21 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
     int32_t node = 1;
     while (node < total) {
         node = __builtin_amdgcn_readfirstlane(node);
-        const float4 q = f4[node];
-        const int4 r = i4[node];
+        const WalkNode nd = wnode[node];
+        const float4 q = nd.q;
+        const int4 r = nd.r;
         const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
         const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         const bool acc_lane = d2 > __int_as_float(r.w);
         if ((__ballot(acc_lane) & active) == active) {
-            monopole(q, p, eps2, SOFT, ax, ay, az);
+            // the monopole of the accepted node from the MAC's d and |d|^2
+            const float s2 = d2 + eps2;
+            float inv = __builtin_amdgcn_rsqf(s2);
+            if (!SOFT) inv = s2 > 0.f ? inv : 0.f;
+            const float w = (q.w * inv) * (inv * inv);
+            ax = __builtin_fmaf(w, dx, ax);
+            ay = __builtin_fmaf(w, dy, ay);
+            az = __builtin_fmaf(w, dz, az);
             ++cnt;
             node = r.x;
         } else if (r.z <= bucket) {
@@ -638,7 +655,7 @@ static void bh_build(nb_system *s) {
         size_t t = (size_t)total + total / 4;
         w.first.ensure(t); w.last.ensure(t); w.skip.ensure(t); w.parent.ensure(t); w.body.ensure(t);
         w.depth.ensure(t); w.nchild.ensure(t); w.octant.ensure(t); w.mass.ensure(t); w.com.ensure(3 * t);
-        w.side.ensure(t); w.ready.ensure(t); w.f4.ensure(t); w.i4.ensure(t);
+        w.side.ensure(t); w.ready.ensure(t); w.wnode.ensure(t);
         w.cap_nodes = (int64_t)t;
     }
     w.nodes = total;
@@ -668,16 +685,14 @@ void bh_forces(nb_system *s, int64_t *interactions) {
         ScopedTimer tm(s->prof, "bh_walk", st);
         if (s->precision == NB_PREC_MIXED) {
             hipLaunchKernelGGL(k_walk_prep, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.mass.p, w.com.p,
-                               w.side.p, w.body.p, w.skip.p, w.first.p, w.last.p, s->G, s->theta, w.f4.p, w.i4.p);
+                               w.side.p, w.body.p, w.skip.p, w.first.p, w.last.p, s->G, s->theta, w.wnode.p);
             hipLaunchKernelGGL(k_sorted_pos, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.idx.p, s->x.p, s->m.p, s->G,
                                w.pos32s.p);
             const int bucket = s->tune_bh_bucket > 0 ? s->tune_bh_bucket : w.bucket;
-            if (s->eps > 0)
-                hipLaunchKernelGGL(k_walk_wave<true>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p,
-                                   w.f4.p, w.i4.p, bucket, (float)(s->eps * s->eps), w.accs.p, w.icount.p);
-            else
-                hipLaunchKernelGGL(k_walk_wave<false>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p,
-                                   w.f4.p, w.i4.p, bucket, 0.0f, w.accs.p, w.icount.p);
+            const float eps2 = s->eps > 0 ? (float)(s->eps * s->eps) : 0.0f;
+            auto kern = s->eps > 0 ? k_walk_wave<true> : k_walk_wave<false>;
+            hipLaunchKernelGGL(kern, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wnode.p, bucket,
+                               eps2, w.accs.p, w.icount.p);
         } else {
             hipLaunchKernelGGL(k_walk_exact, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.cnt.p, w.st.p,
                                s->x.p, w.mass.p, w.com.p, w.side.p, w.body.p, w.skip.p, s->G, s->eps * s->eps,
