@@ -23,9 +23,10 @@
 //     (first e >= s with c[e] < L) comes from a sparse-table range minimum;
 //     its skip pointer (next node outside its subtree) is the first node
 //     starting at e + 1;
-//  5. node mass and COM bottom up in FP64, one launch per depth from the
-//     deepest level up (leaves first), children summed in reverse octant
-//     order as calc_com;
+//  5. node mass and COM bottom up in FP64, internal nodes grouped by depth
+//     (one radix pass); the sparse deep depths in one single-workgroup
+//     launch, then one launch per depth up to the root, children summed in
+//     reverse octant order as calc_com;
 //  6. the walk is stackless over (node, skip): descend = node + 1, accept or
 //     skip = skip[node].  Two walks: per lane with the reference's per-body
 //     MAC in FP64 (parity), and per wave with FP32 monopoles where a node is
@@ -48,6 +49,8 @@ constexpr int kBlock = 256;
 constexpr int kLevels = 42;   // octant digits per body
 constexpr int kHiLevels = 21; // digits in the high key
 constexpr int kMaxLog = 32;
+constexpr int kComGrid = 512;  // workgroups of a per-depth COM launch
+constexpr int kComDeep = 20;   // depths >= this are summed by one workgroup in one launch
 
 enum { BH_ERR_NONE = 0, BH_ERR_PLACEMENT = 1, BH_ERR_INDIST = 2, BH_ERR_DEPTH = 3 };
 }  // namespace
@@ -77,6 +80,9 @@ struct BHWork {
     DBuf<int8_t> octant;
     DBuf<double> mass, com, side;
     DBuf<uint32_t> ready;
+    DBuf<uint64_t> lvl_key; // node depth (leaves: 63), radix-sorted with
+    DBuf<uint32_t> lvl_node;//   the node index: internal nodes grouped by depth
+    DBuf<uint32_t> lvl_off; // first position of each depth in the sorted list
     DBuf<WalkNode> wnode;   // walk record per node (one 32-byte line)
     int bucket = 32;        // rejected nodes with at most this many bodies are summed directly
     DBuf<float4> pos32s;    // sorted body positions (FP32) and G*m
@@ -410,11 +416,30 @@ __global__ void k_com_leaves(int64_t n, int32_t total, const uint32_t *__restric
     for (int k = 0; k < 3; ++k) com[(size_t)k * total + leaf] = x[k * n + b];
 }
 
-__global__ void k_com_level(int32_t total, int level, const uint8_t *__restrict__ depth,
-                            const int32_t *__restrict__ body, const int32_t *__restrict__ skip,
-                            double *__restrict__ mass, double *__restrict__ com) {
+// Nodes grouped by depth: key = depth for internal nodes, 63 for leaves.
+__global__ void k_level_keys(int32_t total, const uint8_t *__restrict__ depth, const int32_t *__restrict__ body,
+                             uint64_t *__restrict__ key, uint32_t *__restrict__ node) {
     const int32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= total || depth[p] != level || body[p] >= 0) return;
+    if (p >= total) return;
+    key[p] = body[p] >= 0 ? 63u : depth[p];
+    node[p] = (uint32_t)p;
+}
+
+// off[L] = first sorted position with depth >= L, for L = 0..kLevels.
+__global__ void k_level_bounds(int32_t total, const uint64_t *__restrict__ key, uint32_t *__restrict__ off) {
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    const int prev = i == 0 ? -1 : (int)key[i - 1];
+    const int cur = (int)key[i];
+    for (int L = prev + 1; L <= cur && L <= kLevels; ++L) off[L] = (uint32_t)i;
+    if (i == total - 1)
+        for (int L = cur + 1; L <= kLevels; ++L) off[L] = (uint32_t)total;
+}
+
+// calc_com (greg8tree.hpp:211-282) for internal node p from its children,
+// summed in reverse octant order (:262-266).
+__device__ __forceinline__ void com_node(int32_t total, int32_t p, const int32_t *__restrict__ skip,
+                                         double *__restrict__ mass, double *__restrict__ com) {
     int32_t kids[8];
     int nk = 0;
     for (int32_t ch = p + 1; ch < skip[p] && nk < 8; ch = skip[ch]) kids[nk++] = ch;
@@ -431,6 +456,28 @@ __global__ void k_com_level(int32_t total, int level, const uint8_t *__restrict_
     com[p] = sx / ms;
     com[(size_t)total + p] = sy / ms;
     com[2 * (size_t)total + p] = sz / ms;
+}
+
+// One depth: the nodes list[off[L] .. off[L+1]) over a fixed grid.
+__global__ void k_com_level(int32_t total, int level, const uint32_t *__restrict__ off,
+                            const uint32_t *__restrict__ list, const int32_t *__restrict__ skip,
+                            double *__restrict__ mass, double *__restrict__ com) {
+    const uint32_t i0 = off[level], i1 = off[level + 1];
+    for (uint32_t i = i0 + blockIdx.x * kBlock + threadIdx.x; i < i1; i += gridDim.x * kBlock)
+        com_node(total, (int32_t)list[i], skip, mass, com);
+}
+
+// The deep, sparsely populated depths kLevels-1 .. lo in one workgroup, a
+// barrier between depths (one launch instead of one per depth).
+__global__ __launch_bounds__(1024) void k_com_deep(int32_t total, int lo, const uint32_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ list,
+                                                   const int32_t *__restrict__ skip, double *__restrict__ mass,
+                                                   double *__restrict__ com) {
+    for (int L = kLevels - 1; L >= lo; --L) {
+        const uint32_t i0 = off[L], i1 = off[L + 1];
+        for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) com_node(total, (int32_t)list[i], skip, mass, com);
+        __syncthreads();
+    }
 }
 
 // Walk record per node: q = (com, G*mass) in FP32 and r = (skip, first
@@ -656,6 +703,8 @@ static void bh_build(nb_system *s) {
         w.first.ensure(t); w.last.ensure(t); w.skip.ensure(t); w.parent.ensure(t); w.body.ensure(t);
         w.depth.ensure(t); w.nchild.ensure(t); w.octant.ensure(t); w.mass.ensure(t); w.com.ensure(3 * t);
         w.side.ensure(t); w.ready.ensure(t); w.wnode.ensure(t);
+        w.lvl_key.ensure(t); w.lvl_node.ensure(t); w.lvl_off.ensure(kLevels + 1);
+        w.sorter.ensure((int64_t)t);
         w.cap_nodes = (int64_t)t;
     }
     w.nodes = total;
@@ -668,8 +717,16 @@ static void bh_build(nb_system *s) {
     // com is SoA with stride `total`
     hipLaunchKernelGGL(k_com_leaves, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.cnt.p, w.st.p, w.body.p,
                        s->m.p, s->x.p, w.mass.p, w.com.p);
-    for (int L = kLevels - 1; L >= 0; --L)
-        hipLaunchKernelGGL(k_com_level, dim3(blocks(total)), dim3(kBlock), 0, st, total, L, w.depth.p, w.body.p,
+    // internal nodes grouped by depth (one 8-bit radix pass), then one launch
+    // per depth that touches only that depth's nodes
+    hipLaunchKernelGGL(k_level_keys, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.depth.p, w.body.p,
+                       w.lvl_key.p, w.lvl_node.p);
+    w.sorter.sort(st, w.lvl_key.p, w.lvl_node.p, total, 6);
+    hipLaunchKernelGGL(k_level_bounds, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.lvl_key.p, w.lvl_off.p);
+    hipLaunchKernelGGL(k_com_deep, dim3(1), dim3(1024), 0, st, total, kComDeep, w.lvl_off.p, w.lvl_node.p, w.skip.p,
+                       w.mass.p, w.com.p);
+    for (int L = kComDeep - 1; L >= 0; --L)
+        hipLaunchKernelGGL(k_com_level, dim3(kComGrid), dim3(kBlock), 0, st, total, L, w.lvl_off.p, w.lvl_node.p,
                            w.skip.p, w.mass.p, w.com.p);
     NB_HIP(hipGetLastError());
 }
