@@ -51,6 +51,7 @@ constexpr int kHiLevels = 21; // digits in the high key
 constexpr int kMaxLog = 32;
 constexpr int kComGrid = 512;  // workgroups of a per-depth COM launch
 constexpr int kComDeep = 20;   // depths >= this are summed by one workgroup in one launch
+constexpr int kBodyPad = 4;    // extra entries after the sorted FP32 bodies (grouped bucket loads)
 
 enum { BH_ERR_NONE = 0, BH_ERR_PLACEMENT = 1, BH_ERR_INDIST = 2, BH_ERR_DEPTH = 3 };
 }  // namespace
@@ -563,7 +564,16 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
             ++cnt;
             node = r.x;
         } else if (r.z <= bucket) {
-            for (int j = r.y; j < r.y + r.z; ++j) monopole(pos[j], p, eps2, SOFT, ax, ay, az);
+            // four contiguous bodies per scalar load (the sorted array is
+            // padded by kBodyPad entries, so the last group stays in bounds)
+            const int32_t je = r.y + r.z;
+            for (int32_t j = r.y; j < je; j += 4) {
+                const float4 b0 = pos[j], b1 = pos[j + 1], b2 = pos[j + 2], b3 = pos[j + 3];
+                monopole(b0, p, eps2, SOFT, ax, ay, az);
+                if (j + 1 < je) monopole(b1, p, eps2, SOFT, ax, ay, az);
+                if (j + 2 < je) monopole(b2, p, eps2, SOFT, ax, ay, az);
+                if (j + 3 < je) monopole(b3, p, eps2, SOFT, ax, ay, az);
+            }
             cnt += (unsigned)r.z;
             node = r.x;
         } else {
@@ -660,7 +670,7 @@ static void bh_build(nb_system *s) {
     ScopedTimer tm(s->prof, "bh_build", st);
     if (w.cap_n < n) {
         w.khi.ensure(n); w.klo.ensure(n); w.klo_tmp.ensure(n); w.idx.ensure(n);
-        w.cnt.ensure(n); w.pos32s.ensure(n); w.accs.ensure(3 * (size_t)n);
+        w.cnt.ensure(n); w.pos32s.ensure(n + kBodyPad); w.accs.ensure(3 * (size_t)n);
         w.st.ensure((size_t)kMaxLog * n);
         w.blk.ensure((size_t)(n + 1023) / 1024 + 1);
         w.sorter.ensure(n);
