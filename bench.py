@@ -287,6 +287,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=1, help="--profile-direct: direct_kernel knob")
     ap.add_argument("--ipl", type=int, default=2, help="--profile-direct: direct_ipl knob")
     ap.add_argument("--waves", type=int, default=16, help="--profile-direct: direct_waves knob")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="for rocprofv3: the 4096-system ensemble for --batch-steps steps, no JSON")
     ap.add_argument("--profile-bh", action="store_true",
                     help="for rocprofv3: the 1M-body BH step (mixed, bucket 32), no events, no JSON")
     args = ap.parse_args()
@@ -300,6 +302,9 @@ def main():
 
     from gregpackagescpp_amd import capi
 
+    if args.profile_batch:
+        print(json.dumps(bench_batch(capi, args.batch_steps)), flush=True)
+        return
     if args.profile_direct or args.profile_bh:
         profile_run(capi, args)
         return
