@@ -565,8 +565,8 @@ int nb_evolve(nb_system *s, int method, int64_t *elapsed_ns) {
         }
         s->step_count = 0;
         if ((method & NB_BARNES_HUT) == NB_BARNES_HUT) {
-            if (s->coll_mode != NB_NO_COLL_CHECK)
-                throw ApiError(NB_ERR_NOT_SUPPORTED, "Barnes-Hut with collisions (bh_s_coll) is not on this engine's path");
+            // with collisions each step ends with bh_s_coll (bounces only, also
+            // in the mergers build: gregsys.hpp:2937-2939)
             bh_forces(s, nullptr);
             launch_kick(s->stream, s->n, s->a.p, s->v.p, s->half_dt);  // initial half kick (:2910-2913)
             bh_steps(s, s->iterations);

@@ -93,9 +93,17 @@ struct BHWork {
     int64_t nodes = 0;
     uint32_t *h_total = nullptr;
     int *h_err = nullptr;
+    // collision pass
+    DBuf<double> nbox;      // node corners btm[3], top[3]
+    DBuf<int32_t> leaf_of;  // sorted position -> leaf node
+    DBuf<uint32_t> bc_cnt, bc_blk, bc_total;  // overlaps per sorted body, then their exclusive scan
+    DBuf<int32_t> e_own, e_nb;                // overlaps: owner body, neighbour body
+    DBuf<double> e_geo, e_mom;                // distance and normal; axis momenta
+    uint32_t *h_bc_total = nullptr;
     ~BHWork() {
         if (h_total) (void)hipHostFree(h_total);
         if (h_err) (void)hipHostFree(h_err);
+        if (h_bc_total) (void)hipHostFree(h_bc_total);
     }
 };
 
@@ -401,6 +409,165 @@ __global__ void k_side(int32_t total, const double *__restrict__ box, const int3
         hi = up ? hi : mid;
     }
     side[x] = hi - lo;
+}
+
+// ---- collision pass on the tree: system::bh_s_coll (gregsys.hpp:631-686) ----
+// Node cube corners (btm xyz, top xyz), replaying set_corners' recursion on
+// all three axes along the node's digits (the same FP64 midpoints as k_keys).
+__global__ void k_node_box(int32_t total, const double *__restrict__ box, const int32_t *__restrict__ first,
+                           const uint8_t *__restrict__ depth, const uint64_t *__restrict__ khi,
+                           const uint64_t *__restrict__ klo, double *__restrict__ nbox) {
+    const int32_t x = blockIdx.x * kBlock + threadIdx.x;
+    if (x >= total) return;
+    const int32_t s = first[x];
+    const uint64_t h = khi[s], l = klo[s];
+    double lo[3] = {box[0], box[1], box[2]}, hi[3] = {box[3], box[4], box[5]};
+    const int dep = depth[x];
+    for (int L = 1; L <= dep; ++L) {
+        const unsigned d = digit_at(h, l, L);
+        for (int c = 0; c < 3; ++c) {
+            double mid = (lo[c] + hi[c]) * 0.5;
+            bool up = (d >> c) & 1u;
+            lo[c] = up ? mid : lo[c];
+            hi[c] = up ? hi[c] : mid;
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        nbox[(size_t)6 * x + c] = lo[c];
+        nbox[(size_t)6 * x + 3 + c] = hi[c];
+    }
+}
+
+__global__ void k_leaf_of(int32_t total, const int32_t *__restrict__ body, const int32_t *__restrict__ first,
+                          int32_t *__restrict__ leaf_of) {
+    const int32_t x = blockIdx.x * kBlock + threadIdx.x;
+    if (x < total && body[x] >= 0) leaf_of[first[x]] = x;
+}
+
+// nn_iterator (greg8tree.hpp:615-700) for the body at sorted position s: the
+// leaves whose closed box touches the own leaf's box, in DFS pre-order with
+// octants 0..7 (stackless over the skip pointers from the root's first child),
+// the own leaf skipped.  For each, the reference's overlap test in FP64:
+// r_o + r_j > |p_j - p_o|, |.| the square root of the ordered sum of squares,
+// n = (p_j - p_o)/|.|.  FILL = false counts the overlaps of s; FILL = true
+// writes them, in visit order, from out_off[s].
+template <bool FILL>
+__global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, const uint32_t *__restrict__ idx,
+                                                     const int32_t *__restrict__ leaf_of,
+                                                     const double *__restrict__ nbox, const int32_t *__restrict__ skip,
+                                                     const int32_t *__restrict__ body, const double *__restrict__ x,
+                                                     const double *__restrict__ r, uint32_t *__restrict__ cnt,
+                                                     int32_t *__restrict__ e_own, int32_t *__restrict__ e_nb,
+                                                     double *__restrict__ e_geo) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const int32_t own = leaf_of[s];
+    const int32_t o = (int32_t)idx[s];
+    double bb[6];
+    for (int k = 0; k < 6; ++k) bb[k] = nbox[(size_t)6 * own + k];
+    const double xo = x[o], yo = x[n + o], zo = x[2 * n + o], ro = r[o];
+    uint32_t k = FILL ? cnt[s] : 0u;
+    int32_t p = 1;
+    while (p < total) {
+        if (p == own) {
+            p = skip[p];
+            continue;
+        }
+        const double *cb = nbox + (size_t)6 * p;
+        const bool touch = !(bb[3] < cb[0] || bb[0] > cb[3] || bb[4] < cb[1] || bb[1] > cb[4] || bb[5] < cb[2] ||
+                             bb[2] > cb[5]);
+        if (!touch) {
+            p = skip[p];
+            continue;
+        }
+        const int32_t j = body[p];
+        if (j < 0) {  // internal: descend
+            ++p;
+            continue;
+        }
+        const double rad = ro + r[j];
+        const double dx = x[j] - xo, dy = x[n + j] - yo, dz = x[2 * n + j] - zo;
+        const double dist = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+        if (rad > dist) {
+            if (FILL) {
+                e_own[k] = o;
+                e_nb[k] = j;
+                e_geo[(size_t)4 * k + 0] = dist;
+                e_geo[(size_t)4 * k + 1] = dx / dist;
+                e_geo[(size_t)4 * k + 2] = dy / dist;
+                e_geo[(size_t)4 * k + 3] = dz / dist;
+            }
+            ++k;
+        }
+        p = skip[p];
+    }
+    if (!FILL) cnt[s] = k;
+}
+
+// The bounces in order: owners in leaf order (sorted order), each owner's
+// overlaps sorted by distance with an equal distance keeping the first found
+// (the ordered map's emplace), axis momenta m v . n taken from the velocities
+// as they stand when the owner is reached (its scan reads them all before any
+// of its bounces), then s_coll's bounce arithmetic (gregsys.hpp:660-684).
+// Serial by nature: one lane does it.
+__global__ void k_bhc_replay(int64_t n, uint32_t nent, const int32_t *__restrict__ e_own, int32_t *__restrict__ e_nb,
+                             double *__restrict__ e_geo, double *__restrict__ e_mom, const double *__restrict__ m,
+                             const double *__restrict__ rc, double *__restrict__ v) {
+    if (threadIdx.x != 0) return;
+    uint32_t k = 0;
+    while (k < nent) {
+        const int32_t o = e_own[k];
+        uint32_t ke = k + 1;
+        while (ke < nent && e_own[ke] == o) ++ke;
+        for (uint32_t a = k + 1; a < ke; ++a) {  // stable insertion sort by distance
+            const int32_t jn = e_nb[a];
+            double g[4];
+            for (int q = 0; q < 4; ++q) g[q] = e_geo[4 * (size_t)a + q];
+            uint32_t b = a;
+            while (b > k && e_geo[4 * (size_t)(b - 1)] > g[0]) {
+                e_nb[b] = e_nb[b - 1];
+                for (int q = 0; q < 4; ++q) e_geo[4 * (size_t)b + q] = e_geo[4 * (size_t)(b - 1) + q];
+                --b;
+            }
+            e_nb[b] = jn;
+            for (int q = 0; q < 4; ++q) e_geo[4 * (size_t)b + q] = g[q];
+        }
+        const double mo = m[o], eo = rc[o];
+        double vo[3] = {v[o], v[n + o], v[2 * n + o]};
+        for (uint32_t q = k; q < ke; ++q) {
+            const int32_t j = e_nb[q];
+            const double *g = e_geo + 4 * (size_t)q;
+            const double mj = m[j];
+            e_mom[2 * (size_t)q] = (mo * vo[0]) * g[1] + (mo * vo[1]) * g[2] + (mo * vo[2]) * g[3];
+            e_mom[2 * (size_t)q + 1] = (mj * v[j]) * g[1] + (mj * v[n + j]) * g[2] + (mj * v[2 * n + j]) * g[3];
+        }
+        for (uint32_t q = k; q < ke; ++q) {
+            const double *g = e_geo + 4 * (size_t)q;
+            if (q > k && g[0] == e_geo[4 * (size_t)(q - 1)]) continue;  // equal key: the first one stays
+            const int32_t j = e_nb[q];
+            const double amo = e_mom[2 * (size_t)q], ami = e_mom[2 * (size_t)q + 1];
+            const double mj = m[j];
+            const double o_vel = amo / mo;
+            const double i_vel = ami / mj;
+            const double o_minus_i = o_vel - i_vel;
+            const double avg_rest = (eo + rc[j]) / 2;
+            if (o_vel > 0 || i_vel < 0) {
+                if (o_vel <= 0 && i_vel < 0)
+                    if (o_vel < i_vel) continue;
+                if (o_vel > 0 && i_vel >= 0)
+                    if (o_vel < i_vel) continue;
+                const double tot = amo + ami;
+                const double ms = mo + mj;
+                const double new_o = (tot - mj * avg_rest * o_minus_i) / ms;
+                const double new_i = (tot + mo * avg_rest * o_minus_i) / ms;
+                const double fo = new_o - o_vel, fi = new_i - i_vel;
+                vo[0] += fo * g[1]; vo[1] += fo * g[2]; vo[2] += fo * g[3];
+                v[j] += fi * g[1]; v[n + j] += fi * g[2]; v[2 * n + j] += fi * g[3];
+            }
+        }
+        v[o] = vo[0]; v[n + o] = vo[1]; v[2 * n + o] = vo[2];
+        k = ke;
+    }
 }
 
 // ---- 5. mass and COM bottom up, one launch per depth (deepest first) ----
@@ -777,13 +944,55 @@ void bh_forces(nb_system *s, int64_t *interactions) {
     s->acc_valid = true;
 }
 
-// evolve's KDK + Barnes-Hut loop body (gregsys.hpp:2914-2936): drift a full
-// step, rebuild, walk, kick a full step (velocities stay half a step ahead).
+// bh_s_coll (gregsys.hpp:631-686) on the tree this step built: overlaps of
+// touching leaves counted per body in parallel, then (when there are any)
+// written in leaf order and bounced in order by one lane.  Synchronises once
+// to size the overlap list.
+static void bh_collision_pass(nb_system *s) {
+    BHWork &w = *s->bh;
+    const int64_t n = s->n;
+    const int32_t total = (int32_t)w.nodes;
+    hipStream_t st = s->stream;
+    ScopedTimer tm(s->prof, "bh_coll", st);
+    w.nbox.ensure(6 * (size_t)w.cap_nodes);
+    w.leaf_of.ensure((size_t)w.cap_n);
+    w.bc_cnt.ensure((size_t)w.cap_n);
+    w.bc_blk.ensure((size_t)(w.cap_n + 1023) / 1024 + 1);
+    w.bc_total.ensure(1);
+    if (!w.h_bc_total) NB_HIP(hipHostMalloc((void **)&w.h_bc_total, sizeof(uint32_t), hipHostMallocDefault));
+    hipLaunchKernelGGL(k_node_box, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.box.p, w.first.p, w.depth.p,
+                       w.khi.p, w.klo.p, w.nbox.p);
+    hipLaunchKernelGGL(k_leaf_of, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.body.p, w.first.p, w.leaf_of.p);
+    hipLaunchKernelGGL(k_bhc_scan<false>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.leaf_of.p,
+                       w.nbox.p, w.skip.p, w.body.p, s->x.p, s->r.p, w.bc_cnt.p, nullptr, nullptr, nullptr);
+    const int nb = (int)((n + 1023) / 1024);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.bc_cnt.p, w.bc_blk.p);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nb, w.bc_blk.p, w.bc_total.p);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, st, n, w.bc_cnt.p, w.bc_blk.p);
+    NB_HIP(hipMemcpyAsync(w.h_bc_total, w.bc_total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    NB_HIP(hipStreamSynchronize(st));
+    const uint32_t nent = *w.h_bc_total - 1u;  // the scan's total counts one extra (the tree root)
+    if (nent == 0) return;
+    if (w.e_own.n < nent) {
+        const size_t cap = (size_t)nent + nent / 4 + 64;
+        w.e_own.ensure(cap); w.e_nb.ensure(cap); w.e_geo.ensure(4 * cap); w.e_mom.ensure(2 * cap);
+    }
+    hipLaunchKernelGGL(k_bhc_scan<true>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.leaf_of.p,
+                       w.nbox.p, w.skip.p, w.body.p, s->x.p, s->r.p, w.bc_cnt.p, w.e_own.p, w.e_nb.p, w.e_geo.p);
+    hipLaunchKernelGGL(k_bhc_replay, dim3(1), dim3(64), 0, st, n, nent, w.e_own.p, w.e_nb.p, w.e_geo.p, w.e_mom.p,
+                       s->m.p, s->e.p, s->v.p);
+    NB_HIP(hipGetLastError());
+}
+
+// evolve's KDK + Barnes-Hut loop body (gregsys.hpp:2914-2939): drift a full
+// step, rebuild, walk, kick a full step (velocities stay half a step ahead),
+// then bh_s_coll when collisions are on.
 void bh_steps(nb_system *s, uint64_t steps) {
     for (uint64_t k = 0; k < steps; ++k) {
         launch_drift(s->stream, s->n, s->v.p, s->x.p, s->dt);
         bh_forces(s, nullptr);
         launch_kick(s->stream, s->n, s->a.p, s->v.p, s->dt);
+        if (s->coll_mode == NB_OVERLAP_COLL_CHECK) bh_collision_pass(s);
         ++s->step_count;
     }
 }

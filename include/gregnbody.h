@@ -112,7 +112,9 @@ int nb_set_precision(nb_system *sys, int precision);
  * steps of NB_LEAPFROG_KDK (direct summation, or | NB_BARNES_HUT), or of the
  * reference's other direct-sum integrators NB_EULER, NB_MODIFIED_EULER,
  * NB_MIDPOINT (:2666-2891) and NB_LEAPFROG_DKD (:2993-3039), with s_coll
- * after every step when collisions are on; the first matching flag wins, as
+ * after every step when collisions are on (with Barnes-Hut, bh_s_coll
+ * :631-686 instead: bounces between bodies of touching tree cells, never a
+ * merge, :2937-2939); the first matching flag wins, as
  * in the reference's dispatch.  Like the reference, a KDK call first
  * recomputes a(x0).  elapsed_ns (may be NULL) receives the host wall time of
  * the call (evolve's return value).  NB_ERR_NOT_SUPPORTED: rk4 / rk3_8 (empty
@@ -210,7 +212,7 @@ int nb_autotune(nb_system *sys, int method, const char *cache_path);
 
 /* Kernel timing (engine extension): when enabled, every launch of the named
  * kernel class is bracketed by HIP events on the context's stream.
- * names: "force", "update", "energy", "bh_build", "bh_walk", "adjoint". */
+ * names: "force", "update", "energy", "bh_build", "bh_walk", "bh_coll", "adjoint". */
 int nb_profile_enable(nb_system *sys, int on);
 int nb_profile_read(nb_system *sys, const char *name, double *total_ms, int64_t *launches);
 

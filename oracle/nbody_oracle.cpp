@@ -188,6 +188,34 @@ struct Overlap { int64_t idx; T mo, mi; T n[3]; };
 template <typename T>
 inline T dot3(const T *a, const T *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
+/* Bounce pass over the overlaps of one outer body in distance order
+ * (gregsys.hpp:810-837; the same arithmetic in bh_s_coll :660-684): axis
+ * momenta were captured when the overlap was found. */
+template <typename T, typename Key>
+void bounce_all(CBody<T> *bo, std::map<Key, Overlap<T>> &overlapping, std::vector<CBody<T>> &bods) {
+    for (auto &kv : overlapping) {
+        Overlap<T> &ov = kv.second;
+        CBody<T> &bi = bods[ov.idx];
+        T o_vel = ov.mo / bo->m;
+        T i_vel = ov.mi / bi.m;
+        T o_minus_i = o_vel - i_vel;
+        T avg_rest = (bo->e + bi.e) / 2;
+        if (o_vel > 0 || i_vel < 0) {
+            if (o_vel <= 0 && i_vel < 0)
+                if (o_vel < i_vel) continue;
+            if (o_vel > 0 && i_vel >= 0)
+                if (o_vel < i_vel) continue;
+            T tot = ov.mo + ov.mi;
+            T msum = bo->m + bi.m;
+            T new_o = (tot - bi.m * avg_rest * o_minus_i) / msum;
+            T new_i = (tot + bo->m * avg_rest * o_minus_i) / msum;
+            T fo = new_o - o_vel, fi = new_i - i_vel;
+            for (int k = 0; k < 3; ++k) bo->v[k] += fo * ov.n[k];
+            for (int k = 0; k < 3; ++k) bi.v[k] += fi * ov.n[k];
+        }
+    }
+}
+
 /* One s_coll pass.  merge=false is the default build; merge=true is the
  * -DGREGSYS_MERGERS build.  Follows gregsys.hpp:690-914 including: overlaps
  * keyed by distance in an ordered map (equal distances dropped, :760-761);
@@ -262,28 +290,7 @@ void s_coll(std::vector<CBody<T>> &bods, bool merge, T min_tot_com_mom, int64_t 
             overlapping.clear();
             rescan = true;
         }
-        /* bounce pass (gregsys.hpp:810-837) */
-        for (auto &kv : overlapping) {
-            Overlap<T> &ov = kv.second;
-            CBody<T> &bi = bods[ov.idx];
-            T o_vel = ov.mo / bo->m;
-            T i_vel = ov.mi / bi.m;
-            T o_minus_i = o_vel - i_vel;
-            T avg_rest = (bo->e + bi.e) / 2;
-            if (o_vel > 0 || i_vel < 0) {
-                if (o_vel <= 0 && i_vel < 0)
-                    if (o_vel < i_vel) continue;
-                if (o_vel > 0 && i_vel >= 0)
-                    if (o_vel < i_vel) continue;
-                T tot = ov.mo + ov.mi;
-                T msum = bo->m + bi.m;
-                T new_o = (tot - bi.m * avg_rest * o_minus_i) / msum;
-                T new_i = (tot + bo->m * avg_rest * o_minus_i) / msum;
-                T fo = new_o - o_vel, fi = new_i - i_vel;
-                for (int k = 0; k < 3; ++k) bo->v[k] += fo * ov.n[k];
-                for (int k = 0; k < 3; ++k) bi.v[k] += fi * ov.n[k];
-            }
-        }
+        bounce_all<T>(bo, overlapping, bods);
         if (first_merged) bods[outer] = merged;
         ++outer;
     }
@@ -604,6 +611,110 @@ static void walk_body(const Tree &t, int64_t own_leaf, const double *p, double G
     }
 }
 
+/* nn_iterator (greg8tree.hpp:615-700): the bodies of the leaf cells whose
+ * closed box touches the own leaf's box (faces, edges and corners count),
+ * found by a DFS from the root's children in octant order that descends into
+ * touching internal cells and skips the own leaf; a body alone in the root
+ * cell has none.  (The iterator's climb back up rescans the rest of the level
+ * it just finished, which finds nothing new: it neither repeats nor skips a
+ * cell, so the plain DFS yields the same sequence.) */
+static inline bool touches(const Cell &a, const Cell &b) {
+    return !(a.top[0] < b.btm[0] || a.btm[0] > b.top[0] || a.top[1] < b.btm[1] || a.btm[1] > b.top[1] ||
+             a.top[2] < b.btm[2] || a.btm[2] > b.top[2]);
+}
+static void nn_bodies(const Tree &t, int64_t own, std::vector<int64_t> &out) {
+    out.clear();
+    const Cell &bc = t.cells[own];
+    if (bc.parent < 0) return;
+    std::vector<int64_t> stack;
+    for (int o = 7; o >= 0; --o)
+        if (t.cells[0].sub[o] >= 0) stack.push_back(t.cells[0].sub[o]);
+    while (!stack.empty()) {
+        int64_t c = stack.back();
+        stack.pop_back();
+        if (c == own) continue;
+        const Cell &cell = t.cells[c];
+        if (!touches(bc, cell)) continue;
+        if (cell.body >= 0) {
+            out.push_back(cell.body);
+            continue;
+        }
+        for (int o = 7; o >= 0; --o)
+            if (cell.sub[o] >= 0) stack.push_back(cell.sub[o]);
+    }
+}
+
+/* Leaves in DFS pre-order, octants 0..7: the order of bh_tree::begin() and
+ * body_iterator::operator++ (greg8tree.hpp:502-614, the Morton order of the
+ * leaves). */
+static void leaves_in_order(const Tree &t, std::vector<int64_t> &out) {
+    out.clear();
+    std::vector<int64_t> stack{0};
+    while (!stack.empty()) {
+        int64_t c = stack.back();
+        stack.pop_back();
+        const Cell &cell = t.cells[c];
+        if (cell.body >= 0) {
+            out.push_back(c);
+            continue;
+        }
+        for (int o = 7; o >= 0; --o)
+            if (cell.sub[o] >= 0) stack.push_back(cell.sub[o]);
+    }
+}
+
+/* system::bh_s_coll (gregsys.hpp:631-686): for every body in leaf order, the
+ * touching-cell neighbours j (nn_iterator order) with r_o + r_j > |p_j - p_o|
+ * are keyed by that distance in an ordered map (an equal distance keeps the
+ * first), with the axis momenta m v . n taken when found, and bounced in
+ * distance order as in s_coll.  Every pair is met from both sides; no merges.
+ * The tree is the one built this step (positions do not move after it). */
+template <typename T>
+void bh_s_coll(const Tree &t, std::vector<CBody<T>> &bods) {
+    std::vector<int64_t> leaves, nb;
+    leaves_in_order(t, leaves);
+    std::map<T, Overlap<T>> overlapping;
+    for (int64_t leaf : leaves) {
+        overlapping.clear();
+        CBody<T> *bo = &bods[(size_t)t.cells[leaf].body];
+        nn_bodies(t, leaf, nb);
+        for (int64_t j : nb) {
+            CBody<T> &bi = bods[(size_t)j];
+            T rad_dist = bo->r + bi.r;
+            T d[3] = {bi.p[0] - bo->p[0], bi.p[1] - bo->p[1], bi.p[2] - bo->p[2]};
+            T dist = t_sqrt<T>(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            d[0] /= dist; d[1] /= dist; d[2] /= dist;
+            if (rad_dist > dist) {
+                T mom_o[3] = {bo->m * bo->v[0], bo->m * bo->v[1], bo->m * bo->v[2]};
+                T mom_i[3] = {bi.m * bi.v[0], bi.m * bi.v[1], bi.m * bi.v[2]};
+                Overlap<T> ov{j, dot3<T>(mom_o, d), dot3<T>(mom_i, d), {d[0], d[1], d[2]}};
+                overlapping.emplace(dist, ov);
+            }
+        }
+        if (!overlapping.empty()) bounce_all<T>(bo, overlapping, bods);
+    }
+}
+
+template <typename T>
+int bh_s_coll_arrays(int64_t n, const T *m, const T *r, const T *e, const T *pos, T *vel) {
+    std::vector<double> pd((size_t)(3 * n)), md((size_t)n);
+    for (int64_t i = 0; i < 3 * n; ++i) pd[(size_t)i] = (double)pos[i];
+    for (int64_t i = 0; i < n; ++i) md[(size_t)i] = (double)m[i];
+    Tree t;
+    int rc = build_tree(t, n, md.data(), pd.data());
+    if (rc) return rc;
+    std::vector<CBody<T>> bods((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        CBody<T> &b = bods[(size_t)i];
+        b.label = i; b.m = m[i]; b.r = r[i]; b.e = e[i];
+        for (int k = 0; k < 3; ++k) { b.p[k] = pos[3 * i + k]; b.v[k] = vel[3 * i + k]; b.a[k] = 0; }
+    }
+    bh_s_coll<T>(t, bods);
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) vel[3 * i + k] = bods[(size_t)i].v[k];
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -865,6 +976,54 @@ int orc_bh_kdk_d(int64_t n, double G, double dt, double eps, double theta, int64
         rc = orc_bh_acc_d(n, G, eps, theta, m, pos, acc, &cnt, nullptr);
         if (rc) return rc;
         for (int64_t i = 0; i < 3 * n; ++i) vel[i] += dt * acc[i];
+    }
+    return 0;
+}
+
+/* One bh_s_coll pass on a state (velocities updated in place), in long double
+ * (the reference's system<long double> arithmetic) or in double (the GPU's
+ * FP64 form).  The octree is built from the positions in double, as in the
+ * other BH functions here. */
+int orc_bh_s_coll_ld(int64_t n, const long double *m, const long double *r, const long double *e,
+                     const long double *pos, long double *vel) {
+    return bh_s_coll_arrays<long double>(n, m, r, e, pos, vel);
+}
+int orc_bh_s_coll_d(int64_t n, const double *m, const double *r, const double *e, const double *pos, double *vel) {
+    return bh_s_coll_arrays<double>(n, m, r, e, pos, vel);
+}
+
+/* Neighbour bodies of body b as nn_iterator yields them (for tests). */
+int64_t orc_bh_neighbours_d(int64_t n, const double *m, const double *pos, int64_t b, int64_t cap, int64_t *out) {
+    Tree t;
+    int rc = build_tree(t, n, m, pos);
+    if (rc) return rc;
+    int64_t leaf = -1;
+    for (size_t c = 0; c < t.cells.size(); ++c)
+        if (t.cells[c].body == b) leaf = (int64_t)c;
+    std::vector<int64_t> nb;
+    nn_bodies(t, leaf, nb);
+    for (size_t k = 0; k < nb.size() && (int64_t)k < cap; ++k) out[k] = nb[k];
+    return (int64_t)nb.size();
+}
+
+/* orc_bh_kdk_d followed, after each step's kick, by bh_s_coll on that step's
+ * tree when coll != 0 (gregsys.hpp:2914-2939). */
+int orc_bh_kdk_coll_d(int64_t n, double G, double dt, double eps, double theta, int64_t steps, int coll,
+                      const double *m, const double *r, const double *e, double *pos, double *vel, double *acc) {
+    double half_dt = dt / 2;
+    int64_t cnt;
+    int rc = orc_bh_acc_d(n, G, eps, theta, m, pos, acc, &cnt, nullptr);
+    if (rc) return rc;
+    for (int64_t i = 0; i < 3 * n; ++i) vel[i] += half_dt * acc[i];
+    for (int64_t s = 0; s < steps; ++s) {
+        for (int64_t i = 0; i < 3 * n; ++i) pos[i] += dt * vel[i];
+        rc = orc_bh_acc_d(n, G, eps, theta, m, pos, acc, &cnt, nullptr);
+        if (rc) return rc;
+        for (int64_t i = 0; i < 3 * n; ++i) vel[i] += dt * acc[i];
+        if (coll) {
+            rc = bh_s_coll_arrays<double>(n, m, r, e, pos, vel);
+            if (rc) return rc;
+        }
     }
     return 0;
 }

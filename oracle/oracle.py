@@ -63,6 +63,12 @@ def lib():
         L.orc_max_threads.restype = C.c_int
         L.orc_integrate_ld.argtypes = [C.c_int] + L.orc_kdk_ld.argtypes
         L.orc_integrate_d.argtypes = [C.c_int] + L.orc_kdk_d.argtypes
+        L.orc_bh_s_coll_ld.argtypes = [C.c_int64, _LDP, _LDP, _LDP, _LDP, _LDP]
+        L.orc_bh_s_coll_d.argtypes = [C.c_int64, _D, _D, _D, _D, _D]
+        L.orc_bh_neighbours_d.restype = C.c_int64
+        L.orc_bh_neighbours_d.argtypes = [C.c_int64, _D, _D, C.c_int64, C.c_int64, _I64]
+        L.orc_bh_kdk_coll_d.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
+                                        C.c_int, _D, _D, _D, _D, _D, _D]
         L.orc_trace.argtypes = [C.c_int]
         L.orc_trace_len.restype = C.c_int64
         L.orc_trace_copy.argtypes = [_I64, _LDP]
@@ -264,6 +270,40 @@ def bh_kdk(m, pos, vel, G, dt, eps, theta, steps):
     return pos, vel, acc
 
 
+def bh_s_coll(m, r, e, pos, vel, dtype=LD):
+    """One bh_s_coll pass (touching-cell neighbours, bounces) on a state;
+    returns the new velocities."""
+    n = len(m)
+    conv, ptr = (_ld, _lp) if dtype == LD else (_d, _dp)
+    m, r, e, pos, vel = conv(m), conv(r), conv(e), conv(pos), conv(vel).copy()
+    f = lib().orc_bh_s_coll_ld if dtype == LD else lib().orc_bh_s_coll_d
+    rc = f(n, ptr(m), ptr(r), ptr(e), ptr(pos), ptr(vel))
+    if rc:
+        raise RuntimeError(f"oracle bh_s_coll failed: {rc}")
+    return vel
+
+
+def bh_neighbours(m, pos, body):
+    """The bodies nn_iterator yields for `body`, in its order."""
+    n = len(m)
+    out = np.zeros(n, np.int64)
+    k = lib().orc_bh_neighbours_d(n, _dp(_d(m)), _dp(_d(pos)), body, out.size, _ip(out))
+    if k < 0:
+        raise RuntimeError(f"oracle bh_neighbours failed: {k}")
+    return out[:k]
+
+
+def bh_kdk_coll(m, r, e, pos, vel, G, dt, eps, theta, steps, coll=True):
+    n = len(m)
+    m, r, e, pos, vel = _d(m), _d(r), _d(e), _d(pos).copy(), _d(vel).copy()
+    acc = np.zeros((n, 3))
+    rc = lib().orc_bh_kdk_coll_d(n, G, dt, eps, theta, steps, int(coll), _dp(m), _dp(r), _dp(e), _dp(pos), _dp(vel),
+                                 _dp(acc))
+    if rc:
+        raise RuntimeError(f"oracle bh_kdk_coll failed: {rc}")
+    return pos, vel, acc
+
+
 def adjoint(m, x0, v0, G, dt, eps, steps, gxT, gvT):
     n = len(m)
     gx0, gv0, gm = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
@@ -295,6 +335,14 @@ def write_ref_input(path, m, pos, vel, r=None, e=None, steps=0, dt=1.0):
         f.write(np.array([n, steps], dtype=np.int64).tobytes())
         f.write(np.array([dt], dtype=np.float64).tobytes())
         f.write(rec.astype(np.float64).tobytes())
+
+
+def ref_bhcoll(m, r, e, pos, vel, tmpdir):
+    """The compiled reference's bh_s_coll on one state: velocities (long double)."""
+    inp, out = os.path.join(tmpdir, "bhc_in.bin"), os.path.join(tmpdir, "bhc_out.bin")
+    write_ref_input(inp, m, pos, vel, r=r, e=e, steps=1, dt=1.0)
+    subprocess.run([os.path.join(REF_DIR, "ref_bhcoll"), inp, out], check=True, timeout=600)
+    return np.fromfile(out, dtype=LD).reshape(len(m), 3)
 
 
 def run_ref(mode, inp, out, merge=False, timeout=3600):

@@ -16,9 +16,13 @@ generators; outputs are the reference's own results, stored as .npz:
   ref_icoll_{method}_{bounce,merge}_seed1.npz  the same integrators with
                          collisions = 3, rubble cluster: every body after
                          every 25th step, and the final body count.
+  ref_bhcoll_n{N}_seed{K}.npz  one bh_s_coll pass (oracle/ref_bhcoll, the
+                         reference's own private method on its own tree) on
+                         a dense rubble cluster: inputs and the velocities
+                         after the pass (x87 long double).
 
-Usage: python tests/golden/make_fixtures.py [integ]   (integ: only the
-       integrator fixtures)
+Usage: python tests/golden/make_fixtures.py [integ|bhcoll]   (only the
+       integrator / only the bh_s_coll fixtures)
 """
 import os
 import sys
@@ -39,6 +43,7 @@ KDK_CASES = [(2, 1), (2, 1000), (3, 10), (64, 10), (64, 1000), (1024, 10), (1024
 TREE_CASES = [2, 3, 17, 1000, 16384]
 COLL_CASES = [(24, 300, s) for s in range(4)]
 INTEG_METHODS = ["euler", "modeuler", "midpoint", "dkd"]
+BHCOLL_CASES = [(300, 7, 2.5), (2000, 8, 1.5), (64, 9, 4.0)]  # (n, seed, radius factor)
 
 
 def coll_ics(n, seed):
@@ -74,6 +79,21 @@ def integ_fixtures(inp, out):
             print("icoll", meth, tag, len(st))
 
 
+def bhcoll_ics(n, seed, rfac):
+    """Rubble cluster with radii inflated by rfac: many overlapping pairs."""
+    m, r, e, pos, vel = coll_ics(n, seed)
+    return m, r * rfac, e, pos, vel
+
+
+def bhcoll_fixtures(tmp):
+    for n, seed, rfac in BHCOLL_CASES:
+        m, r, e, pos, vel = bhcoll_ics(n, seed, rfac)
+        v = O.ref_bhcoll(m, r, e, pos, vel, tmp)
+        np.savez_compressed(os.path.join(OUT, f"ref_bhcoll_n{n}_seed{seed}.npz"), m=m, r=r, e=e, pos=pos, vel=vel,
+                            vel_after=v)
+        print("bhcoll", n, seed, int((v != vel.astype(O.LD)).any(axis=1).sum()), "bodies bounced")
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref is not built (needs /root/reference): nothing generated")
@@ -82,6 +102,10 @@ def main():
     if sys.argv[1:] == ["integ"]:
         integ_fixtures(inp, out)
         return
+    if sys.argv[1:] == ["bhcoll"]:
+        bhcoll_fixtures(tmp)
+        return
+    bhcoll_fixtures(tmp)
     integ_fixtures(inp, out)
     for n, steps in KDK_CASES:
         m, pos, vel = capi.ic_uniform_ball(0, n, 1e20, 1e22, 1e9, 1e3)
