@@ -81,6 +81,9 @@ SIGNATURES = {
     "nb_advance": (C.c_int, [_P, C.c_uint64, _I64]),
     "nb_energy": (C.c_int, [_P, _D, _D]),
     "nb_merge_log": (C.c_int, [_P, C.c_int64, _I64, _I64]),
+    "nb_set_recording": (C.c_int, [_P, C.c_uint64, C.c_uint64]),
+    "nb_record_count": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "nb_get_record": (C.c_int, [_P, C.c_uint64, _I64, _I64, _D, _D, _I64, _D, _D]),
     "nb_tree_topology": (C.c_int, [_P, C.c_int64, _I32, _I32, _I64, _I64, _D, _D, _D, _I64]),
     "nb_bh_accelerations": (C.c_int, [_P, _D, _I64]),
     "nb_adjoint": (C.c_int, [_P, C.c_uint64, C.c_uint64, _D, _D, _D, _D, _D, _I64, _I64]),
@@ -259,6 +262,25 @@ class System:
         ke, pe = C.c_double(), C.c_double()
         self._ck(self._L.nb_energy(self.h, C.byref(ke), C.byref(pe)))
         return ke.value, pe.value
+
+    def set_recording(self, every, capacity):
+        self._ck(self._L.nb_set_recording(self.h, every, capacity))
+
+    def record_count(self):
+        taken, stored = C.c_uint64(), C.c_uint64()
+        self._ck(self._L.nb_record_count(self.h, C.byref(taken), C.byref(stored)))
+        return taken.value, stored.value
+
+    def record(self, k):
+        """Stored record k (0 = oldest kept): dict(step, pos, vel, ids, ke, pe)."""
+        step, n = C.c_int64(), C.c_int64()
+        self._ck(self._L.nb_get_record(self.h, k, C.byref(step), C.byref(n), None, None, None, None, None))
+        nb = n.value
+        pos, vel = np.zeros((nb, 3)), np.zeros((nb, 3))
+        ids = np.zeros(max(nb, 1), np.int64)
+        ke, pe = C.c_double(), C.c_double()
+        self._ck(self._L.nb_get_record(self.h, k, None, None, dp(pos), dp(vel), ip(ids), C.byref(ke), C.byref(pe)))
+        return dict(step=step.value, pos=pos, vel=vel, ids=ids[:nb], ke=ke.value, pe=pe.value)
 
     def merge_log(self):
         cnt = C.c_int64()

@@ -220,6 +220,7 @@ void direct_other_steps(nb_system *s, int method, uint64_t steps) {
         }
         ++s->step_count;
         if (coll) collision_pass(s, s->step_count);
+        if (record_due(s, s->step_count)) record_state(s, s->step_count);
     }
 }
 
@@ -278,7 +279,7 @@ void direct_kdk_steps(nb_system *s, uint64_t steps) {
     };
     // Interior steps of small systems replay a captured graph of kGraphSteps
     // steps (no collisions, no per-launch timing).
-    const bool graphs = !coll && !s->prof.on &&
+    const bool graphs = !coll && !s->prof.on && !s->rec_every &&
                         (s->tune_step_graphs == 1 || (s->tune_step_graphs == 0 && s->n <= kGraphMaxBodies));
     bool need_open = true;
     for (uint64_t st = 1; st <= steps;) {
@@ -300,7 +301,8 @@ void direct_kdk_steps(nb_system *s, uint64_t steps) {
         }
         if (fast64 && s->dw.partial64.n < (size_t)s->dw.nsplit64 * 3 * (size_t)n) direct_forces(s);
         if (mixed && s->dw.pos32.n < (size_t)n) direct_forces(s);  // defensive: sizes follow n
-        need_open = issue_step(need_open, st == steps || coll);
+        // a step that records or runs collisions ends with its closing kick
+        need_open = issue_step(need_open, st == steps || coll || record_due(s, s->step_count + 1));
         ++st;
         ++s->step_count;
         if (coll) {
@@ -311,6 +313,7 @@ void direct_kdk_steps(nb_system *s, uint64_t steps) {
                 s->dw.partial.ensure((size_t)s->dw.nsplit * 6 * (size_t)s->n);
             }
         }
+        if (record_due(s, s->step_count)) record_state(s, s->step_count);
     }
 }
 
@@ -401,6 +404,8 @@ int nb_set_bodies(nb_system *s, int64_t n, const double *m, const double *r, con
         s->acc_valid = false;
         s->step_count = 0;
         s->merge_log.clear();
+        s->rec_total = 0;  // records of the previous bodies are dropped
+        s->rec_n0 = std::max<int64_t>(s->rec_n0, n);
         s->time_elapsed = 0;
         return NB_OK;
     });
@@ -551,6 +556,7 @@ int nb_evolve(nb_system *s, int method, int64_t *elapsed_ns) {
                                "Barnes-Hut is built with leapfrog_kdk only (the reference's other BH branches are "
                                "empty or do not terminate)");
             s->step_count = 0;
+            if (s->rec_every) record_state(s, 0);
             direct_other_steps(s, direct_other, s->iterations);
             s->acc_method = direct_other;
             s->acc_valid = false;
@@ -567,12 +573,14 @@ int nb_evolve(nb_system *s, int method, int64_t *elapsed_ns) {
         if ((method & NB_BARNES_HUT) == NB_BARNES_HUT) {
             // with collisions each step ends with bh_s_coll (bounces only, also
             // in the mergers build: gregsys.hpp:2937-2939)
+            if (s->rec_every) record_state(s, 0);
             bh_forces(s, nullptr);
             launch_kick(s->stream, s->n, s->a.p, s->v.p, s->half_dt);  // initial half kick (:2910-2913)
             bh_steps(s, s->iterations);
             s->acc_method = NB_BARNES_HUT;
         } else {
             direct_forces(s);
+            if (s->rec_every) record_state(s, 0);
             direct_kdk_steps(s, s->iterations);
             s->acc_method = NB_LEAPFROG_KDK;
         }
@@ -640,6 +648,63 @@ int nb_energy(nb_system *s, double *ke, double *pe) {
         }
         *ke = tke;
         *pe = tpe;
+        return NB_OK;
+    });
+}
+
+int nb_set_recording(nb_system *s, uint64_t every, uint64_t capacity) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    if (every && !capacity) return fail(s, NB_ERR_INVALID_ARGUMENT, "nb_set_recording: capacity must be positive");
+    return guarded(s, [&]() {
+        s->rec_every = every;
+        s->rec_cap = every ? capacity : 0;
+        s->rec_total = 0;
+        s->rec_n0 = s->n;
+        s->rec_step.clear();
+        s->rec_n.clear();
+        s->rec_ids.clear();
+        if (!every) {
+            s->rec_x.release();
+            s->rec_v.release();
+            s->rec_e.release();
+        }
+        return NB_OK;
+    });
+}
+
+int nb_record_count(const nb_system *s, uint64_t *taken, uint64_t *stored) {
+    if (!s || !taken || !stored) return fail(const_cast<nb_system *>(s), NB_ERR_INVALID_ARGUMENT, "NULL argument");
+    *taken = s->rec_total;
+    *stored = std::min<uint64_t>(s->rec_total, s->rec_cap);
+    return NB_OK;
+}
+
+int nb_get_record(nb_system *s, uint64_t k, int64_t *step, int64_t *n, double *pos, double *vel, int64_t *ids,
+                  double *ke, double *pe) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    const uint64_t stored = std::min<uint64_t>(s->rec_total, s->rec_cap);
+    if (k >= stored) return fail(s, NB_ERR_INVALID_ARGUMENT, "nb_get_record: no such record");
+    return guarded(s, [&]() {
+        const uint64_t slot = (s->rec_total - stored + k) % s->rec_cap;  // k = 0: the oldest kept
+        const int64_t nb = s->rec_n[slot];
+        if (step) *step = s->rec_step[slot];
+        if (n) *n = nb;
+        const size_t stride = (size_t)3 * (size_t)s->rec_n0;
+        double e2[2] = {0, 0};
+        NB_HIP(hipMemcpyAsync(e2, s->rec_e.p + 2 * slot, 2 * 8, hipMemcpyDeviceToHost, s->stream));
+        NB_HIP(hipStreamSynchronize(s->stream));
+        if (ke) *ke = e2[0];
+        if (pe) *pe = e2[1];
+        std::vector<double> buf((size_t)3 * (size_t)nb);
+        auto get = [&](const DBuf<double> &src, double *dst) {
+            if (!dst || nb == 0) return;
+            NB_HIP(hipMemcpy(buf.data(), src.p + slot * stride, buf.size() * 8, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < nb; ++i)
+                for (int c = 0; c < 3; ++c) dst[3 * i + c] = buf[(size_t)c * nb + i];
+        };
+        get(s->rec_x, pos);
+        get(s->rec_v, vel);
+        if (ids) std::copy(s->rec_ids[slot].begin(), s->rec_ids[slot].end(), ids);
         return NB_OK;
     });
 }

@@ -994,6 +994,7 @@ void bh_steps(nb_system *s, uint64_t steps) {
         launch_kick(s->stream, s->n, s->a.p, s->v.p, s->dt);
         if (s->coll_mode == NB_OVERLAP_COLL_CHECK) bh_collision_pass(s);
         ++s->step_count;
+        if (record_due(s, s->step_count)) record_state(s, s->step_count);
     }
 }
 

@@ -38,6 +38,14 @@ struct nb_system {
     long double time_elapsed = 0;  // system::time_elapsed (gregsys.hpp:206)
     std::vector<int64_t> merge_log;  // 4 per record
 
+    // state recording (nb_set_recording): a ring of rec_cap records taken
+    // every rec_every steps; a slot holds 3 * rec_n0 doubles per vector
+    uint64_t rec_every = 0, rec_cap = 0, rec_total = 0;
+    int64_t rec_n0 = 0;
+    nbe::DBuf<double> rec_x, rec_v, rec_e;   // [slot][x block | y | z], same for v; [slot][ke, pe]
+    std::vector<int64_t> rec_step, rec_n;    // per slot: step index, body count
+    std::vector<std::vector<int64_t>> rec_ids;  // per slot: labels
+
     // tuning knobs (0 = engine default); set by nb_set_tuning / nb_autotune
     int tune_bh_bucket = 0;
     int tune_direct_ipl = 0;
@@ -64,6 +72,10 @@ void direct_kdk_steps(nb_system *s, uint64_t steps); // steps of KDK from valid 
 int form_of(const nb_system *s);                     // 0 = unsoftened reference form, 1 = softened
 GraphKey graph_key(const nb_system *s);
 void set_static_error(const std::string &msg);  // message for nb_last_error(NULL)
+// state recording (nb_record.hip): a record of the current state when `step`
+// is a multiple of the recording interval
+void record_state(nb_system *s, uint64_t step);
+inline bool record_due(const nb_system *s, uint64_t step) { return s->rec_every && step % s->rec_every == 0; }
 // collisions after a step's closing kick (nb_coll.hip)
 void collision_pass(nb_system *s, uint64_t step);
 // Barnes-Hut (nb_bh.hip)

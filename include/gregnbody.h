@@ -129,6 +129,26 @@ int nb_advance(nb_system *sys, uint64_t steps, int64_t *elapsed_ns);
  * formulae (set_ke gregbod.hpp:917-920, cumulative_pe gregsys.hpp:358-363,
  * calc_energy :608-630; bh_pe's softened form when eps > 0). */
 int nb_energy(nb_system *sys, double *ke, double *pe);
+/* Engine extension in place of the memFreq template argument (gregsys.hpp:135-170,
+ * body::add_pos_vel_ke gregbod.hpp:311-315, tot_ke/tot_pe/tot_e :420-476, whose
+ * step loop records nothing on the paths that compile): every `every` steps of
+ * nb_evolve / nb_advance, and before the first step of nb_evolve, keep the
+ * positions, velocities and labels of the bodies and the total KE and PE (the
+ * nb_energy formulae, summed in body order) in a ring of `capacity` records in
+ * device memory; the oldest is overwritten first.  Recording adds no host
+ * synchronisation but turns the captured step graphs off; with Barnes-Hut the
+ * velocities are those of the loop, half a step ahead of the positions, and
+ * the PE is the direct O(N^2) sum.  every = 0 stops recording and frees the
+ * ring; nb_set_bodies drops the records. */
+int nb_set_recording(nb_system *sys, uint64_t every, uint64_t capacity);
+/* taken: records made since recording was set; stored: how many are kept. */
+int nb_record_count(const nb_system *sys, uint64_t *taken, uint64_t *stored);
+/* Record k of the stored ones (0 = the oldest kept): step index since the last
+ * nb_evolve, body count n, positions and velocities (3n interleaved), labels
+ * (n), total KE and PE.  Any output may be NULL; call with pos = vel = ids =
+ * NULL to learn n first. */
+int nb_get_record(nb_system *sys, uint64_t k, int64_t *step, int64_t *n, double *pos, double *vel, int64_t *ids,
+                  double *ke, double *pe);
 /* Merge records since nb_set_bodies: 4 int64 per record
  * (step, outer body label, absorbed body label, merged body label). */
 int nb_merge_log(const nb_system *sys, int64_t cap, int64_t *records, int64_t *count);

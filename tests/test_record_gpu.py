@@ -1,0 +1,147 @@
+"""State and energy recording every k steps (nb_set_recording), on the GPU.
+
+Bars:
+  * FP64 direct KDK (and Euler, and the collision path): every record's
+    positions and velocities are bit-identical to the double oracle's state
+    after that many steps, and its KE / PE bit-identical to the oracle's
+    energy of that state (the nb_energy formulae, summed in body order);
+  * the ring keeps the newest `capacity` records, oldest first; nb_advance
+    continues the step index;
+  * Barnes-Hut: records bit-identical to the oracle's BH loop state (the
+    velocities half a step ahead, as in the loop);
+  * mixed precision: recorded energies drift like the FP64 run's, within
+    1e-6 of |E0| over 1000 config-1 steps, read from the records.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from gregpackagescpp_amd import capi
+
+pytestmark = pytest.mark.gpu
+G_SI = float(np.longdouble(66743) / np.longdouble(10 ** 15))
+
+
+def make(m, pos, vel, dt, steps, eps=0.0, prec=capi.PREC_FP64, G=G_SI):
+    s = capi.System(0)
+    s.set_G(G)
+    s.set_timestep(dt)
+    s.set_iterations(steps)
+    s.set_softening(eps)
+    s.set_precision(prec)
+    s.set_bodies(m, pos, vel)
+    return s
+
+
+@pytest.mark.parametrize("eps,form", [(0.0, O.FORM_NEWTON), (1e6, O.FORM_SOFT)])
+def test_kdk_records_bit_exact(eps, form):
+    print(f"case record kdk eps={eps}", flush=True)
+    m, pos, vel = capi.ic_uniform_ball(31, 256, 1e20, 1e22, 1e9, 1e3)
+    s = make(m, pos, vel, 10.0, 60, eps)
+    s.set_recording(10, 16)
+    s.evolve(capi.LEAPFROG_KDK)
+    taken, stored = s.record_count()
+    assert (taken, stored) == (7, 7)
+    for k in range(stored):
+        r = s.record(k)
+        assert r["step"] == 10 * k
+        ref = O.kdk(m, pos, vel, G_SI, 10.0, 10 * k, eps=eps, form=form, dtype=np.float64)
+        assert np.array_equal(r["pos"], ref["pos"]) and np.array_equal(r["vel"], ref["vel"])
+        ke, pe = O.energy(m, ref["pos"], ref["vel"], G_SI, eps, form, dtype=np.float64)
+        assert r["ke"] == ke and r["pe"] == pe
+        assert np.array_equal(r["ids"], np.arange(256))
+    # the run's final state is unchanged by recording
+    plain = make(m, pos, vel, 10.0, 60, eps)
+    plain.evolve(capi.LEAPFROG_KDK)
+    assert np.array_equal(plain.state()["pos"], s.state()["pos"])
+
+
+def test_ring_and_advance():
+    print("case record ring", flush=True)
+    m, pos, vel = capi.ic_uniform_ball(32, 100, 1e20, 1e22, 1e9, 1e3)
+    s = make(m, pos, vel, 10.0, 20, 0.0)
+    s.set_recording(5, 3)
+    s.evolve(capi.LEAPFROG_KDK)  # records at 0, 5, 10, 15, 20
+    assert s.record_count() == (5, 3)
+    assert [s.record(k)["step"] for k in range(3)] == [10, 15, 20]
+    s.advance(10)  # 25, 30
+    assert s.record_count() == (7, 3)
+    assert [s.record(k)["step"] for k in range(3)] == [20, 25, 30]
+    ref = O.kdk(m, pos, vel, G_SI, 10.0, 30, dtype=np.float64)
+    assert np.array_equal(s.record(2)["pos"], ref["pos"])
+    with pytest.raises(capi.NBodyError):
+        s.record(3)
+
+
+def test_euler_and_collision_records():
+    print("case record euler/coll", flush=True)
+    m, pos, vel = capi.ic_uniform_ball(33, 64, 1e20, 1e22, 1e9, 1e3)
+    s = make(m, pos, vel, 10.0, 12, 0.0)
+    s.set_recording(4, 8)
+    s.evolve(capi.EULER)
+    for k in range(4):
+        ref = O.kdk(m, pos, vel, G_SI, 10.0, 4 * k, dtype=np.float64, method=O.METHOD_EULER)
+        assert np.array_equal(s.record(k)["pos"], ref["pos"])
+    # collisions with merges: the body count and labels of each record follow the oracle
+    rng = np.random.default_rng(1)
+    n = 24
+    mm = 10 ** rng.uniform(11, 13, n)
+    rr = (3 * mm / (4 * np.pi * 500)) ** (1 / 3)
+    pp = rng.uniform(-6e3, 6e3, (n, 3))
+    vv = rng.normal(0, 0.5, (n, 3))
+    ee = rng.uniform(0.1, 0.9, n)
+    c = capi.System(0)
+    c.set_G(G_SI)
+    c.set_timestep(5.0)
+    c.set_iterations(300)
+    c.set_collisions(capi.OVERLAP_COLL_CHECK, mergers=True, min_tot_com_mom=1e10)
+    c.set_bodies(mm, pp, vv, r=rr, rest_c=ee)
+    c.set_recording(100, 4)
+    c.evolve(capi.LEAPFROG_KDK)
+    assert c.record_count() == (4, 4)
+    for k in range(4):
+        rec = c.record(k)
+        ref = O.kdk(mm, pp, vv, G_SI, 5.0, 100 * k, coll=2, r=rr, e=ee, dtype=np.float64)
+        assert len(rec["ids"]) == ref["n"]
+        assert np.array_equal(rec["ids"], ref["labels"])
+        assert np.array_equal(rec["pos"], ref["pos"])
+
+
+def test_bh_records_bit_exact():
+    print("case record bh", flush=True)
+    m, pos, vel = capi.ic_two_plummer(11, 4096)
+    s = capi.System(0)
+    s.set_G(1.0)
+    s.set_softening(0.01)
+    s.set_bh_opening_angle(0.5)
+    s.set_timestep(1e-3)
+    s.set_iterations(6)
+    s.set_bodies(m, pos, vel)
+    s.set_recording(3, 4)
+    s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)
+    assert s.record_count() == (3, 3)
+    r0 = s.record(0)
+    assert np.array_equal(r0["pos"], pos) and np.array_equal(r0["vel"], vel)
+    for k in (1, 2):
+        rp, rv, _ = O.bh_kdk(m, pos, vel, 1.0, 1e-3, 0.01, 0.5, 3 * k)
+        r = s.record(k)
+        assert r["step"] == 3 * k
+        assert np.array_equal(r["pos"], rp) and np.array_equal(r["vel"], rv)
+
+
+def test_mixed_energy_records_config1():
+    """Config 1 (Plummer 65,536, eps 0.01, dt 1e-3): the mixed path's recorded
+    total energy stays within 1e-6 of |E0| of the FP64 path's at every record."""
+    n, steps = 65536, 1000
+    m, pos, vel = capi.ic_plummer(1, n)
+    rec = {}
+    for prec in (capi.PREC_MIXED, capi.PREC_FP64):
+        print(f"case record config1 prec={prec}", flush=True)
+        s = make(m, pos, vel, 1e-3, steps, 0.01, prec, G=1.0)
+        s.set_recording(250, 8)
+        s.evolve(capi.LEAPFROG_KDK)
+        assert s.record_count() == (5, 5)
+        rec[prec] = np.array([s.record(k)["ke"] + s.record(k)["pe"] for k in range(5)])
+    e0 = rec[capi.PREC_FP64][0]
+    assert rec[capi.PREC_MIXED][0] == e0  # same initial state
+    assert np.max(np.abs(rec[capi.PREC_MIXED] - rec[capi.PREC_FP64])) <= 1e-6 * abs(e0)
