@@ -135,6 +135,61 @@ def bench_bh(capi, steps):
                         "eps=0.01, dt=1e-3, FP32 wave walk, FP64 state and node COM"}
 
 
+def bench_bh_coll(capi, steps, radius):
+    """The BH step with bh_s_coll (touching-cell overlaps, bounces) on the same
+    1M-body two-Plummer workload with every radius `radius` (0.005: about 48k
+    bodies bounce in the first pass; 0.002: about 3k): the pass's own time per
+    step from HIP events, and the whole step."""
+    m, pos, vel = capi.ic_two_plummer(2, BH_BODIES)
+    r = np.full(BH_BODIES, radius)
+    e = np.full(BH_BODIES, 0.5)
+    s = capi.System(0)
+    s.set_G(1.0)
+    s.set_timestep(DT)
+    s.set_softening(EPS)
+    s.set_bh_opening_angle(BH_THETA)
+    s.set_precision(capi.PREC_MIXED)
+    s.set_collisions(capi.OVERLAP_COLL_CHECK, mergers=False, min_tot_com_mom=1e10)
+    s.set_bodies(m, pos, vel, r=r, rest_c=e)
+    s.set_iterations(1)
+    s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)  # warm-up
+    t0 = time.perf_counter()
+    s.advance(steps)
+    t = (time.perf_counter() - t0) / steps
+    s.profile(True)
+    s.advance(2)
+    c_ms, nc = s.profile_read("bh_coll")
+    s.profile(False)
+    s.close()
+    pass_ms = c_ms / max(nc, 1)
+    return {"bodies": BH_BODIES, "radius": radius, "ms_per_step": t * 1e3, "pass_ms": pass_ms,
+            "bodies_per_s": BH_BODIES / (pass_ms * 1e-3), "steps": steps,
+            "workload": f"BH KDK + bh_s_coll, two Plummer spheres of 524288, r={radius}, e=0.5, FP32 walk, FP64 pass"}
+
+
+def bench_integrators(capi, steps=20):
+    """ms per step of the other direct-sum integrators on the 65,536-body
+    Plummer workload, mixed precision (Euler, modified Euler and midpoint
+    evaluate forces once, twice and twice per step; DKD once)."""
+    m, pos, vel = capi.ic_plummer(1, N_BODIES)
+    out = {}
+    for name, meth in (("euler", capi.EULER), ("modified_euler", capi.MODIFIED_EULER),
+                       ("midpoint", capi.MIDPOINT), ("leapfrog_dkd", capi.LEAPFROG_DKD)):
+        s = capi.System(0)
+        s.set_G(1.0)
+        s.set_timestep(DT)
+        s.set_softening(EPS)
+        s.set_precision(capi.PREC_MIXED)
+        s.set_bodies(m, pos, vel)
+        s.set_iterations(1)
+        s.evolve(meth)  # warm-up
+        t0 = time.perf_counter()
+        s.advance(steps)
+        out[name] = (time.perf_counter() - t0) / steps * 1e3
+        s.close()
+    return {"unit": "ms_per_step", "bodies": N_BODIES, **out}
+
+
 def bench_adjoint(capi):
     """Adjoint workload: N = 4096 uniform ball (seed 4), eps = 1e6 m, dt = 10 s,
     2000 KDK steps, checkpoints every 64.  ratio = backward wall time (segment
@@ -375,6 +430,8 @@ def main():
         line["bh"] = bench_bh(capi, args.bh_steps)
         line["adjoint"] = bench_adjoint(capi)
         line["batch"] = bench_batch(capi, args.batch_steps)
+        line["bh_coll"] = [bench_bh_coll(capi, 3, 0.002), bench_bh_coll(capi, 3, 0.005)]
+        line["integrators"] = bench_integrators(capi)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(m, pos, vel)
         line["cpu_baseline_allcore"] = cpu_direct_allcore(m, pos)

@@ -99,6 +99,12 @@ struct BHWork {
     DBuf<uint32_t> bc_cnt, bc_blk, bc_total;  // overlaps per sorted body, then their exclusive scan
     DBuf<int32_t> e_own, e_nb;                // overlaps: owner body, neighbour body
     DBuf<double> e_geo, e_mom;                // distance and normal; axis momenta
+    DBuf<double> rmax;                        // largest radius
+    DBuf<int32_t> cc_par;                     // union-find parents over bodies
+    DBuf<uint64_t> cc_keys;                   // (component, entry) sort keys
+    DBuf<uint32_t> cc_vals;
+    DBuf<int32_t> g_own, g_nb;                // entries grouped by component
+    DBuf<double> g_geo;
     uint32_t *h_bc_total = nullptr;
     ~BHWork() {
         if (h_total) (void)hipHostFree(h_total);
@@ -451,21 +457,31 @@ __global__ void k_leaf_of(int32_t total, const int32_t *__restrict__ body, const
 // r_o + r_j > |p_j - p_o|, |.| the square root of the ordered sum of squares,
 // n = (p_j - p_o)/|.|.  FILL = false counts the overlaps of s; FILL = true
 // writes them, in visit order, from out_off[s].
+// A subtree whose box lies farther from p_o than r_o + r_max (with margins far
+// above the rounding of |.| and of the gaps) holds no body that can overlap o, so
+// it is skipped: only non-overlapping leaves drop out of the walk, and the
+// overlaps and their order are those of the full walk.
 template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, const uint32_t *__restrict__ idx,
                                                      const int32_t *__restrict__ leaf_of,
                                                      const double *__restrict__ nbox, const int32_t *__restrict__ skip,
                                                      const int32_t *__restrict__ body, const double *__restrict__ x,
-                                                     const double *__restrict__ r, uint32_t *__restrict__ cnt,
+                                                     const double *__restrict__ r, const double *__restrict__ rmax_p,
+                                                     uint32_t *__restrict__ cnt, uint32_t nent,
                                                      int32_t *__restrict__ e_own, int32_t *__restrict__ e_nb,
                                                      double *__restrict__ e_geo) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
+    if (FILL && (s + 1 < n ? cnt[s + 1] : nent) == cnt[s]) return;  // no overlaps: nothing to write
     const int32_t own = leaf_of[s];
     const int32_t o = (int32_t)idx[s];
     double bb[6];
     for (int k = 0; k < 6; ++k) bb[k] = nbox[(size_t)6 * own + k];
     const double xo = x[o], yo = x[n + o], zo = x[2 * n + o], ro = r[o];
+    // margins: relative to the radii, and relative to the coordinates (the
+    // rounding of the box gaps below)
+    const double reach = (ro + *rmax_p) * (1.0 + 1e-9) + 1e-9 * (fabs(xo) + fabs(yo) + fabs(zo));
+    const double reach2 = reach * reach;
     uint32_t k = FILL ? cnt[s] : 0u;
     int32_t p = 1;
     while (p < total) {
@@ -476,7 +492,10 @@ __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, c
         const double *cb = nbox + (size_t)6 * p;
         const bool touch = !(bb[3] < cb[0] || bb[0] > cb[3] || bb[4] < cb[1] || bb[1] > cb[4] || bb[5] < cb[2] ||
                              bb[2] > cb[5]);
-        if (!touch) {
+        const double gx = fmax(fmax(cb[0] - xo, xo - cb[3]), 0.0);
+        const double gy = fmax(fmax(cb[1] - yo, yo - cb[4]), 0.0);
+        const double gz = fmax(fmax(cb[2] - zo, zo - cb[5]), 0.0);
+        if (!touch || gx * gx + gy * gy + gz * gz > reach2) {
             p = skip[p];
             continue;
         }
@@ -504,33 +523,121 @@ __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, c
     if (!FILL) cnt[s] = k;
 }
 
-// The bounces in order: owners in leaf order (sorted order), each owner's
-// overlaps sorted by distance with an equal distance keeping the first found
-// (the ordered map's emplace), axis momenta m v . n taken from the velocities
-// as they stand when the owner is reached (its scan reads them all before any
-// of its bounces), then s_coll's bounce arithmetic (gregsys.hpp:660-684).
-// Serial by nature: one lane does it.
-__global__ void k_bhc_replay(int64_t n, uint32_t nent, const int32_t *__restrict__ e_own, int32_t *__restrict__ e_nb,
+// Largest radius (for the walk's reach); radii are not negative.
+__global__ __launch_bounds__(kBlock) void k_rmax(int64_t n, const double *__restrict__ r, double *__restrict__ out) {
+    __shared__ double sm[kBlock];
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) m = fmax(m, r[i]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sm[0];
+}
+
+// Connected components of the overlap graph (owner o -- neighbour j), by a
+// lock-free union-find over the bodies that have overlaps: roots are hooked
+// under smaller roots with a compare-and-swap, so parent indices only
+// decrease and every read, stale or not, names an ancestor.
+__device__ __forceinline__ int32_t cc_parent(int32_t *par, int32_t x) {
+    return __hip_atomic_load(par + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ int32_t cc_find(int32_t *par, int32_t x) {
+    while (true) {
+        const int32_t p = cc_parent(par, x);
+        if (p == x) return x;
+        const int32_t g = cc_parent(par, p);
+        if (g != p) __hip_atomic_store(par + x, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // halve the path
+        x = p;
+    }
+}
+
+__global__ void k_cc_init(uint32_t nent, const int32_t *__restrict__ e_own, const int32_t *__restrict__ e_nb,
+                          int32_t *__restrict__ par) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nent) return;
+    par[e_own[k]] = e_own[k];
+    par[e_nb[k]] = e_nb[k];
+}
+
+__global__ void k_cc_union(uint32_t nent, const int32_t *__restrict__ e_own, const int32_t *__restrict__ e_nb,
+                           int32_t *par) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nent) return;
+    int32_t a = cc_find(par, e_own[k]), b = cc_find(par, e_nb[k]);
+    while (a != b) {
+        if (a < b) {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        int32_t expected = a;
+        if (__hip_atomic_compare_exchange_strong(par + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            break;  // root a now hangs under b
+        a = cc_find(par, expected);
+        b = cc_find(par, b);
+    }
+}
+
+// key = (component root << 32) | entry index: a stable order that keeps each
+// component's entries together in owner (leaf) order and visit order.
+__global__ void k_cc_keys(uint32_t nent, const int32_t *__restrict__ e_own, int32_t *par, uint64_t *__restrict__ keys,
+                          uint32_t *__restrict__ vals) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nent) return;
+    keys[k] = ((uint64_t)(uint32_t)cc_find(par, e_own[k]) << 32) | k;
+    vals[k] = k;
+}
+
+__global__ void k_cc_gather(uint32_t nent, const uint32_t *__restrict__ vals, const int32_t *__restrict__ e_own,
+                            const int32_t *__restrict__ e_nb, const double *__restrict__ e_geo,
+                            int32_t *__restrict__ g_own, int32_t *__restrict__ g_nb, double *__restrict__ g_geo) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= nent) return;
+    const uint32_t k = vals[q];
+    g_own[q] = e_own[k];
+    g_nb[q] = e_nb[k];
+    for (int c = 0; c < 4; ++c) g_geo[4 * (size_t)q + c] = e_geo[4 * (size_t)k + c];
+}
+
+// The bounces in order, one lane per component of the overlap graph: owners
+// in leaf order (sorted order), each owner's overlaps sorted by distance with
+// an equal distance keeping the first found (the ordered map's emplace), axis
+// momenta m v . n taken from the velocities as they stand when the owner is
+// reached (its scan reads them all before any of its bounces), then s_coll's
+// bounce arithmetic (gregsys.hpp:660-684).  Owners of different components
+// touch disjoint bodies, so running components side by side gives the serial
+// result; within a component the order is the serial one.
+__global__ void k_bhc_replay(int64_t n, uint32_t nent, const uint64_t *__restrict__ keys,
+                             const int32_t *__restrict__ e_own, int32_t *__restrict__ e_nb,
                              double *__restrict__ e_geo, double *__restrict__ e_mom, const double *__restrict__ m,
                              const double *__restrict__ rc, double *__restrict__ v) {
-    if (threadIdx.x != 0) return;
-    uint32_t k = 0;
-    while (k < nent) {
+    const uint32_t q0 = blockIdx.x * kBlock + threadIdx.x;
+    if (q0 >= nent) return;
+    const uint32_t comp = (uint32_t)(keys[q0] >> 32);
+    if (q0 > 0 && (uint32_t)(keys[q0 - 1] >> 32) == comp) return;  // not the first entry of its component
+    uint32_t cend = q0 + 1;
+    while (cend < nent && (uint32_t)(keys[cend] >> 32) == comp) ++cend;
+    uint32_t k = q0;
+    while (k < cend) {
         const int32_t o = e_own[k];
         uint32_t ke = k + 1;
-        while (ke < nent && e_own[ke] == o) ++ke;
+        while (ke < cend && e_own[ke] == o) ++ke;
         for (uint32_t a = k + 1; a < ke; ++a) {  // stable insertion sort by distance
             const int32_t jn = e_nb[a];
             double g[4];
-            for (int q = 0; q < 4; ++q) g[q] = e_geo[4 * (size_t)a + q];
+            for (int c = 0; c < 4; ++c) g[c] = e_geo[4 * (size_t)a + c];
             uint32_t b = a;
             while (b > k && e_geo[4 * (size_t)(b - 1)] > g[0]) {
                 e_nb[b] = e_nb[b - 1];
-                for (int q = 0; q < 4; ++q) e_geo[4 * (size_t)b + q] = e_geo[4 * (size_t)(b - 1) + q];
+                for (int c = 0; c < 4; ++c) e_geo[4 * (size_t)b + c] = e_geo[4 * (size_t)(b - 1) + c];
                 --b;
             }
             e_nb[b] = jn;
-            for (int q = 0; q < 4; ++q) e_geo[4 * (size_t)b + q] = g[q];
+            for (int c = 0; c < 4; ++c) e_geo[4 * (size_t)b + c] = g[c];
         }
         const double mo = m[o], eo = rc[o];
         double vo[3] = {v[o], v[n + o], v[2 * n + o]};
@@ -959,12 +1066,15 @@ static void bh_collision_pass(nb_system *s) {
     w.bc_cnt.ensure((size_t)w.cap_n);
     w.bc_blk.ensure((size_t)(w.cap_n + 1023) / 1024 + 1);
     w.bc_total.ensure(1);
+    w.rmax.ensure(1);
     if (!w.h_bc_total) NB_HIP(hipHostMalloc((void **)&w.h_bc_total, sizeof(uint32_t), hipHostMallocDefault));
     hipLaunchKernelGGL(k_node_box, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.box.p, w.first.p, w.depth.p,
                        w.khi.p, w.klo.p, w.nbox.p);
     hipLaunchKernelGGL(k_leaf_of, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.body.p, w.first.p, w.leaf_of.p);
+    hipLaunchKernelGGL(k_rmax, dim3(1), dim3(kBlock), 0, st, n, s->r.p, w.rmax.p);
     hipLaunchKernelGGL(k_bhc_scan<false>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.leaf_of.p,
-                       w.nbox.p, w.skip.p, w.body.p, s->x.p, s->r.p, w.bc_cnt.p, nullptr, nullptr, nullptr);
+                       w.nbox.p, w.skip.p, w.body.p, s->x.p, s->r.p, w.rmax.p, w.bc_cnt.p, 0u, nullptr, nullptr,
+                       nullptr);
     const int nb = (int)((n + 1023) / 1024);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.bc_cnt.p, w.bc_blk.p);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nb, w.bc_blk.p, w.bc_total.p);
@@ -978,9 +1088,27 @@ static void bh_collision_pass(nb_system *s) {
         w.e_own.ensure(cap); w.e_nb.ensure(cap); w.e_geo.ensure(4 * cap); w.e_mom.ensure(2 * cap);
     }
     hipLaunchKernelGGL(k_bhc_scan<true>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.leaf_of.p,
-                       w.nbox.p, w.skip.p, w.body.p, s->x.p, s->r.p, w.bc_cnt.p, w.e_own.p, w.e_nb.p, w.e_geo.p);
-    hipLaunchKernelGGL(k_bhc_replay, dim3(1), dim3(64), 0, st, n, nent, w.e_own.p, w.e_nb.p, w.e_geo.p, w.e_mom.p,
-                       s->m.p, s->e.p, s->v.p);
+                       w.nbox.p, w.skip.p, w.body.p, s->x.p, s->r.p, w.rmax.p, w.bc_cnt.p, nent, w.e_own.p,
+                       w.e_nb.p, w.e_geo.p);
+    // components of the overlap graph, entries grouped by component in
+    // (owner, visit) order, then one lane per component
+    w.cc_par.ensure((size_t)w.cap_n);
+    if (w.cc_keys.n < nent) {
+        const size_t cap = (size_t)nent + nent / 4 + 64;
+        w.cc_keys.ensure(cap); w.cc_vals.ensure(cap);
+        w.g_own.ensure(cap); w.g_nb.ensure(cap); w.g_geo.ensure(4 * cap);
+        w.sorter.ensure((int64_t)cap);
+    }
+    const unsigned eb = blocks((int64_t)nent);
+    hipLaunchKernelGGL(k_cc_init, dim3(eb), dim3(kBlock), 0, st, nent, w.e_own.p, w.e_nb.p, w.cc_par.p);
+    hipLaunchKernelGGL(k_cc_union, dim3(eb), dim3(kBlock), 0, st, nent, w.e_own.p, w.e_nb.p, w.cc_par.p);
+    hipLaunchKernelGGL(k_cc_keys, dim3(eb), dim3(kBlock), 0, st, nent, w.e_own.p, w.cc_par.p, w.cc_keys.p,
+                       w.cc_vals.p);
+    w.sorter.sort(st, w.cc_keys.p, w.cc_vals.p, (int64_t)nent, 63);
+    hipLaunchKernelGGL(k_cc_gather, dim3(eb), dim3(kBlock), 0, st, nent, w.cc_vals.p, w.e_own.p, w.e_nb.p, w.e_geo.p,
+                       w.g_own.p, w.g_nb.p, w.g_geo.p);
+    hipLaunchKernelGGL(k_bhc_replay, dim3(eb), dim3(kBlock), 0, st, n, nent, w.cc_keys.p, w.g_own.p, w.g_nb.p,
+                       w.g_geo.p, w.e_mom.p, s->m.p, s->e.p, s->v.p);
     NB_HIP(hipGetLastError());
 }
 
