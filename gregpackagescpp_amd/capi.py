@@ -82,6 +82,7 @@ SIGNATURES = {
     "nb_energy": (C.c_int, [_P, _D, _D]),
     "nb_merge_log": (C.c_int, [_P, C.c_int64, _I64, _I64]),
     "nb_set_recording": (C.c_int, [_P, C.c_uint64, C.c_uint64]),
+    "nb_track": (C.c_int, [_P, _I64, C.c_int64, _D, _D, _D, _D, _D, _D]),
     "nb_record_count": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "nb_get_record": (C.c_int, [_P, C.c_uint64, _I64, _I64, _D, _D, _I64, _D, _D]),
     "nb_tree_topology": (C.c_int, [_P, C.c_int64, _I32, _I32, _I64, _I64, _D, _D, _D, _I64]),
@@ -262,6 +263,16 @@ class System:
         ke, pe = C.c_double(), C.c_double()
         self._ck(self._L.nb_energy(self.h, C.byref(ke), C.byref(pe)))
         return ke.value, pe.value
+
+    def track(self, idx, point=None):
+        """body_tracker over slots idx: dict(mass, com, com_vel, mean_dist, mean_sep)."""
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        mass, md, ms = C.c_double(), C.c_double(), C.c_double()
+        com, cv = np.zeros(3), np.zeros(3)
+        pt = None if point is None else np.ascontiguousarray(point, dtype=np.float64)
+        self._ck(self._L.nb_track(self.h, ip(idx), len(idx), None if pt is None else dp(pt), C.byref(mass), dp(com),
+                                  dp(cv), C.byref(md), C.byref(ms)))
+        return dict(mass=mass.value, com=com, com_vel=cv, mean_dist=md.value, mean_sep=ms.value)
 
     def set_recording(self, every, capacity):
         self._ck(self._L.nb_set_recording(self.h, every, capacity))

@@ -709,6 +709,26 @@ int nb_get_record(nb_system *s, uint64_t k, int64_t *step, int64_t *n, double *p
     });
 }
 
+int nb_track(nb_system *s, const int64_t *idx, int64_t k, const double *point, double *mass, double *com,
+             double *com_vel, double *mean_dist, double *mean_sep) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    if (k < 0 || (k > 0 && !idx)) return fail(s, NB_ERR_INVALID_ARGUMENT, "nb_track: bad body list");
+    return guarded(s, [&]() {
+        if (k == 0 && mean_sep)  // mean_sep() on an empty tracker throws nbody_error (:3564-3565)
+            throw ApiError(NB_ERR_INVALID_ARGUMENT, "Error: no bodies being tracked.\n");
+        double out[9];
+        track_bodies(s, idx, k, point, out);
+        if (mass) *mass = out[0];
+        for (int c = 0; c < 3; ++c) {
+            if (com) com[c] = out[1 + c];
+            if (com_vel) com_vel[c] = out[4 + c];
+        }
+        if (mean_dist) *mean_dist = out[7];
+        if (mean_sep) *mean_sep = out[8];
+        return NB_OK;
+    });
+}
+
 int nb_merge_log(const nb_system *cs, int64_t cap, int64_t *records, int64_t *count) {
     nb_system *s = const_cast<nb_system *>(cs);
     if (!s || !count) return fail(s, NB_ERR_INVALID_ARGUMENT, "NULL argument");

@@ -76,6 +76,9 @@ void set_static_error(const std::string &msg);  // message for nb_last_error(NUL
 // is a multiple of the recording interval
 void record_state(nb_system *s, uint64_t step);
 inline bool record_due(const nb_system *s, uint64_t step) { return s->rec_every && step % s->rec_every == 0; }
+// body_tracker reductions over bodies idx[0..k) (nb_track.hip): out9 = mass,
+// com[3], com_vel[3], mean distance to point (0 when point is NULL), mean separation
+void track_bodies(nb_system *s, const int64_t *idx, int64_t k, const double *point, double *out9);
 // collisions after a step's closing kick (nb_coll.hip)
 void collision_pass(nb_system *s, uint64_t step);
 // Barnes-Hut (nb_bh.hip)

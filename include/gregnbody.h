@@ -149,6 +149,16 @@ int nb_record_count(const nb_system *sys, uint64_t *taken, uint64_t *stored);
  * NULL to learn n first. */
 int nb_get_record(nb_system *sys, uint64_t k, int64_t *step, int64_t *n, double *pos, double *vel, int64_t *ids,
                   double *ke, double *pe);
+/* body_tracker (gregsys.hpp:3492-3611) over the bodies at slots idx[0..k), in
+ * that order (the tracker's predicate is the caller's: it picks the slots).
+ * Outputs (each may be NULL): total mass (mass()), centre of mass com[3]
+ * (com()), COM velocity com_vel[3] (com_vel()), mean distance to point[3]
+ * (mean_dist_to(point); needs point), mean pairwise separation (mean_sep()).
+ * All FP64: the O(k) sums in the subset's order, the separation summed per
+ * outer body and then over the rows in order.  k = 0 with mean_sep requested
+ * returns NB_ERR_INVALID_ARGUMENT, as mean_sep() throws on an empty tracker. */
+int nb_track(nb_system *sys, const int64_t *idx, int64_t k, const double *point, double *mass, double *com,
+             double *com_vel, double *mean_dist, double *mean_sep);
 /* Merge records since nb_set_bodies: 4 int64 per record
  * (step, outer body label, absorbed body label, merged body label). */
 int nb_merge_log(const nb_system *sys, int64_t cap, int64_t *records, int64_t *count);

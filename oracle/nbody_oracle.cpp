@@ -715,6 +715,54 @@ int bh_s_coll_arrays(int64_t n, const T *m, const T *r, const T *e, const T *pos
     return 0;
 }
 
+/* body_tracker (gregsys.hpp:3492-3611) over the bodies idx[0..k): total mass
+ * (:3527-3531), centre of mass and COM velocity (:3533-3554: the mass sum,
+ * then the ordered sum of m*r, divided by the mass unless it is zero),
+ * mean distance to a point (:3555-3562) and mean pairwise separation
+ * (:3563-3582: distance of every pair outer < inner, divided by the integer
+ * k(k-1)/2).  The long-double form keeps the reference's single running sum
+ * for the separation; the double form (ROWS) sums each outer body's row first
+ * and then the rows in order, the association the GPU uses. */
+template <typename T, bool ROWS>
+void track(int64_t k, const int64_t *idx, const T *m, const T *pos, const T *vel, const T *point, T *out) {
+    T tm = 0;
+    for (int64_t q = 0; q < k; ++q) tm += m[idx[q]];
+    T mr[3] = {0, 0, 0}, mv[3] = {0, 0, 0};
+    for (int64_t q = 0; q < k; ++q) {
+        const int64_t i = idx[q];
+        for (int c = 0; c < 3; ++c) mr[c] += m[i] * pos[3 * i + c];
+    }
+    for (int64_t q = 0; q < k; ++q) {
+        const int64_t i = idx[q];
+        for (int c = 0; c < 3; ++c) mv[c] += m[i] * vel[3 * i + c];
+    }
+    out[0] = tm;
+    for (int c = 0; c < 3; ++c) {
+        out[1 + c] = (k == 0 || tm == 0) ? T(0) : mr[c] / tm;
+        out[4 + c] = (k == 0 || tm == 0) ? T(0) : mv[c] / tm;
+    }
+    T sd = 0;
+    for (int64_t q = 0; q < k; ++q) {
+        const int64_t i = idx[q];
+        T d[3] = {pos[3 * i] - point[0], pos[3 * i + 1] - point[1], pos[3 * i + 2] - point[2]};
+        sd += t_sqrt<T>(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    }
+    out[7] = k ? sd / (T)(uint64_t)k : T(0);
+    T tot = 0;
+    for (int64_t a = 0; a < k; ++a) {
+        T row = 0;
+        for (int64_t b = a + 1; b < k; ++b) {
+            const int64_t o = idx[a], i = idx[b];
+            T d[3] = {pos[3 * o] - pos[3 * i], pos[3 * o + 1] - pos[3 * i + 1], pos[3 * o + 2] - pos[3 * i + 2]};
+            T dist = t_sqrt<T>(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (ROWS) row += dist;
+            else tot += dist;
+        }
+        if (ROWS) tot += row;
+    }
+    out[8] = k > 1 ? tot / (T)(uint64_t)(((uint64_t)k * (uint64_t)(k - 1)) / 2) : T(0);
+}
+
 } // namespace
 
 extern "C" {
@@ -990,6 +1038,19 @@ int orc_bh_s_coll_ld(int64_t n, const long double *m, const long double *r, cons
 }
 int orc_bh_s_coll_d(int64_t n, const double *m, const double *r, const double *e, const double *pos, double *vel) {
     return bh_s_coll_arrays<double>(n, m, r, e, pos, vel);
+}
+
+/* body_tracker reductions: out = mass, com[3], com_vel[3], mean distance to
+ * `point`, mean separation. */
+int orc_track_ld(int64_t k, const int64_t *idx, const long double *m, const long double *pos, const long double *vel,
+                 const long double *point, long double *out) {
+    track<long double, false>(k, idx, m, pos, vel, point, out);
+    return 0;
+}
+int orc_track_d(int64_t k, const int64_t *idx, const double *m, const double *pos, const double *vel,
+                const double *point, double *out) {
+    track<double, true>(k, idx, m, pos, vel, point, out);
+    return 0;
 }
 
 /* Neighbour bodies of body b as nn_iterator yields them (for tests). */

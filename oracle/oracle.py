@@ -69,6 +69,8 @@ def lib():
         L.orc_bh_neighbours_d.argtypes = [C.c_int64, _D, _D, C.c_int64, C.c_int64, _I64]
         L.orc_bh_kdk_coll_d.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
                                         C.c_int, _D, _D, _D, _D, _D, _D]
+        L.orc_track_ld.argtypes = [C.c_int64, _I64, _LDP, _LDP, _LDP, _LDP, _LDP]
+        L.orc_track_d.argtypes = [C.c_int64, _I64, _D, _D, _D, _D, _D]
         L.orc_trace.argtypes = [C.c_int]
         L.orc_trace_len.restype = C.c_int64
         L.orc_trace_copy.argtypes = [_I64, _LDP]
@@ -283,6 +285,17 @@ def bh_s_coll(m, r, e, pos, vel, dtype=LD):
     return vel
 
 
+def track(m, pos, vel, idx, point, dtype=LD):
+    """body_tracker over bodies idx: dict(mass, com, com_vel, mean_dist, mean_sep)."""
+    conv, ptr = (_ld, _lp) if dtype == LD else (_d, _dp)
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    m, pos, vel, point = conv(m), conv(pos), conv(vel), conv(point)
+    out = np.zeros(9, dtype=m.dtype)
+    f = lib().orc_track_ld if dtype == LD else lib().orc_track_d
+    f(len(idx), _ip(idx), ptr(m), ptr(pos), ptr(vel), ptr(point), ptr(out))
+    return dict(mass=out[0], com=out[1:4], com_vel=out[4:7], mean_dist=out[7], mean_sep=out[8])
+
+
 def bh_neighbours(m, pos, body):
     """The bodies nn_iterator yields for `body`, in its order."""
     n = len(m)
@@ -343,6 +356,20 @@ def ref_bhcoll(m, r, e, pos, vel, tmpdir):
     write_ref_input(inp, m, pos, vel, r=r, e=e, steps=1, dt=1.0)
     subprocess.run([os.path.join(REF_DIR, "ref_bhcoll"), inp, out], check=True, timeout=600)
     return np.fromfile(out, dtype=LD).reshape(len(m), 3)
+
+
+TRACK_POINT = (1.5e8, -2.5e8, 0.5e8)  # the point ref_harness's track mode measures distances to
+
+
+def ref_track(m, pos, vel, tmpdir):
+    """The compiled reference's body_tracker over the bodies with x > 0:
+    (count, [mass, com xyz, com_vel xyz, mean distance to TRACK_POINT, mean separation]) in long double."""
+    inp, out = os.path.join(tmpdir, "trk_in.bin"), os.path.join(tmpdir, "trk_out.bin")
+    write_ref_input(inp, m, pos, vel, steps=1, dt=1.0)
+    run_ref("track", inp, out)
+    raw = open(out, "rb").read()
+    cnt = int(np.frombuffer(raw[:8], np.int64)[0])
+    return cnt, np.frombuffer(raw[8:], LD).copy()
 
 
 def run_ref(mode, inp, out, merge=False, timeout=3600):

@@ -16,6 +16,8 @@
  *               system<ld,ld,ld,false,false,3,0,0,false> (built with and without
  *               -DGREGSYS_MERGERS); evolve(leapfrog_kdk, nullptr, per_body,
  *               per_step) dumps every body after each step's s_coll.
+ *   mode track : body_tracker<ld,ld,ld,0> (gregsys.hpp:3492-3611) over the bodies
+ *               with x > 0: mass, com, com_vel, mean_dist_to(point), mean_sep.
  *   mode tree : bh_tree<double,double,double,0>::add_bodies (greg8tree.hpp:1066)
  *               and its operator<< dump (:975-1001) parsed to DFS pre-order
  *               records (depth, octant, body index, mass).
@@ -104,6 +106,31 @@ static int run_coll(const In &in, FILE *out, int method) {
         fwrite(v, sizeof(ld), 8, out);
     };
     sys.evolve(method, nullptr, per_body, per_step);
+    return 0;
+}
+
+/* body_tracker (gregsys.hpp:3492-3611) over the bodies with x > 0: count, then
+ * mass(), com(), com_vel(), mean_dist_to(1.5e8, -2.5e8, 0.5e8), mean_sep() in
+ * long double. */
+static int run_track(const In &in, FILE *out) {
+    using bod_t = gtd::body<ld, ld, ld, 0>;
+    std::vector<bod_t> bods;
+    bods.reserve((size_t)in.n);
+    for (int64_t i = 0; i < in.n; ++i) {
+        const double *r = &in.rec[(size_t)i * 9];
+        bods.emplace_back((ld)r[0], (ld)r[1], gtd::vector3D<ld>{(ld)r[3], (ld)r[4], (ld)r[5]},
+                          gtd::vector3D<ld>{(ld)r[6], (ld)r[7], (ld)r[8]}, (ld)r[2]);
+    }
+    gtd::system<ld, ld, ld, false, false, 0, 0, 0, false> sys{bods, (ld)in.dt, (uint64_t)1, "M1:1,D1:1,T1:1", true};
+    gtd::body_tracker<ld, ld, ld, 0> tr{sys, [](const bod_t &b) { return b.pos().get_x() > 0; }};
+    int64_t cnt = (int64_t)tr.num_bods();
+    fwrite(&cnt, 8, 1, out);
+    auto com = tr.com();
+    auto cv = tr.com_vel();
+    gtd::vector3D<ld> point{(ld)1.5e8, (ld)-2.5e8, (ld)0.5e8};
+    ld v[9] = {tr.mass(), com.get_x(), com.get_y(), com.get_z(), cv.get_x(), cv.get_y(), cv.get_z(),
+               tr.mean_dist_to(point), tr.mean_sep()};
+    fwrite(v, sizeof(ld), 9, out);
     return 0;
 }
 
@@ -252,6 +279,7 @@ int main(int argc, char **argv) {
                 rc = run_coll(in, out, mm.second);
         }
         if (mode == "tree") rc = run_tree(in, out);
+        if (mode == "track") rc = run_track(in, out);
     } catch (const std::exception &e) {
         fprintf(stderr, "reference threw: %s\n", e.what());
         rc = 4;

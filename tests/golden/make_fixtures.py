@@ -16,13 +16,16 @@ generators; outputs are the reference's own results, stored as .npz:
   ref_icoll_{method}_{bounce,merge}_seed1.npz  the same integrators with
                          collisions = 3, rubble cluster: every body after
                          every 25th step, and the final body count.
+  ref_track_n{N}_seed{K}.npz  body_tracker over the bodies with x > 0 (uniform
+                         ball): count, mass, com, com_vel, mean distance to
+                         oracle.TRACK_POINT, mean separation (long double).
   ref_bhcoll_n{N}_seed{K}.npz  one bh_s_coll pass (oracle/ref_bhcoll, the
                          reference's own private method on its own tree) on
                          a dense rubble cluster: inputs and the velocities
                          after the pass (x87 long double).
 
-Usage: python tests/golden/make_fixtures.py [integ|bhcoll]   (only the
-       integrator / only the bh_s_coll fixtures)
+Usage: python tests/golden/make_fixtures.py [integ|bhcoll|track]   (only the
+       integrator / bh_s_coll / body_tracker fixtures)
 """
 import os
 import sys
@@ -94,6 +97,18 @@ def bhcoll_fixtures(tmp):
         print("bhcoll", n, seed, int((v != vel.astype(O.LD)).any(axis=1).sum()), "bodies bounced")
 
 
+TRACK_CASES = [(1024, 0), (3000, 6)]
+
+
+def track_fixtures(tmp):
+    for n, seed in TRACK_CASES:
+        m, pos, vel = capi.ic_uniform_ball(seed, n, 1e20, 1e22, 1e9, 1e3)
+        cnt, vals = O.ref_track(m, pos, vel, tmp)
+        np.savez_compressed(os.path.join(OUT, f"ref_track_n{n}_seed{seed}.npz"), m=m, pos=pos, vel=vel, count=cnt,
+                            vals=vals, point=np.array(O.TRACK_POINT))
+        print("track", n, seed, cnt)
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref is not built (needs /root/reference): nothing generated")
@@ -105,7 +120,11 @@ def main():
     if sys.argv[1:] == ["bhcoll"]:
         bhcoll_fixtures(tmp)
         return
+    if sys.argv[1:] == ["track"]:
+        track_fixtures(tmp)
+        return
     bhcoll_fixtures(tmp)
+    track_fixtures(tmp)
     integ_fixtures(inp, out)
     for n, steps in KDK_CASES:
         m, pos, vel = capi.ic_uniform_ball(0, n, 1e20, 1e22, 1e9, 1e3)
