@@ -48,6 +48,14 @@ int main() {
     bh.set_precision(NB_PREC_MIXED);
     auto bns = bh.evolve(sys_t::leapfrog_kdk | sys_t::barnes_hut);
     std::printf("Barnes-Hut: %lld bodies, 10 steps in %.3f ms\n", (long long)bh.num_bodies(), bns.count() / 1e6);
+    // body_tracker over the inner bodies (the reference's COM-of-a-subset helper)
+    gtd::body_tracker<> core{bh, [](const gtd::body &b) {
+        return b.pos().get_x() * b.pos().get_x() + b.pos().get_y() * b.pos().get_y() < 0.25;
+    }};
+    const auto c = core.com();
+    std::printf("tracker: %llu bodies, mass %.4f, com (%.2e, %.2e, %.2e), mean separation %.4f\n",
+                (unsigned long long)core.num_bods(), core.mass(), c.x, c.y, c.z, core.mean_sep());
+    if (!(core.num_bods() > 0 && core.mass() > 0 && core.mean_sep() > 0)) return 5;
     try {
         sys.set_timestep(-1);
         if (sys.set_timestep(-1)) return 3;  // setters answer false, as the reference's do

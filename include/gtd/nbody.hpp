@@ -12,9 +12,12 @@
  * num_bodies(), G_val(), operator[], begin()/end()), the method bit flags
  * (gregsys.hpp:156-165) and the exception types (gregbod.hpp:19-93,
  * greg8tree.hpp:11-46), re-declared here.
- * Different by design: state is FP64 (M = R = T = double); memFreq/fileFreq
- * trajectory recording is not on this path; evolve's per-body callbacks are
- * not supported (a continue_ev_if predicate is: it is polled once per step).
+ * Also kept: body_tracker(system, predicate) with mass/com/com_vel/
+ * mean_dist_to/mean_sep.
+ * Different by design: state is FP64 (M = R = T = double); the memFreq/
+ * fileFreq template arguments stay 0 (recording is nb_set_recording on the
+ * C ABI); evolve's per-body callbacks are not supported (a continue_ev_if
+ * predicate is: it is polled once per step).
  */
 #ifndef GTD_NBODY_HPP
 #define GTD_NBODY_HPP
@@ -105,10 +108,15 @@ public:
     uint64_t get_id() const noexcept { return id_; }
 };
 
+template <typename M, typename R, typename T, uint64_t rF>
+class body_tracker;
+
 template <typename M = double, typename R = double, typename T = double, bool prog = false,
           bool mergeOverlappingBodies = false, int collisions = 0, uint64_t memFreq = 0, uint64_t fileFreq = 0,
           bool binaryFile = false>
 class system {
+    template <typename, typename, typename, uint64_t>
+    friend class body_tracker;
     static_assert(std::is_same_v<M, double> && std::is_same_v<R, double> && std::is_same_v<T, double>,
                   "the MI355X engine keeps masses, radii and positions in FP64");
     static_assert(memFreq == 0 && fileFreq == 0, "trajectory recording is not on the engine's path");
@@ -317,6 +325,60 @@ public:
     auto end() const {
         refresh();
         return cache_.cend();
+    }
+};
+
+/* body_tracker (gregsys.hpp:3492-3611): the bodies of a system that a
+ * predicate picks at construction, with the tracker's reductions (mass, com,
+ * com_vel, mean_dist_to, mean_sep) computed on the GPU by nb_track.  As in
+ * the reference, it assumes that no merger happens and no body is added to the
+ * system after construction. */
+template <typename M = double, typename R = double, typename T = double, uint64_t rF = 0>
+class body_tracker {
+    nb_system *h_ = nullptr;
+    std::vector<int64_t> idx_;
+    void check(int rc) const { detail::raise(rc, nb_last_error(h_)); }
+    const int64_t *ids() const noexcept { return idx_.empty() ? nullptr : idx_.data(); }
+
+public:
+    template <bool prog, bool merge, int coll, uint64_t fF, bool bin, typename F>
+    body_tracker(const system<M, R, T, prog, merge, coll, rF, fF, bin> &sys, const F &pred,
+                 typename std::vector<int64_t>::size_type reserve_bods = 0)
+        : h_{sys.h_} {
+        if (reserve_bods) idx_.reserve(reserve_bods);
+        int64_t k = 0;
+        for (const body &b : sys) {
+            if (pred(b)) idx_.push_back(k);
+            ++k;
+        }
+    }
+    uint64_t num_bods() const noexcept { return idx_.size(); }
+    M mass() const {
+        double m = 0;
+        check(nb_track(h_, ids(), (int64_t)idx_.size(), nullptr, &m, nullptr, nullptr, nullptr, nullptr));
+        return m;
+    }
+    vector3D<T> com() const {
+        double c[3] = {0, 0, 0};
+        check(nb_track(h_, ids(), (int64_t)idx_.size(), nullptr, nullptr, c, nullptr, nullptr, nullptr));
+        return {c[0], c[1], c[2]};
+    }
+    vector3D<T> com_vel() const {
+        double c[3] = {0, 0, 0};
+        check(nb_track(h_, ids(), (int64_t)idx_.size(), nullptr, nullptr, nullptr, c, nullptr, nullptr));
+        return {c[0], c[1], c[2]};
+    }
+    T mean_dist_to(const vector3D<T> &point) const {
+        const double p[3] = {point.x, point.y, point.z};
+        double d = 0;
+        check(nb_track(h_, ids(), (int64_t)idx_.size(), p, nullptr, nullptr, nullptr, &d, nullptr));
+        return d;
+    }
+    T mean_sep() const {
+        if (idx_.empty()) throw nbody_error{"Error: no bodies being tracked.\n"};
+        double d = 0;
+        check(nb_track(h_, ids(), (int64_t)idx_.size(), nullptr, nullptr, nullptr, nullptr, nullptr, &d));
+        return d;
     }
 };
 
