@@ -81,7 +81,7 @@ SIGNATURES = {
     "nb_advance": (C.c_int, [_P, C.c_uint64, _I64]),
     "nb_energy": (C.c_int, [_P, _D, _D]),
     "nb_merge_log": (C.c_int, [_P, C.c_int64, _I64, _I64]),
-    "nb_set_recording": (C.c_int, [_P, C.c_uint64, C.c_uint64]),
+    "nb_set_recording": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_int]),
     "nb_track": (C.c_int, [_P, _I64, C.c_int64, _D, _D, _D, _D, _D, _D]),
     "nb_record_count": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "nb_get_record": (C.c_int, [_P, C.c_uint64, _I64, _I64, _D, _D, _I64, _D, _D]),
@@ -274,8 +274,9 @@ class System:
                                   dp(cv), C.byref(md), C.byref(ms)))
         return dict(mass=mass.value, com=com, com_vel=cv, mean_dist=md.value, mean_sep=ms.value)
 
-    def set_recording(self, every, capacity):
-        self._ck(self._L.nb_set_recording(self.h, every, capacity))
+    def set_recording(self, every, capacity, what=3):
+        """what: 1 state, 2 energies, 3 both (NB_RECORD_STATE | NB_RECORD_ENERGY)."""
+        self._ck(self._L.nb_set_recording(self.h, every, capacity, what))
 
     def record_count(self):
         taken, stored = C.c_uint64(), C.c_uint64()

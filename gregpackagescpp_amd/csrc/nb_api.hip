@@ -652,10 +652,13 @@ int nb_energy(nb_system *s, double *ke, double *pe) {
     });
 }
 
-int nb_set_recording(nb_system *s, uint64_t every, uint64_t capacity) {
+int nb_set_recording(nb_system *s, uint64_t every, uint64_t capacity, int what) {
     if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
     if (every && !capacity) return fail(s, NB_ERR_INVALID_ARGUMENT, "nb_set_recording: capacity must be positive");
+    if (every && (what & ~(NB_RECORD_STATE | NB_RECORD_ENERGY) || !what))
+        return fail(s, NB_ERR_INVALID_ARGUMENT, "nb_set_recording: what is NB_RECORD_STATE and/or NB_RECORD_ENERGY");
     return guarded(s, [&]() {
+        s->rec_what = every ? what : 0;
         s->rec_every = every;
         s->rec_cap = every ? capacity : 0;
         s->rec_total = 0;
@@ -698,6 +701,10 @@ int nb_get_record(nb_system *s, uint64_t k, int64_t *step, int64_t *n, double *p
         std::vector<double> buf((size_t)3 * (size_t)nb);
         auto get = [&](const DBuf<double> &src, double *dst) {
             if (!dst || nb == 0) return;
+            if (!(s->rec_what & NB_RECORD_STATE) || !src.p) {  // an energy-only ring keeps no state
+                std::fill(dst, dst + 3 * nb, 0.0);
+                return;
+            }
             NB_HIP(hipMemcpy(buf.data(), src.p + slot * stride, buf.size() * 8, hipMemcpyDeviceToHost));
             for (int64_t i = 0; i < nb; ++i)
                 for (int c = 0; c < 3; ++c) dst[3 * i + c] = buf[(size_t)c * nb + i];

@@ -53,8 +53,9 @@ void record_state(nb_system *s, uint64_t step) {
     const int64_t n = s->n;
     const uint64_t cap = s->rec_cap;
     if (!s->rec_every || !cap) return;
+    const bool state = s->rec_what & NB_RECORD_STATE, energy = s->rec_what & NB_RECORD_ENERGY;
     const size_t per = (size_t)3 * (size_t)std::max<int64_t>(s->rec_n0, n);
-    if (s->rec_x.n < per * cap) {
+    if (state && s->rec_x.n < per * cap) {
         // (re)sizing drops what was recorded
         s->rec_x.ensure(per * cap);
         s->rec_v.ensure(per * cap);
@@ -70,12 +71,18 @@ void record_state(nb_system *s, uint64_t step) {
     const uint64_t slot = s->rec_total % cap;
     const size_t stride = (size_t)3 * (size_t)s->rec_n0;
     hipStream_t st = s->stream;
-    if (n > 0) {
+    if (n > 0 && state) {
         NB_HIP(hipMemcpyAsync(s->rec_x.p + slot * stride, s->x.p, (size_t)3 * n * 8, hipMemcpyDeviceToDevice, st));
         NB_HIP(hipMemcpyAsync(s->rec_v.p + slot * stride, s->v.p, (size_t)3 * n * 8, hipMemcpyDeviceToDevice, st));
-        launch_pe_fp64(st, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->pe.p);
     }
-    hipLaunchKernelGGL(k_energy_totals, dim3(1), dim3(64), 0, st, n, s->m.p, s->v.p, s->pe.p, s->rec_e.p + 2 * slot);
+    if (energy) {
+        if (n > 0) launch_pe_fp64(st, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->pe.p);
+        hipLaunchKernelGGL(k_energy_totals, dim3(1), dim3(64), 0, st, n, s->m.p, s->v.p, s->pe.p,
+                           s->rec_e.p + 2 * slot);
+    } else {
+        hipLaunchKernelGGL(k_energy_totals, dim3(1), dim3(64), 0, st, (int64_t)0, s->m.p, s->v.p, s->pe.p,
+                           s->rec_e.p + 2 * slot);  // zero totals
+    }
     NB_HIP(hipGetLastError());
     s->rec_step[slot] = (int64_t)step;
     s->rec_n[slot] = n;

@@ -41,6 +41,7 @@ struct nb_system {
     // state recording (nb_set_recording): a ring of rec_cap records taken
     // every rec_every steps; a slot holds 3 * rec_n0 doubles per vector
     uint64_t rec_every = 0, rec_cap = 0, rec_total = 0;
+    int rec_what = 0;  // NB_RECORD_STATE | NB_RECORD_ENERGY
     int64_t rec_n0 = 0;
     nbe::DBuf<double> rec_x, rec_v, rec_e;   // [slot][x block | y | z], same for v; [slot][ke, pe]
     std::vector<int64_t> rec_step, rec_n;    // per slot: step index, body count

@@ -129,6 +129,8 @@ int nb_advance(nb_system *sys, uint64_t steps, int64_t *elapsed_ns);
  * formulae (set_ke gregbod.hpp:917-920, cumulative_pe gregsys.hpp:358-363,
  * calc_energy :608-630; bh_pe's softened form when eps > 0). */
 int nb_energy(nb_system *sys, double *ke, double *pe);
+#define NB_RECORD_STATE 1   /* positions, velocities, labels */
+#define NB_RECORD_ENERGY 2  /* total KE and PE (PE: the direct O(N^2) FP64 sum) */
 /* Engine extension in place of the memFreq template argument (gregsys.hpp:135-170,
  * body::add_pos_vel_ke gregbod.hpp:311-315, tot_ke/tot_pe/tot_e :420-476, whose
  * step loop records nothing on the paths that compile): every `every` steps of
@@ -138,9 +140,11 @@ int nb_energy(nb_system *sys, double *ke, double *pe);
  * device memory; the oldest is overwritten first.  Recording adds no host
  * synchronisation but turns the captured step graphs off; with Barnes-Hut the
  * velocities are those of the loop, half a step ahead of the positions, and
- * the PE is the direct O(N^2) sum.  every = 0 stops recording and frees the
- * ring; nb_set_bodies drops the records. */
-int nb_set_recording(nb_system *sys, uint64_t every, uint64_t capacity);
+ * the PE is the direct O(N^2) sum.  `what` is NB_RECORD_STATE and/or
+ * NB_RECORD_ENERGY (an energy-only ring keeps no state and returns zero
+ * positions and velocities; a state-only ring reports KE = PE = 0).  every = 0 stops recording and frees the ring;
+ * nb_set_bodies drops the records. */
+int nb_set_recording(nb_system *sys, uint64_t every, uint64_t capacity, int what);
 /* taken: records made since recording was set; stored: how many are kept. */
 int nb_record_count(const nb_system *sys, uint64_t *taken, uint64_t *stored);
 /* Record k of the stored ones (0 = the oldest kept): step index since the last

@@ -107,6 +107,26 @@ def test_euler_and_collision_records():
         assert np.array_equal(rec["pos"], ref["pos"])
 
 
+def test_record_what_flags():
+    print("case record what", flush=True)
+    m, pos, vel = capi.ic_uniform_ball(34, 128, 1e20, 1e22, 1e9, 1e3)
+    both = make(m, pos, vel, 10.0, 10, 0.0)
+    both.set_recording(5, 4)
+    both.evolve(capi.LEAPFROG_KDK)
+    e_only = make(m, pos, vel, 10.0, 10, 0.0)
+    e_only.set_recording(5, 4, what=2)
+    e_only.evolve(capi.LEAPFROG_KDK)
+    s_only = make(m, pos, vel, 10.0, 10, 0.0)
+    s_only.set_recording(5, 4, what=1)
+    s_only.evolve(capi.LEAPFROG_KDK)
+    for k in range(3):
+        a, b, c = both.record(k), e_only.record(k), s_only.record(k)
+        assert (a["ke"], a["pe"]) == (b["ke"], b["pe"]) and not b["pos"].any()
+        assert np.array_equal(a["pos"], c["pos"]) and c["ke"] == 0 and c["pe"] == 0
+    with pytest.raises(capi.NBodyError):
+        both.set_recording(5, 4, what=4)
+
+
 def test_bh_records_bit_exact():
     print("case record bh", flush=True)
     m, pos, vel = capi.ic_two_plummer(11, 4096)
