@@ -342,6 +342,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=1, help="--profile-direct: direct_kernel knob")
     ap.add_argument("--ipl", type=int, default=2, help="--profile-direct: direct_ipl knob")
     ap.add_argument("--waves", type=int, default=16, help="--profile-direct: direct_waves knob")
+    ap.add_argument("--profile-bh-coll", action="store_true",
+                    help="for rocprofv3: 3 BH steps with bh_s_coll on the 1M-body workload, radius 0.005")
     ap.add_argument("--profile-batch", action="store_true",
                     help="for rocprofv3: the 4096-system ensemble for --batch-steps steps, no JSON")
     ap.add_argument("--profile-bh", action="store_true",
@@ -359,6 +361,9 @@ def main():
 
     if args.profile_batch:
         print(json.dumps(bench_batch(capi, args.batch_steps)), flush=True)
+        return
+    if args.profile_bh_coll:
+        print(json.dumps(bench_bh_coll(capi, 3, 0.005)), flush=True)
         return
     if args.profile_direct or args.profile_bh:
         profile_run(capi, args)

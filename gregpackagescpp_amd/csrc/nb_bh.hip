@@ -523,18 +523,22 @@ __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, c
     if (!FILL) cnt[s] = k;
 }
 
-// Largest radius (for the walk's reach); radii are not negative.
+// Largest radius (for the walk's reach), over a grid: radii are not
+// negative, so the IEEE bit patterns order like the values and one integer
+// atomicMax per workgroup combines them (*out starts at +0.0).
 __global__ __launch_bounds__(kBlock) void k_rmax(int64_t n, const double *__restrict__ r, double *__restrict__ out) {
     __shared__ double sm[kBlock];
     double m = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += kBlock) m = fmax(m, r[i]);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        m = fmax(m, r[i]);
     sm[threadIdx.x] = m;
     __syncthreads();
     for (int off = kBlock / 2; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = sm[0];
+    if (threadIdx.x == 0)
+        atomicMax(reinterpret_cast<unsigned long long *>(out), (unsigned long long)__double_as_longlong(sm[0]));
 }
 
 // Connected components of the overlap graph (owner o -- neighbour j), by a
@@ -1071,7 +1075,8 @@ static void bh_collision_pass(nb_system *s) {
     hipLaunchKernelGGL(k_node_box, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.box.p, w.first.p, w.depth.p,
                        w.khi.p, w.klo.p, w.nbox.p);
     hipLaunchKernelGGL(k_leaf_of, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.body.p, w.first.p, w.leaf_of.p);
-    hipLaunchKernelGGL(k_rmax, dim3(1), dim3(kBlock), 0, st, n, s->r.p, w.rmax.p);
+    NB_HIP(hipMemsetAsync(w.rmax.p, 0, sizeof(double), st));
+    hipLaunchKernelGGL(k_rmax, dim3(std::min<unsigned>(1024u, blocks(n))), dim3(kBlock), 0, st, n, s->r.p, w.rmax.p);
     hipLaunchKernelGGL(k_bhc_scan<false>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.leaf_of.p,
                        w.nbox.p, w.skip.p, w.body.p, s->x.p, s->r.p, w.rmax.p, w.bc_cnt.p, 0u, nullptr, nullptr,
                        nullptr);
