@@ -48,3 +48,5 @@ def test_track_empty_subset():
     s.set_bodies(m, pos, vel)
     with pytest.raises(capi.NBodyError):
         s.track(np.zeros(0, np.int64))
+    with pytest.raises(capi.NBodyError):
+        s.track(np.array([0, 10]))  # slot 10 of a 10-body system
