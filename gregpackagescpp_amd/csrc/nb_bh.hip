@@ -482,6 +482,14 @@ __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, c
     // rounding of the box gaps below)
     const double reach = (ro + *rmax_p) * (1.0 + 1e-9) + 1e-9 * (fabs(xo) + fabs(yo) + fabs(zo));
     const double reach2 = reach * reach;
+    // every other body lies outside the own leaf's cube (one body per leaf),
+    // so a body farther than `reach` from all six faces overlaps nothing
+    const double face = fmin(fmin(fmin(xo - bb[0], bb[3] - xo), fmin(yo - bb[1], bb[4] - yo)),
+                             fmin(zo - bb[2], bb[5] - zo));
+    if (face > reach) {
+        if (!FILL) cnt[s] = 0u;
+        return;
+    }
     uint32_t k = FILL ? cnt[s] : 0u;
     int32_t p = 1;
     while (p < total) {
