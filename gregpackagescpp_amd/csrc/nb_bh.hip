@@ -545,8 +545,10 @@ __global__ __launch_bounds__(kBlock) void k_rmax(int64_t n, const double *__rest
         if ((int)threadIdx.x < off) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
         __syncthreads();
     }
+    // -0.0 (allowed as a radius) would order above every positive pattern
+    const double top = sm[0] > 0.0 ? sm[0] : 0.0;
     if (threadIdx.x == 0)
-        atomicMax(reinterpret_cast<unsigned long long *>(out), (unsigned long long)__double_as_longlong(sm[0]));
+        atomicMax(reinterpret_cast<unsigned long long *>(out), (unsigned long long)__double_as_longlong(top));
 }
 
 // Connected components of the overlap graph (owner o -- neighbour j), by a
