@@ -74,3 +74,19 @@ def test_bh_coll_mixed_tracks_fp64():
     rel = np.linalg.norm(dv - dv_ref, axis=1) / np.linalg.norm(dv_ref, axis=1)
     assert np.median(rel) < 1e-2
     assert np.all(np.isfinite(out[0]["vel"]))
+
+
+def test_bh_coll_signed_zero_and_one_large_radius():
+    """Radii of 0 and -0.0 everywhere but a few bodies of radius 0.2: the
+    walk's reach comes from the largest radius, which a -0.0 must not hide."""
+    print("case bh coll signed zero", flush=True)
+    m, pos, vel = capi.ic_two_plummer(24, 4096)
+    r = np.zeros(4096)
+    r[1::2] = -0.0
+    r[::100] = 0.2
+    e = np.full(4096, 0.7)
+    one = O.bh_s_coll(m, r, e, pos, vel, dtype=np.float64)
+    assert (one != vel).any(axis=1).sum() > 30
+    st = run(m, r, e, pos, vel, 3)
+    rp, rv, _ = O.bh_kdk_coll(m, r, e, pos, vel, 1.0, 1e-3, 0.01, 0.5, 3)
+    assert np.array_equal(st["pos"], rp) and np.array_equal(st["vel"], rv)
