@@ -29,6 +29,14 @@
  *     reference with -DGREGSYS_MERGERS (tests/golden/ref_coll_*); momentum and
  *     mass conservation; equal-mass elastic swap closed form.
  *   adjoint: central finite differences on tiny systems (tests/test_oracle.py).
+ *   integrate (Euler, modified Euler, midpoint, DKD, with and without s_coll):
+ *     bit-exact vs the compiled reference's evolve (tests/golden/ref_integ_*,
+ *     ref_icoll_*).
+ *   bh_s_coll / nn_bodies: bit-exact vs the compiled reference's own pass on
+ *     its own tree (tests/golden/ref_bhcoll_*); neighbour lists equal a
+ *     brute-force enumeration of touching leaf boxes (tests/test_bh_coll.py).
+ *   track (body_tracker): bit-exact vs the compiled reference's tracker
+ *     (tests/golden/ref_track_*); one- and two-body closed forms.
  */
 #include <omp.h>
 
