@@ -22,7 +22,8 @@ struct Status {
     do {                                                                                               \
         hipError_t _e = (expr);                                                                        \
         if (_e != hipSuccess) {                                                                        \
-            throw ::nbe::DeviceError(std::string(#expr) + ": " + hipGetErrorString(_e));               \
+            throw ::nbe::DeviceError(std::string(hipGetErrorName(_e)) + " (" + hipGetErrorString(_e) + ") in " + \
+                                     #expr);                                                           \
         }                                                                                              \
     } while (0)
 
