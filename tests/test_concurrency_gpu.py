@@ -1,6 +1,6 @@
-"""Contexts on separate host threads at the same time (one stream each, as
-INTEGRATION.md allows): results equal those of the same runs one after
-another, bit for bit."""
+"""Contexts on separate host threads at the same time (one context per host
+thread, each with its own stream): results equal those of the same runs one
+after another, bit for bit."""
 import threading
 
 import numpy as np
