@@ -1,7 +1,7 @@
 /*
  * nbody_oracle.cpp -- CPU restatement of the reference's force-and-step path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the oracle: the parity checker that
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: the parity reference that
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg compare the
  * HIP library against.  Nothing in the product library links, loads or calls
  * it.  It is written fresh from the reference's algorithm (GregW10/GregPackagesCPP,

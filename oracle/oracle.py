@@ -2,7 +2,7 @@
 
 The oracle is the CPU restatement of the reference's force-and-step path
 (nbody_oracle.cpp).  Only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg use it, and only as the checker.
+cpu_baseline leg use it, and only as the reference to compare against.
 """
 from __future__ import annotations
 
