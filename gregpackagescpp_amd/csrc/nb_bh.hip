@@ -465,7 +465,8 @@ template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, const uint32_t *__restrict__ idx,
                                                      const int32_t *__restrict__ leaf_of,
                                                      const double *__restrict__ nbox, const int32_t *__restrict__ skip,
-                                                     const int32_t *__restrict__ body, const double *__restrict__ x,
+                                                     const int32_t *__restrict__ body,
+                                                     const int32_t *__restrict__ parent, const double *__restrict__ x,
                                                      const double *__restrict__ r, const double *__restrict__ rmax_p,
                                                      uint32_t *__restrict__ cnt, uint32_t nent,
                                                      int32_t *__restrict__ e_own, int32_t *__restrict__ e_nb,
@@ -490,9 +491,21 @@ __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, c
         if (!FILL) cnt[s] = 0u;
         return;
     }
+    // every body within reach lies in the lowest ancestor whose cube holds the
+    // whole reach box around p_o, so only that subtree is walked (in the same
+    // pre-order, so the overlaps come in the same order)
+    int32_t top = parent[own];
+    while (top > 0) {
+        const double *cb = nbox + (size_t)6 * top;
+        if (xo - cb[0] > reach && cb[3] - xo > reach && yo - cb[1] > reach && cb[4] - yo > reach &&
+            zo - cb[2] > reach && cb[5] - zo > reach)
+            break;
+        top = parent[top];
+    }
+    const int32_t end = top > 0 ? skip[top] : total;
     uint32_t k = FILL ? cnt[s] : 0u;
-    int32_t p = 1;
-    while (p < total) {
+    int32_t p = top + 1;
+    while (p < end) {
         if (p == own) {
             p = skip[p];
             continue;
@@ -1088,7 +1101,7 @@ static void bh_collision_pass(nb_system *s) {
     NB_HIP(hipMemsetAsync(w.rmax.p, 0, sizeof(double), st));
     hipLaunchKernelGGL(k_rmax, dim3(std::min<unsigned>(1024u, blocks(n))), dim3(kBlock), 0, st, n, s->r.p, w.rmax.p);
     hipLaunchKernelGGL(k_bhc_scan<false>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.leaf_of.p,
-                       w.nbox.p, w.skip.p, w.body.p, s->x.p, s->r.p, w.rmax.p, w.bc_cnt.p, 0u, nullptr, nullptr,
+                       w.nbox.p, w.skip.p, w.body.p, w.parent.p, s->x.p, s->r.p, w.rmax.p, w.bc_cnt.p, 0u, nullptr, nullptr,
                        nullptr);
     const int nb = (int)((n + 1023) / 1024);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.bc_cnt.p, w.bc_blk.p);
@@ -1103,7 +1116,7 @@ static void bh_collision_pass(nb_system *s) {
         w.e_own.ensure(cap); w.e_nb.ensure(cap); w.e_geo.ensure(4 * cap); w.e_mom.ensure(2 * cap);
     }
     hipLaunchKernelGGL(k_bhc_scan<true>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.leaf_of.p,
-                       w.nbox.p, w.skip.p, w.body.p, s->x.p, s->r.p, w.rmax.p, w.bc_cnt.p, nent, w.e_own.p,
+                       w.nbox.p, w.skip.p, w.body.p, w.parent.p, s->x.p, s->r.p, w.rmax.p, w.bc_cnt.p, nent, w.e_own.p,
                        w.e_nb.p, w.e_geo.p);
     // components of the overlap graph, entries grouped by component in
     // (owner, visit) order, then one lane per component
