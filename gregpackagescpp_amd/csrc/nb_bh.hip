@@ -837,7 +837,12 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
                                                       double *__restrict__ accs,
                                                       unsigned long long *__restrict__ icount) {
 #pragma clang fp contract(fast)
-    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    // Workgroups are dealt to the 8 XCDs round robin; renumber them so that
+    // each XCD walks one contiguous Morton range (its own part of the tree
+    // stays in its L2).  Applies when the grid splits evenly.
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    const uint32_t lb = (g % 8u == 0u) ? (b % 8u) * (g / 8u) + b / 8u : b;
+    const int64_t s = (int64_t)lb * kBlock + threadIdx.x;
     const bool live = s < n;
     const float4 p = live ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint64_t active = __ballot(live);
