@@ -135,6 +135,20 @@ def bench_bh(capi, steps):
                         "eps=0.01, dt=1e-3, FP32 wave walk, FP64 state and node COM"}
 
 
+def host_info():
+    """The host the CPU baselines ran on: CPU model and logical CPU count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
 def bench_bh_coll(capi, steps, radius):
     """The BH step with bh_s_coll (touching-cell overlaps, bounces) on the same
     1M-body two-Plummer workload with every radius `radius` (0.005: about 48k
@@ -438,6 +452,7 @@ def main():
         line["bh_coll"] = [bench_bh_coll(capi, 3, 0.002), bench_bh_coll(capi, 3, 0.005)]
         line["integrators"] = bench_integrators(capi)
     if not args.no_cpu_baseline:
+        line["host"] = host_info()
         line["cpu_baseline"] = cpu_baseline(m, pos, vel)
         line["cpu_baseline_allcore"] = cpu_direct_allcore(m, pos)
         if not args.no_bh:
