@@ -205,8 +205,10 @@ void adjoint_run(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT,
         const uint64_t q0 = seg * ckpt, len = std::min<uint64_t>(ckpt, steps - q0);
         const bool last = seg + 1 == nseg;
         if (graphs && !last && len == ckpt) {
-            if (!w.fwd_seg.valid()) w.fwd_seg.capture(st, [&]() { fwd_steps(q0, len); });
-            w.fwd_seg.launch(st);
+            if (w.fwd_seg.valid() || w.fwd_seg.capture(st, [&]() { fwd_steps(q0, len); }))
+                w.fwd_seg.launch(st);
+            else
+                fwd_steps(q0, len);  // capture invalidated from outside: the same launches, eagerly
         } else {
             fwd_steps(q0, len);
         }
@@ -274,8 +276,10 @@ void adjoint_run(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT,
         copy(c + n3, s->v.p, n3);
         const bool first = seg + 1 == nseg;
         if (graphs && !first && seg > 0 && len == ckpt) {
-            if (!w.rev_seg.valid()) w.rev_seg.capture(st, [&]() { rev_segment(start, len, false); });
-            w.rev_seg.launch(st);
+            if (w.rev_seg.valid() || w.rev_seg.capture(st, [&]() { rev_segment(start, len, false); }))
+                w.rev_seg.launch(st);
+            else
+                rev_segment(start, len, false);
         } else {
             rev_segment(start, len, first);
         }
@@ -285,13 +289,13 @@ void adjoint_run(nb_system *s, uint64_t steps, uint64_t ckpt, const double *gxT,
     auto t3 = std::chrono::high_resolution_clock::now();
 
     // ---- gradients out; state back to time T ----
-    NB_HIP(hipMemcpy(tmp.data(), w.lx.p, n3 * 8, hipMemcpyDeviceToHost));
+    copy_wait(tmp.data(), w.lx.p, n3 * 8, hipMemcpyDeviceToHost, st);
     for (int64_t i = 0; i < n; ++i)
         for (int c = 0; c < 3; ++c) gx0[3 * i + c] = tmp[(size_t)(c * n + i)];
-    NB_HIP(hipMemcpy(tmp.data(), w.lv.p, n3 * 8, hipMemcpyDeviceToHost));
+    copy_wait(tmp.data(), w.lv.p, n3 * 8, hipMemcpyDeviceToHost, st);
     for (int64_t i = 0; i < n; ++i)
         for (int c = 0; c < 3; ++c) gv0[3 * i + c] = tmp[(size_t)(c * n + i)];
-    NB_HIP(hipMemcpy(gm, w.lm.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    copy_wait(gm, w.lm.p, (size_t)n * 8, hipMemcpyDeviceToHost, st);
     copy(w.xT.p, s->x.p, n3);
     copy(w.vT.p, s->v.p, n3);
     force_at(s->x.p);

@@ -287,14 +287,19 @@ void direct_kdk_steps(nb_system *s, uint64_t steps) {
         if (n == 0) break;
         if (graphs && st > 1 && steps - st >= (uint64_t)kGraphSteps) {
             const GraphKey key = graph_key(s);
-            if (!s->step_graph.valid() || !(s->step_graph_key == key)) {
-                s->step_graph.capture(s->stream, [&]() {
-                    bool open = need_open;
-                    for (int q = 0; q < kGraphSteps; ++q) open = issue_step(open, false);
-                });
+            auto chunk = [&]() {
+                bool open = need_open;
+                for (int q = 0; q < kGraphSteps; ++q) open = issue_step(open, false);
+            };
+            bool have = s->step_graph.valid() && s->step_graph_key == key;
+            if (!have) {
+                have = s->step_graph.capture(s->stream, chunk);
                 s->step_graph_key = key;
             }
-            s->step_graph.launch(s->stream);
+            if (have)
+                s->step_graph.launch(s->stream);
+            else
+                chunk();  // capture invalidated from outside: the same launches, eagerly
             st += kGraphSteps;
             s->step_count += kGraphSteps;
             continue;
@@ -387,15 +392,15 @@ int nb_set_bodies(nb_system *s, int64_t n, const double *m, const double *r, con
         s->x.ensure(3 * nn); s->v.ensure(3 * nn); s->a.ensure(3 * nn);
         if (n > 0) {
             std::vector<double> tmp;
-            NB_HIP(hipMemcpy(s->m.p, m, (size_t)n * 8, hipMemcpyHostToDevice));
+            copy_wait(s->m.p, m, (size_t)n * 8, hipMemcpyHostToDevice, s->stream);
             tmp.assign((size_t)n, 1.0);
-            NB_HIP(hipMemcpy(s->r.p, r ? r : tmp.data(), (size_t)n * 8, hipMemcpyHostToDevice));
-            NB_HIP(hipMemcpy(s->e.p, rest_c ? rest_c : tmp.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+            copy_wait(s->r.p, r ? r : tmp.data(), (size_t)n * 8, hipMemcpyHostToDevice, s->stream);
+            copy_wait(s->e.p, rest_c ? rest_c : tmp.data(), (size_t)n * 8, hipMemcpyHostToDevice, s->stream);
             to_soa(n, pos, tmp);
-            NB_HIP(hipMemcpy(s->x.p, tmp.data(), (size_t)(3 * n) * 8, hipMemcpyHostToDevice));
+            copy_wait(s->x.p, tmp.data(), (size_t)(3 * n) * 8, hipMemcpyHostToDevice, s->stream);
             to_soa(n, vel, tmp);
-            NB_HIP(hipMemcpy(s->v.p, tmp.data(), (size_t)(3 * n) * 8, hipMemcpyHostToDevice));
-            NB_HIP(hipMemset(s->a.p, 0, (size_t)(3 * n) * 8));
+            copy_wait(s->v.p, tmp.data(), (size_t)(3 * n) * 8, hipMemcpyHostToDevice, s->stream);
+            zero_wait(s->a.p, (size_t)(3 * n) * 8, s->stream);
         }
         s->labels.resize((size_t)n);
         for (int64_t i = 0; i < n; ++i) s->labels[(size_t)i] = i;
@@ -420,15 +425,15 @@ int nb_get_state(const nb_system *cs, int64_t cap, double *m, double *r, double 
         if (cap < n) throw ApiError(NB_ERR_CAPACITY, "nb_get_state: capacity smaller than the body count");
         NB_HIP(hipStreamSynchronize(s->stream));
         if (n == 0) return NB_OK;
-        if (m) NB_HIP(hipMemcpy(m, s->m.p, (size_t)n * 8, hipMemcpyDeviceToHost));
-        if (r) NB_HIP(hipMemcpy(r, s->r.p, (size_t)n * 8, hipMemcpyDeviceToHost));
-        if (rest_c) NB_HIP(hipMemcpy(rest_c, s->e.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+        if (m) copy_wait(m, s->m.p, (size_t)n * 8, hipMemcpyDeviceToHost, s->stream);
+        if (r) copy_wait(r, s->r.p, (size_t)n * 8, hipMemcpyDeviceToHost, s->stream);
+        if (rest_c) copy_wait(rest_c, s->e.p, (size_t)n * 8, hipMemcpyDeviceToHost, s->stream);
         std::vector<double> tmp((size_t)(3 * n));
         const double *srcs[3] = {s->x.p, s->v.p, s->a.p};
         double *dsts[3] = {pos, vel, acc};
         for (int q = 0; q < 3; ++q) {
             if (!dsts[q]) continue;
-            NB_HIP(hipMemcpy(tmp.data(), srcs[q], (size_t)(3 * n) * 8, hipMemcpyDeviceToHost));
+            copy_wait(tmp.data(), srcs[q], (size_t)(3 * n) * 8, hipMemcpyDeviceToHost, s->stream);
             from_soa(n, tmp, dsts[q]);
         }
         return NB_OK;
@@ -705,7 +710,7 @@ int nb_get_record(nb_system *s, uint64_t k, int64_t *step, int64_t *n, double *p
                 std::fill(dst, dst + 3 * nb, 0.0);
                 return;
             }
-            NB_HIP(hipMemcpy(buf.data(), src.p + slot * stride, buf.size() * 8, hipMemcpyDeviceToHost));
+            copy_wait(buf.data(), src.p + slot * stride, buf.size() * 8, hipMemcpyDeviceToHost, s->stream);
             for (int64_t i = 0; i < nb; ++i)
                 for (int c = 0; c < 3; ++c) dst[3 * i + c] = buf[(size_t)c * nb + i];
         };
@@ -764,7 +769,7 @@ int nb_bh_accelerations(nb_system *s, double *acc, int64_t *interactions) {
         s->prof.collect();
         if (acc) {
             std::vector<double> tmp((size_t)(3 * s->n));
-            NB_HIP(hipMemcpy(tmp.data(), s->a.p, tmp.size() * 8, hipMemcpyDeviceToHost));
+            copy_wait(tmp.data(), s->a.p, tmp.size() * 8, hipMemcpyDeviceToHost, s->stream);
             from_soa(s->n, tmp, acc);
         }
         return NB_OK;

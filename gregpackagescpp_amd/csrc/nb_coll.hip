@@ -98,7 +98,7 @@ void collision_pass(nb_system *s, uint64_t step) {
     if (errv) throw ApiError(NB_ERR_CAPACITY, "collision pass: more than 256 overlapping partners for one body");
     if (nlog > 0) {
         std::vector<int64_t> rec(4 * (size_t)nlog);
-        NB_HIP(hipMemcpy(rec.data(), w.log.p, rec.size() * 8, hipMemcpyDeviceToHost));
+        copy_wait(rec.data(), w.log.p, rec.size() * 8, hipMemcpyDeviceToHost, st);
         s->merge_log.insert(s->merge_log.end(), rec.begin(), rec.end());
         w.synced_log = (int64_t)(s->merge_log.size() / 4);
     }
@@ -109,17 +109,17 @@ void collision_pass(nb_system *s, uint64_t step) {
         std::vector<double> buf;
         for (DBuf<double> *d : {&s->x, &s->v, &s->a}) {
             buf.resize(3 * (size_t)n);
-            NB_HIP(hipMemcpy(buf.data(), d->p, buf.size() * 8, hipMemcpyDeviceToHost));
+            copy_wait(buf.data(), d->p, buf.size() * 8, hipMemcpyDeviceToHost, st);
             std::vector<double> packed(3 * (size_t)m2);
             for (int c = 0; c < 3; ++c)
                 std::copy(buf.begin() + (size_t)c * n, buf.begin() + (size_t)c * n + m2,
                           packed.begin() + (size_t)c * m2);
-            NB_HIP(hipMemcpy(d->p, packed.data(), packed.size() * 8, hipMemcpyHostToDevice));
+            copy_wait(d->p, packed.data(), packed.size() * 8, hipMemcpyHostToDevice, st);
         }
         s->n = m2;
     }
     s->labels.resize((size_t)s->n);
-    if (s->n) NB_HIP(hipMemcpy(s->labels.data(), w.lab.p, (size_t)s->n * 8, hipMemcpyDeviceToHost));
+    if (s->n) copy_wait(s->labels.data(), w.lab.p, (size_t)s->n * 8, hipMemcpyDeviceToHost, st);
 }
 
 }  // namespace nbe
@@ -213,15 +213,15 @@ int nb_batch_create(int device, int64_t nsys, const int64_t *offsets, const doub
         auto up3 = [&](DBuf<double> &d, const double *src) {
             for (int64_t i = 0; i < total; ++i)
                 for (int c = 0; c < 3; ++c) tmp[(size_t)c * T + (size_t)i] = src ? src[3 * i + c] : 0.0;
-            NB_HIP(hipMemcpy(d.p, tmp.data(), 3 * T * 8, hipMemcpyHostToDevice));
+            copy_wait(d.p, tmp.data(), 3 * T * 8, hipMemcpyHostToDevice, b->stream);
         };
         up3(b->x, pos);
         up3(b->v, vel);
         up3(b->a, nullptr);
         std::vector<double> one(T, 1.0);
-        NB_HIP(hipMemcpy(b->m.p, m, (size_t)total * 8, hipMemcpyHostToDevice));
-        NB_HIP(hipMemcpy(b->r.p, r ? r : one.data(), (size_t)total * 8, hipMemcpyHostToDevice));
-        NB_HIP(hipMemcpy(b->e.p, rest_c ? rest_c : one.data(), (size_t)total * 8, hipMemcpyHostToDevice));
+        copy_wait(b->m.p, m, (size_t)total * 8, hipMemcpyHostToDevice, b->stream);
+        copy_wait(b->r.p, r ? r : one.data(), (size_t)total * 8, hipMemcpyHostToDevice, b->stream);
+        copy_wait(b->e.p, rest_c ? rest_c : one.data(), (size_t)total * 8, hipMemcpyHostToDevice, b->stream);
         std::vector<int64_t> lab(T), next((size_t)nsys), zeros((size_t)nsys, 0);
         std::vector<int> cnt((size_t)nsys);
         for (int64_t s = 0; s < nsys; ++s) {
@@ -234,13 +234,13 @@ int nb_batch_create(int device, int64_t nsys, const int64_t *offsets, const doub
         b->order.resize((size_t)nsys);
         std::iota(b->order.begin(), b->order.end(), 0);
         std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) { return cnt[(size_t)a] > cnt[(size_t)c]; });
-        NB_HIP(hipMemcpy(b->lab.p, lab.data(), (size_t)total * 8, hipMemcpyHostToDevice));
-        NB_HIP(hipMemcpy(b->off.p, offsets, (size_t)(nsys + 1) * 8, hipMemcpyHostToDevice));
-        NB_HIP(hipMemcpy(b->next.p, next.data(), (size_t)nsys * 8, hipMemcpyHostToDevice));
-        NB_HIP(hipMemcpy(b->loglen.p, zeros.data(), (size_t)nsys * 8, hipMemcpyHostToDevice));
-        NB_HIP(hipMemcpy(b->cnt.p, cnt.data(), (size_t)nsys * 4, hipMemcpyHostToDevice));
-        NB_HIP(hipMemcpy(b->ord.p, b->order.data(), (size_t)nsys * 4, hipMemcpyHostToDevice));
-        NB_HIP(hipMemset(b->errf.p, 0, sizeof(int)));
+        copy_wait(b->lab.p, lab.data(), (size_t)total * 8, hipMemcpyHostToDevice, b->stream);
+        copy_wait(b->off.p, offsets, (size_t)(nsys + 1) * 8, hipMemcpyHostToDevice, b->stream);
+        copy_wait(b->next.p, next.data(), (size_t)nsys * 8, hipMemcpyHostToDevice, b->stream);
+        copy_wait(b->loglen.p, zeros.data(), (size_t)nsys * 8, hipMemcpyHostToDevice, b->stream);
+        copy_wait(b->cnt.p, cnt.data(), (size_t)nsys * 4, hipMemcpyHostToDevice, b->stream);
+        copy_wait(b->ord.p, b->order.data(), (size_t)nsys * 4, hipMemcpyHostToDevice, b->stream);
+        zero_wait(b->errf.p, sizeof(int), b->stream);
         *out = b;
         return NB_OK;
     });
@@ -320,17 +320,17 @@ int nb_batch_get_state(const nb_batch *cb, int64_t cap, int64_t *counts, double 
         NB_HIP(hipStreamSynchronize(b->stream));
         const int64_t T = b->total;
         std::vector<int> cnt((size_t)b->nsys);
-        NB_HIP(hipMemcpy(cnt.data(), b->cnt.p, cnt.size() * 4, hipMemcpyDeviceToHost));
+        copy_wait(cnt.data(), b->cnt.p, cnt.size() * 4, hipMemcpyDeviceToHost, b->stream);
         int64_t live = 0;
         for (int c : cnt) live += c;
         if (cap < live) throw ApiError(NB_ERR_CAPACITY, "nb_batch_get_state: capacity too small");
         std::vector<double> hx(3 * (size_t)T), hv(3 * (size_t)T), hm((size_t)T), hr((size_t)T);
         std::vector<int64_t> hl((size_t)T);
-        NB_HIP(hipMemcpy(hx.data(), b->x.p, hx.size() * 8, hipMemcpyDeviceToHost));
-        NB_HIP(hipMemcpy(hv.data(), b->v.p, hv.size() * 8, hipMemcpyDeviceToHost));
-        NB_HIP(hipMemcpy(hm.data(), b->m.p, hm.size() * 8, hipMemcpyDeviceToHost));
-        NB_HIP(hipMemcpy(hr.data(), b->r.p, hr.size() * 8, hipMemcpyDeviceToHost));
-        NB_HIP(hipMemcpy(hl.data(), b->lab.p, hl.size() * 8, hipMemcpyDeviceToHost));
+        copy_wait(hx.data(), b->x.p, hx.size() * 8, hipMemcpyDeviceToHost, b->stream);
+        copy_wait(hv.data(), b->v.p, hv.size() * 8, hipMemcpyDeviceToHost, b->stream);
+        copy_wait(hm.data(), b->m.p, hm.size() * 8, hipMemcpyDeviceToHost, b->stream);
+        copy_wait(hr.data(), b->r.p, hr.size() * 8, hipMemcpyDeviceToHost, b->stream);
+        copy_wait(hl.data(), b->lab.p, hl.size() * 8, hipMemcpyDeviceToHost, b->stream);
         int64_t k = 0;
         for (int64_t s = 0; s < b->nsys; ++s) {
             counts[s] = cnt[(size_t)s];
@@ -355,8 +355,8 @@ int nb_batch_merge_log(const nb_batch *cb, int64_t cap, int64_t *records, int64_
     return bguard(b, [&]() {
         NB_HIP(hipStreamSynchronize(b->stream));
         std::vector<int64_t> lens((size_t)b->nsys), lg(4 * (size_t)std::max<int64_t>(b->total, 1));
-        NB_HIP(hipMemcpy(lens.data(), b->loglen.p, lens.size() * 8, hipMemcpyDeviceToHost));
-        NB_HIP(hipMemcpy(lg.data(), b->log.p, lg.size() * 8, hipMemcpyDeviceToHost));
+        copy_wait(lens.data(), b->loglen.p, lens.size() * 8, hipMemcpyDeviceToHost, b->stream);
+        copy_wait(lg.data(), b->log.p, lg.size() * 8, hipMemcpyDeviceToHost, b->stream);
         int64_t tot = 0;
         for (auto l : lens) tot += l;
         *count = tot;

@@ -37,6 +37,19 @@ struct ApiError {
     ApiError(int c, std::string w) : code(c), what(std::move(w)) {}
 };
 
+// Blocking copies and fills go on the context's own stream, never on the
+// legacy null stream: HIP refuses a null-stream copy while any stream of the
+// process is being captured into a graph, and invalidates that capture, so
+// one context's copy would break another thread's step graph.
+inline void copy_wait(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t st) {
+    NB_HIP(hipMemcpyAsync(dst, src, bytes, kind, st));
+    NB_HIP(hipStreamSynchronize(st));
+}
+inline void zero_wait(void *dst, size_t bytes, hipStream_t st) {
+    NB_HIP(hipMemsetAsync(dst, 0, bytes, st));
+    NB_HIP(hipStreamSynchronize(st));
+}
+
 // Device buffer owned by the library.
 template <typename T>
 struct DBuf {
@@ -105,25 +118,41 @@ struct StepGraph {
         if (exec) (void)hipGraphExecDestroy(exec);
         exec = nullptr;
     }
+    // Returns false, with nothing launched and no graph kept, when the capture
+    // was invalidated from outside (another thread's call that HIP refuses
+    // during a capture); the caller then issues the same launches eagerly.
     template <typename F>
-    void capture(hipStream_t st, F &&f) {
+    bool capture(hipStream_t st, F &&f) {
         reset();
         NB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipGraph_t g = nullptr;
         try {
             f();
         } catch (...) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            (void)hipStreamIsCapturing(st, &cs);
             (void)hipStreamEndCapture(st, &g);
             if (g) (void)hipGraphDestroy(g);
+            if (cs == hipStreamCaptureStatusInvalidated) {
+                (void)hipGetLastError();
+                return false;
+            }
             throw;
         }
-        NB_HIP(hipStreamEndCapture(st, &g));
+        const hipError_t ee = hipStreamEndCapture(st, &g);
+        if (ee == hipErrorStreamCaptureInvalidated) {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return false;
+        }
+        NB_HIP(ee);
         const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) {
             exec = nullptr;
             throw DeviceError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         }
+        return true;
     }
     void launch(hipStream_t st) { NB_HIP(hipGraphLaunch(exec, st)); }
 };
