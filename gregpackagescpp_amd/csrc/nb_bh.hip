@@ -51,7 +51,6 @@ constexpr int kHiLevels = 21; // digits in the high key
 constexpr int kMaxLog = 32;
 constexpr int kComGrid = 512;  // workgroups of a per-depth COM launch
 constexpr int kComDeep = 20;   // depths >= this are summed by one workgroup in one launch
-constexpr int kBodyPad = 4;    // extra entries after the sorted FP32 bodies (grouped bucket loads)
 
 enum { BH_ERR_NONE = 0, BH_ERR_PLACEMENT = 1, BH_ERR_INDIST = 2, BH_ERR_DEPTH = 3 };
 }  // namespace
@@ -831,6 +830,15 @@ __device__ __forceinline__ void monopole(float4 q, float4 p, float eps2, bool so
     az = __builtin_fmaf(w, dz, az);
 }
 
+// Orders this wave's LDS writes before its later LDS reads by other lanes
+// (the LDS unit serves one wave's requests in order; the fences stop the
+// compiler from moving the accesses across this point).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool SOFT>
 __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, const float4 *__restrict__ pos,
                                                       const WalkNode *__restrict__ wnode, int bucket, float eps2,
@@ -847,6 +855,8 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
     const float4 p = live ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint64_t active = __ballot(live);
     if (active == 0) return;
+    __shared__ float4 s_leaf[kBlock / 64][64];
+    const unsigned lane = __lane_id(), wv = threadIdx.x / 64;
     float ax = 0.f, ay = 0.f, az = 0.f;
     unsigned cnt = 0;
     int32_t node = 1;
@@ -870,15 +880,26 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
             ++cnt;
             node = r.x;
         } else if (r.z <= bucket) {
-            // four contiguous bodies per scalar load (the sorted array is
-            // padded by kBodyPad entries, so the last group stays in bounds)
-            const int32_t je = r.y + r.z;
-            for (int32_t j = r.y; j < je; j += 4) {
-                const float4 b0 = pos[j], b1 = pos[j + 1], b2 = pos[j + 2], b3 = pos[j + 3];
-                monopole(b0, p, eps2, SOFT, ax, ay, az);
-                if (j + 1 < je) monopole(b1, p, eps2, SOFT, ax, ay, az);
-                if (j + 2 < je) monopole(b2, p, eps2, SOFT, ax, ay, az);
-                if (j + 3 < je) monopole(b3, p, eps2, SOFT, ax, ay, az);
+            // the node's bodies, 64 at a time: one coalesced load by the
+            // lanes into this wave's LDS slot, then the same-address reads
+            // broadcast each body, so the monopoles take VGPR operands (an
+            // SGPR operand halves the VALU issue rate).  Same order as a
+            // body-by-body loop.
+            for (int32_t base = 0; base < r.z; base += 64) {
+                const int32_t k = min(64, r.z - base);
+                if ((int32_t)lane < k) s_leaf[wv][lane] = pos[r.y + base + (int32_t)lane];
+                wave_lds_sync();
+                int32_t j = 0;
+                for (; j + 4 <= k; j += 4) {
+                    const float4 b0 = s_leaf[wv][j], b1 = s_leaf[wv][j + 1], b2 = s_leaf[wv][j + 2],
+                                 b3 = s_leaf[wv][j + 3];
+                    monopole(b0, p, eps2, SOFT, ax, ay, az);
+                    monopole(b1, p, eps2, SOFT, ax, ay, az);
+                    monopole(b2, p, eps2, SOFT, ax, ay, az);
+                    monopole(b3, p, eps2, SOFT, ax, ay, az);
+                }
+                for (; j < k; ++j) monopole(s_leaf[wv][j], p, eps2, SOFT, ax, ay, az);
+                wave_lds_sync();  // every lane has read the slot before it is refilled
             }
             cnt += (unsigned)r.z;
             node = r.x;
@@ -976,7 +997,7 @@ static void bh_build(nb_system *s) {
     ScopedTimer tm(s->prof, "bh_build", st);
     if (w.cap_n < n) {
         w.khi.ensure(n); w.klo.ensure(n); w.klo_tmp.ensure(n); w.idx.ensure(n);
-        w.cnt.ensure(n); w.pos32s.ensure(n + kBodyPad); w.accs.ensure(3 * (size_t)n);
+        w.cnt.ensure(n); w.pos32s.ensure(n); w.accs.ensure(3 * (size_t)n);
         w.st.ensure((size_t)kMaxLog * n);
         w.blk.ensure((size_t)(n + 1023) / 1024 + 1);
         w.sorter.ensure(n);
