@@ -55,13 +55,8 @@ constexpr int kComDeep = 20;   // depths >= this are summed by one workgroup in 
 enum { BH_ERR_NONE = 0, BH_ERR_PLACEMENT = 1, BH_ERR_INDIST = 2, BH_ERR_DEPTH = 3 };
 }  // namespace
 
-// Walk record of a node: FP32 COM and G*mass, then skip pointer, first sorted
-// body, body count and the FP32 bits of the squared opening radius, in one
-// 32-byte record so that a node visit reads one cache line.
-struct __align__(32) WalkNode {
-    float4 q;
-    int4 r;
-};
+#define NB_WAVES_PER_EU(w) __attribute__((amdgpu_waves_per_eu(w)))
+#include "nb_walk_dev.hpp"
 
 struct BHWork {
     int64_t cap_n = 0, cap_nodes = 0;
@@ -83,7 +78,19 @@ struct BHWork {
     DBuf<uint64_t> lvl_key; // node depth (leaves: 63), radix-sorted with
     DBuf<uint32_t> lvl_node;//   the node index: internal nodes grouped by depth
     DBuf<uint32_t> lvl_off; // first position of each depth in the sorted list
-    DBuf<WalkNode> wnode;   // walk record per node (one 32-byte line)
+    DBuf<WalkNode> wnode;   // walk record per node (one 32-byte line), DFS order
+    DBuf<WalkNode> wgrp;    // the same records with each node's children adjacent
+    DBuf<uint8_t> rank;     // index of a node among its parent's children
+    DBuf<uint32_t> cbase;   // exclusive scan of nchild (children of x at 1 + cbase[x])
+    DBuf<uint32_t> cblk, ctot;
+    DBuf<int32_t> fstack;   // frontier walk: one node stack per resident wave
+    DBuf<uint32_t> fctr;    // frontier walk: next group of each XCD's range
+    DBuf<int> werr;         // frontier walk: a stack would have overflowed
+    DBuf<uint32_t> gcost;   // interactions of each 64-body group in the last walk
+    DBuf<int32_t> gorder;   // groups of each XCD's range, heaviest first
+    int64_t gcost_n = 0;
+    DBuf<unsigned long long> icount2;  // count of the gated wave walk
+    int cus = 0;
     int bucket = 32;        // rejected nodes with at most this many bodies are summed directly
     DBuf<float4> pos32s;    // sorted body positions (FP32) and G*m
     DBuf<double> accs;      // sorted accelerations, SoA
@@ -378,23 +385,30 @@ __global__ void k_nodes(int64_t n, int logn, const uint8_t *__restrict__ st, con
 }
 
 __global__ void k_children(int32_t total, const int32_t *__restrict__ skip, const int32_t *__restrict__ body,
-                           int32_t *__restrict__ parent, uint8_t *__restrict__ nchild, uint32_t *__restrict__ ready) {
+                           int32_t *__restrict__ parent, uint8_t *__restrict__ nchild, uint32_t *__restrict__ ready,
+                           uint8_t *__restrict__ rank, uint32_t *__restrict__ cbase) {
     const int32_t x = blockIdx.x * kBlock + threadIdx.x;
     if (x >= total) return;
     ready[x] = 0;
-    if (x == 0) parent[0] = -1;
+    if (x == 0) {
+        parent[0] = -1;
+        rank[0] = 0;
+    }
     if (body[x] >= 0) {
         nchild[x] = 0;
+        cbase[x] = 0;
         return;
     }
     int32_t ch = x + 1, k = 0;
     const int32_t end = skip[x];
     while (ch < end) {
         parent[ch] = x;
+        rank[ch] = (uint8_t)k;
         ch = skip[ch];
         ++k;
     }
     nchild[x] = (uint8_t)k;
+    cbase[x] = (uint32_t)k;  // scanned into the first child's slot
 }
 
 // side = top.x - btm.x of the node, replaying set_corners' recursion on x.
@@ -782,12 +796,17 @@ __global__ __launch_bounds__(1024) void k_com_deep(int32_t total, int lo, const 
 }
 
 // Walk record per node: q = (com, G*mass) in FP32 and r = (skip, first
-// sorted body, body count, r^2_open as float bits; -1 for leaves).
+// sorted body, body count, r^2_open as float bits; -1 for leaves).  The
+// grouped copy puts the root at 0 and the children of y at 1 + cbase[y] +
+// rank, in octant order, with r = (first child's slot, first sorted body,
+// min(count, 2^27 - 1) << 4 | child count, r^2_open).
 __global__ void k_walk_prep(int32_t total, const double *__restrict__ mass, const double *__restrict__ com,
                             const double *__restrict__ side, const int32_t *__restrict__ body,
                             const int32_t *__restrict__ skip, const int32_t *__restrict__ first,
                             const int32_t *__restrict__ last, double G, double theta,
-                            WalkNode *__restrict__ wnode) {
+                            WalkNode *__restrict__ wnode, const int32_t *__restrict__ parent,
+                            const uint8_t *__restrict__ rank, const uint8_t *__restrict__ nchild,
+                            const uint32_t *__restrict__ cbase, WalkNode *__restrict__ wgrp) {
     const int32_t x = blockIdx.x * kBlock + threadIdx.x;
     if (x >= total) return;
     WalkNode w;
@@ -801,6 +820,14 @@ __global__ void k_walk_prep(int32_t total, const double *__restrict__ mass, cons
     } else r2o = INFINITY;
     w.r = make_int4(skip[x], first[x], last[x] - first[x] + 1, __float_as_int(r2o));
     wnode[x] = w;
+    if (!wgrp) return;
+    const int32_t at = x == 0 ? 0 : 1 + (int32_t)cbase[parent[x]] + (int32_t)rank[x];
+    const int nc = nchild[x];
+    const int32_t cnt = min(last[x] - first[x] + 1, (1 << 27) - 1);
+    WalkNode g;
+    g.q = w.q;
+    g.r = make_int4(nc ? 1 + (int32_t)cbase[x] : 0, first[x], (cnt << 4) | nc, __float_as_int(r2o));
+    wgrp[at] = g;
 }
 
 __global__ void k_sorted_pos(int64_t n, const uint32_t *__restrict__ idx, const double *__restrict__ x,
@@ -811,110 +838,46 @@ __global__ void k_sorted_pos(int64_t n, const uint32_t *__restrict__ idx, const 
     out[s] = make_float4((float)x[b], (float)x[n + b], (float)x[2 * n + b], (float)(G * m[b]));
 }
 
-// ---- 6a. per-wave walk, FP32 monopoles ----
-// The wave shares one node sequence.  A node is accepted when every lane's MAC
-// holds; a rejected node with at most `bucket` bodies is not descended but
-// summed body by body (its bodies are contiguous in sorted order), which
-// replaces up to 2*bucket-1 dependent node visits by a streamed loop (the
-// autotuner picks the bucket; 1 disables it).
-__device__ __forceinline__ void monopole(float4 q, float4 p, float eps2, bool soft, float &ax, float &ay,
-                                         float &az) {
-    const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-    const float s2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
-    float inv = __builtin_amdgcn_rsqf(s2);
-    if (!soft) inv = s2 > 0.f ? inv : 0.f;  // own body at eps = 0
-    float w = q.w * inv;
-    w = w * (inv * inv);
-    ax = __builtin_fmaf(w, dx, ax);
-    ay = __builtin_fmaf(w, dy, ay);
-    az = __builtin_fmaf(w, dz, az);
-}
-
-// Orders this wave's LDS writes before its later LDS reads by other lanes
-// (the LDS unit serves one wave's requests in order; the fences stop the
-// compiler from moving the accesses across this point).
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <bool SOFT>
-__global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, const float4 *__restrict__ pos,
-                                                      const WalkNode *__restrict__ wnode, int bucket, float eps2,
-                                                      double *__restrict__ accs,
-                                                      unsigned long long *__restrict__ icount) {
-#pragma clang fp contract(fast)
-    // Workgroups are dealt to the 8 XCDs round robin; renumber them so that
-    // each XCD walks one contiguous Morton range (its own part of the tree
-    // stays in its L2).  Applies when the grid splits evenly.
-    const uint32_t g = gridDim.x, b = blockIdx.x;
-    const uint32_t lb = (g % 8u == 0u) ? (b % 8u) * (g / 8u) + b / 8u : b;
-    const int64_t s = (int64_t)lb * kBlock + threadIdx.x;
-    const bool live = s < n;
-    const float4 p = live ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint64_t active = __ballot(live);
-    if (active == 0) return;
-    __shared__ float4 s_leaf[kBlock / 64][64];
-    const unsigned lane = __lane_id(), wv = threadIdx.x / 64;
-    float ax = 0.f, ay = 0.f, az = 0.f;
-    unsigned cnt = 0;
-    int32_t node = 1;
-    while (node < total) {
-        node = __builtin_amdgcn_readfirstlane(node);
-        const WalkNode nd = wnode[node];
-        const float4 q = nd.q;
-        const int4 r = nd.r;
-        const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        const bool acc_lane = d2 > __int_as_float(r.w);
-        if ((__ballot(acc_lane) & active) == active) {
-            // the monopole of the accepted node from the MAC's d and |d|^2
-            const float s2 = d2 + eps2;
-            float inv = __builtin_amdgcn_rsqf(s2);
-            if (!SOFT) inv = s2 > 0.f ? inv : 0.f;
-            const float w = (q.w * inv) * (inv * inv);
-            ax = __builtin_fmaf(w, dx, ax);
-            ay = __builtin_fmaf(w, dy, ay);
-            az = __builtin_fmaf(w, dz, az);
-            ++cnt;
-            node = r.x;
-        } else if (r.z <= bucket) {
-            // the node's bodies, 64 at a time: one coalesced load by the
-            // lanes into this wave's LDS slot, then the same-address reads
-            // broadcast each body, so the monopoles take VGPR operands (an
-            // SGPR operand halves the VALU issue rate).  Same order as a
-            // body-by-body loop.
-            for (int32_t base = 0; base < r.z; base += 64) {
-                const int32_t k = min(64, r.z - base);
-                if ((int32_t)lane < k) s_leaf[wv][lane] = pos[r.y + base + (int32_t)lane];
-                wave_lds_sync();
-                int32_t j = 0;
-                for (; j + 4 <= k; j += 4) {
-                    const float4 b0 = s_leaf[wv][j], b1 = s_leaf[wv][j + 1], b2 = s_leaf[wv][j + 2],
-                                 b3 = s_leaf[wv][j + 3];
-                    monopole(b0, p, eps2, SOFT, ax, ay, az);
-                    monopole(b1, p, eps2, SOFT, ax, ay, az);
-                    monopole(b2, p, eps2, SOFT, ax, ay, az);
-                    monopole(b3, p, eps2, SOFT, ax, ay, az);
-                }
-                for (; j < k; ++j) monopole(s_leaf[wv][j], p, eps2, SOFT, ax, ay, az);
-                wave_lds_sync();  // every lane has read the slot before it is refilled
-            }
-            cnt += (unsigned)r.z;
-            node = r.x;
-        } else {
-            node = node + 1;
+// Heaviest groups first within each XCD's range (one workgroup per range): a
+// counting sort of the previous walk's per-group interactions into 64 bins.
+// The order within a bin is arbitrary; each group's sums do not depend on it.
+__global__ __launch_bounds__(1024) void k_group_order(int64_t ngroups, const uint32_t *__restrict__ cost,
+                                                      int32_t *__restrict__ order) {
+    __shared__ uint32_t s_max;
+    __shared__ uint32_t s_hist[64], s_base[64];
+    const int64_t lo = ngroups * blockIdx.x / 8, hi = ngroups * (blockIdx.x + 1) / 8;
+    if (threadIdx.x == 0) s_max = 0;
+    if (threadIdx.x < 64) s_hist[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t mx = 0;
+    for (int64_t g = lo + threadIdx.x; g < hi; g += blockDim.x) mx = max(mx, cost[g]);
+    atomicMax(&s_max, mx);
+    __syncthreads();
+    const uint64_t M = (uint64_t)s_max + 1;
+    for (int64_t g = lo + threadIdx.x; g < hi; g += blockDim.x)
+        atomicAdd(&s_hist[63u - (uint32_t)(((uint64_t)cost[g] * 64u) / M)], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int b = 0; b < 64; ++b) {
+            s_base[b] = run;
+            run += s_hist[b];
         }
     }
-    if (live) {
-        accs[s] = ax;
-        accs[n + s] = ay;
-        accs[2 * n + s] = az;
+    __syncthreads();
+    for (int64_t g = lo + threadIdx.x; g < hi; g += blockDim.x) {
+        const uint32_t at = atomicAdd(&s_base[63u - (uint32_t)(((uint64_t)cost[g] * 64u) / M)], 1u);
+        order[lo + at] = (int32_t)g;
     }
-    // cnt is wave-uniform: one atomic per wave
-    if (__lane_id() == (unsigned)__ffsll((unsigned long long)active) - 1)
-        atomicAdd(icount, (unsigned long long)cnt * (unsigned long long)__popcll(active));
+}
+
+__global__ void k_frontier_reset(uint32_t *__restrict__ ctr, int *__restrict__ werr,
+                                 unsigned long long *__restrict__ icount2) {
+    if (threadIdx.x < 8) ctr[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        werr[0] = 0;
+        icount2[0] = 0;
+    }
 }
 
 // ---- 6b. per-lane walk with the reference's per-body MAC, FP64 ----
@@ -1040,6 +1003,7 @@ static void bh_build(nb_system *s) {
         w.first.ensure(t); w.last.ensure(t); w.skip.ensure(t); w.parent.ensure(t); w.body.ensure(t);
         w.depth.ensure(t); w.nchild.ensure(t); w.octant.ensure(t); w.mass.ensure(t); w.com.ensure(3 * t);
         w.side.ensure(t); w.ready.ensure(t); w.wnode.ensure(t);
+        w.wgrp.ensure(t); w.rank.ensure(t); w.cbase.ensure(t); w.cblk.ensure(t / 1024 + 2); w.ctot.ensure(1);
         w.lvl_key.ensure(t); w.lvl_node.ensure(t); w.lvl_off.ensure(kLevels + 1);
         w.sorter.ensure((int64_t)t);
         w.cap_nodes = (int64_t)t;
@@ -1048,7 +1012,14 @@ static void bh_build(nb_system *s) {
     hipLaunchKernelGGL(k_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, logn, w.st.p, w.cnt.p, w.total.p, w.khi.p,
                        w.klo.p, w.idx.p, w.first.p, w.last.p, w.skip.p, w.body.p, w.depth.p, w.octant.p);
     hipLaunchKernelGGL(k_children, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.skip.p, w.body.p, w.parent.p,
-                       w.nchild.p, w.ready.p);
+                       w.nchild.p, w.ready.p, w.rank.p, w.cbase.p);
+    if (s->tune_bh_walk == 1) {
+        // first-child slots of the grouped walk layout (frontier walk only)
+        const int nbn = (int)((total + 1023) / 1024);
+        hipLaunchKernelGGL(k_scan_blocks, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nbn, w.cblk.p, w.ctot.p);
+        hipLaunchKernelGGL(k_scan_add, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
+    }
     hipLaunchKernelGGL(k_side, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.box.p, w.first.p, w.depth.p,
                        w.khi.p, w.klo.p, w.side.p);
     // com is SoA with stride `total`
@@ -1075,18 +1046,47 @@ void bh_forces(nb_system *s, int64_t *interactions) {
     hipStream_t st = s->stream;
     const int32_t total = (int32_t)w.nodes;
     NB_HIP(hipMemsetAsync(w.icount.p, 0, sizeof(unsigned long long), st));
+    bool frontier = false;
     {
         ScopedTimer tm(s->prof, "bh_walk", st);
         if (s->precision == NB_PREC_MIXED) {
             hipLaunchKernelGGL(k_walk_prep, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.mass.p, w.com.p,
-                               w.side.p, w.body.p, w.skip.p, w.first.p, w.last.p, s->G, s->theta, w.wnode.p);
+                               w.side.p, w.body.p, w.skip.p, w.first.p, w.last.p, s->G, s->theta, w.wnode.p,
+                               w.parent.p, w.rank.p, w.nchild.p, w.cbase.p,
+                               s->tune_bh_walk == 1 ? w.wgrp.p : nullptr);
             hipLaunchKernelGGL(k_sorted_pos, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.idx.p, s->x.p, s->m.p, s->G,
                                w.pos32s.p);
             const int bucket = s->tune_bh_bucket > 0 ? s->tune_bh_bucket : w.bucket;
             const float eps2 = s->eps > 0 ? (float)(s->eps * s->eps) : 0.0f;
-            auto kern = s->eps > 0 ? k_walk_wave<true> : k_walk_wave<false>;
-            hipLaunchKernelGGL(kern, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wnode.p, bucket,
-                               eps2, w.accs.p, w.icount.p);
+            auto wave = s->eps > 0 ? k_walk_wave<true> : k_walk_wave<false>;
+            if (s->tune_bh_walk != 1) {  // the frontier walk is switched off (nb_set_tuning refuses 1)
+                hipLaunchKernelGGL(wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wnode.p,
+                                   bucket, eps2, w.accs.p, w.icount.p, nullptr);
+            } else {
+                // persistent grid: eight workgroups per CU, a multiple of 8
+                // so that each XCD gets its own range of groups
+                if (!w.cus) NB_HIP(hipDeviceGetAttribute(&w.cus, hipDeviceAttributeMultiprocessorCount, s->device));
+                const int64_t ngroups = (n + 63) / 64;
+                const int64_t G = std::max<int64_t>(8, std::min<int64_t>((int64_t)blocks(n), (int64_t)8 * w.cus) / 8 * 8);
+                const int32_t cap = s->tune_bh_stack > 0 ? s->tune_bh_stack : kFStackDefault;
+                w.fstack.ensure((size_t)G * kFWaves * (size_t)cap);
+                w.fctr.ensure(8); w.werr.ensure(1); w.icount2.ensure(1);
+                if (w.gcost_n != ngroups) {
+                    w.gcost.ensure((size_t)ngroups);
+                    w.gorder.ensure((size_t)ngroups);
+                    NB_HIP(hipMemsetAsync(w.gcost.p, 0, (size_t)ngroups * sizeof(uint32_t), st));
+                    w.gcost_n = ngroups;
+                }
+                hipLaunchKernelGGL(k_frontier_reset, dim3(1), dim3(64), 0, st, w.fctr.p, w.werr.p, w.icount2.p);
+                hipLaunchKernelGGL(k_group_order, dim3(8), dim3(1024), 0, st, ngroups, w.gcost.p, w.gorder.p);
+                auto front = s->eps > 0 ? k_walk_frontier<true> : k_walk_frontier<false>;
+                hipLaunchKernelGGL(front, dim3((unsigned)G), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wgrp.p,
+                                   bucket, eps2, w.accs.p, w.icount.p, w.fstack.p, cap, w.fctr.p, w.gorder.p,
+                                   w.gcost.p, w.werr.p);
+                hipLaunchKernelGGL(wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wnode.p,
+                                   bucket, eps2, w.accs.p, w.icount2.p, w.werr.p);
+                frontier = true;
+            }
         } else {
             hipLaunchKernelGGL(k_walk_exact, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.cnt.p, w.st.p,
                                s->x.p, w.mass.p, w.com.p, w.side.p, w.body.p, w.skip.p, s->G, s->eps * s->eps,
@@ -1096,10 +1096,16 @@ void bh_forces(nb_system *s, int64_t *interactions) {
     hipLaunchKernelGGL(k_unsort_acc, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.idx.p, w.accs.p, s->a.p);
     NB_HIP(hipGetLastError());
     if (interactions) {
-        unsigned long long c = 0;
+        // after a frontier overflow the gated wave walk's count is the one
+        unsigned long long c = 0, c2 = 0;
+        int over = 0;
         NB_HIP(hipMemcpyAsync(&c, w.icount.p, sizeof(c), hipMemcpyDeviceToHost, st));
+        if (frontier) {
+            NB_HIP(hipMemcpyAsync(&c2, w.icount2.p, sizeof(c2), hipMemcpyDeviceToHost, st));
+            NB_HIP(hipMemcpyAsync(&over, w.werr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        }
         NB_HIP(hipStreamSynchronize(st));
-        *interactions = (int64_t)c;
+        *interactions = (int64_t)(over ? c2 : c);
     }
     s->acc_valid = true;
 }

@@ -234,7 +234,14 @@ int nb_time_elapsed(const nb_system *sys, double *t);
  *   "direct_ipl"   i-bodies per lane of the FP32 direct kernel (1, 2, 4)
  *   "direct_waves" target waves per SIMD for the direct kernel's j split
  *   "bh_bucket"    rejected tree nodes with at most this many bodies are summed
- *                  body by body in the wave walk
+ *                  body by body in the FP32 Barnes-Hut walk
+ *   "bh_walk"      FP32 Barnes-Hut walk: 2 the per-wave depth-first walk (the
+ *                  default); 1 the frontier walk (64 cell tests per step, the
+ *                  same cells accepted), switched off in this build:
+ *                  nb_set_tuning returns NB_ERR_NOT_SUPPORTED for it
+ *   "bh_stack"     node entries of each wave's frontier stack (16..4096); when
+ *                  a group would overflow it, that step's walk is redone
+ *                  depth-first (same cells accepted)
  *   "step_graphs"  1 capture step loops of small systems in HIP graphs, 2 off */
 int nb_set_tuning(nb_system *sys, const char *key, int64_t value);
 int nb_get_tuning(const nb_system *sys, const char *key, int64_t *value);

@@ -1,0 +1,81 @@
+"""The FP32 Barnes-Hut walks' device source (nb_walk_dev.hpp) built for the
+host and run under the wave emulator (tests/emu/walk_emu.cpp): 64 lanes per
+wave in lockstep at every intrinsic, guard pages around every buffer.
+
+Bars, on octrees built from the oracle's tree with the walk-record
+arithmetic of the GPU build:
+  * the depth-first wave walk and the frontier walk accept the same cells:
+    identical interaction counts, for several bucket sizes, with far
+    outliers (64-body groups whose box spans the domain), with eps = 0 and
+    with a ragged last group;
+  * their accelerations differ only by FP32 summation order;
+  * no access leaves a buffer, no intrinsic is reached by part of a wave,
+    and every launch ends;
+  * a frontier stack too small for the tree reports the overflow.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "emu"))
+
+import make_walk_input as M  # noqa: E402
+from gregpackagescpp_amd import capi  # noqa: E402
+
+EXE = os.path.join(HERE, "emu", "_build", "walk_emu")
+SRC = os.path.join(HERE, "emu", "walk_emu.cpp")
+INC = os.path.join(HERE, "..", "gregpackagescpp_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def emu():
+    os.makedirs(os.path.dirname(EXE), exist_ok=True)
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-U_FORTIFY_SOURCE", "-D_FORTIFY_SOURCE=0",
+                           "-ffp-contract=off", "-Wno-unknown-pragmas", "-I", INC, "-o", EXE, SRC])
+    return EXE
+
+
+def run(emu, tmp_path, m, pos, bucket, eps=0.01, theta=0.5, cap=None):
+    inp, out = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    M.write(inp, m, pos, theta, eps, bucket)
+    cmd = [emu, inp, out, "--max-rounds", "200000000"]
+    if cap:
+        cmd += ["--cap", str(cap)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    return M.read_output(out, len(m))
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+
+
+@pytest.mark.parametrize("bucket", [1, 8, 32])
+def test_walks_accept_same_cells(emu, tmp_path, bucket):
+    m, pos, _ = capi.ic_two_plummer(51, 4096)
+    (cw, _, aw), (cf, ov, af) = run(emu, tmp_path, m, pos, bucket)
+    assert ov == 0
+    assert cw == cf and cw > 4096
+    assert rel(af, aw).max() < 1e-4
+
+
+def test_far_outliers_ragged_and_unsoftened(emu, tmp_path):
+    m, pos, _ = capi.ic_plummer(52, 3000)
+    rng = np.random.default_rng(52)
+    far = rng.choice(3000, 30, replace=False)
+    pos[far] = rng.uniform(-500, 500, (30, 3))
+    for eps in (0.01, 0.0):
+        (cw, _, aw), (cf, ov, af) = run(emu, tmp_path, m, pos, 8, eps=eps)
+        assert ov == 0 and cw == cf
+        assert np.all(np.isfinite(af))
+        assert rel(af, aw).max() < 1e-4
+
+
+def test_small_stack_reports_overflow(emu, tmp_path):
+    m, pos, _ = capi.ic_two_plummer(53, 4096)
+    (cw, _, _), (cf, ov, _) = run(emu, tmp_path, m, pos, 1, cap=16)
+    assert ov == 1
