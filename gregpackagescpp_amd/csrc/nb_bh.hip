@@ -83,14 +83,12 @@ struct BHWork {
     DBuf<uint8_t> rank;     // index of a node among its parent's children
     DBuf<uint32_t> cbase;   // exclusive scan of nchild (children of x at 1 + cbase[x])
     DBuf<uint32_t> cblk, ctot;
-    DBuf<int32_t> fstack;   // frontier walk: one node stack per resident wave
-    DBuf<uint32_t> fctr;    // frontier walk: next group of each XCD's range
+    DBuf<int32_t> fstack;   // frontier walk: one node stack per wave
     DBuf<int> werr;         // frontier walk: a stack would have overflowed
     DBuf<uint32_t> gcost;   // interactions of each 64-body group in the last walk
     DBuf<int32_t> gorder;   // groups of each XCD's range, heaviest first
     int64_t gcost_n = 0;
     DBuf<unsigned long long> icount2;  // count of the gated wave walk
-    int cus = 0;
     int bucket = 32;        // rejected nodes with at most this many bodies are summed directly
     DBuf<float4> pos32s;    // sorted body positions (FP32) and G*m
     DBuf<double> accs;      // sorted accelerations, SoA
@@ -838,14 +836,15 @@ __global__ void k_sorted_pos(int64_t n, const uint32_t *__restrict__ idx, const 
     out[s] = make_float4((float)x[b], (float)x[n + b], (float)x[2 * n + b], (float)(G * m[b]));
 }
 
-// Heaviest groups first within each XCD's range (one workgroup per range): a
-// counting sort of the previous walk's per-group interactions into 64 bins.
-// The order within a bin is arbitrary; each group's sums do not depend on it.
-__global__ __launch_bounds__(1024) void k_group_order(int64_t ngroups, const uint32_t *__restrict__ cost,
+// Heaviest groups first within each XCD's range of `span` groups (one
+// workgroup per range): a counting sort of the previous walk's per-group
+// interactions into 64 bins.  The order within a bin is arbitrary; each
+// group's sums do not depend on it.
+__global__ __launch_bounds__(1024) void k_group_order(int64_t ngroups, int64_t span, const uint32_t *__restrict__ cost,
                                                       int32_t *__restrict__ order) {
     __shared__ uint32_t s_max;
     __shared__ uint32_t s_hist[64], s_base[64];
-    const int64_t lo = ngroups * blockIdx.x / 8, hi = ngroups * (blockIdx.x + 1) / 8;
+    const int64_t lo = std::min<int64_t>(ngroups, span * blockIdx.x), hi = std::min<int64_t>(ngroups, span * (blockIdx.x + 1));
     if (threadIdx.x == 0) s_max = 0;
     if (threadIdx.x < 64) s_hist[threadIdx.x] = 0;
     __syncthreads();
@@ -871,9 +870,7 @@ __global__ __launch_bounds__(1024) void k_group_order(int64_t ngroups, const uin
     }
 }
 
-__global__ void k_frontier_reset(uint32_t *__restrict__ ctr, int *__restrict__ werr,
-                                 unsigned long long *__restrict__ icount2) {
-    if (threadIdx.x < 8) ctr[threadIdx.x] = 0;
+__global__ void k_frontier_reset(int *__restrict__ werr, unsigned long long *__restrict__ icount2) {
     if (threadIdx.x == 0) {
         werr[0] = 0;
         icount2[0] = 0;
@@ -1013,7 +1010,7 @@ static void bh_build(nb_system *s) {
                        w.klo.p, w.idx.p, w.first.p, w.last.p, w.skip.p, w.body.p, w.depth.p, w.octant.p);
     hipLaunchKernelGGL(k_children, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.skip.p, w.body.p, w.parent.p,
                        w.nchild.p, w.ready.p, w.rank.p, w.cbase.p);
-    if (s->tune_bh_walk == 1) {
+    if (s->precision == NB_PREC_MIXED && s->tune_bh_walk != 2) {
         // first-child slots of the grouped walk layout (frontier walk only)
         const int nbn = (int)((total + 1023) / 1024);
         hipLaunchKernelGGL(k_scan_blocks, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
@@ -1053,36 +1050,36 @@ void bh_forces(nb_system *s, int64_t *interactions) {
             hipLaunchKernelGGL(k_walk_prep, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.mass.p, w.com.p,
                                w.side.p, w.body.p, w.skip.p, w.first.p, w.last.p, s->G, s->theta, w.wnode.p,
                                w.parent.p, w.rank.p, w.nchild.p, w.cbase.p,
-                               s->tune_bh_walk == 1 ? w.wgrp.p : nullptr);
+                               s->tune_bh_walk != 2 ? w.wgrp.p : nullptr);
             hipLaunchKernelGGL(k_sorted_pos, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.idx.p, s->x.p, s->m.p, s->G,
                                w.pos32s.p);
             const int bucket = s->tune_bh_bucket > 0 ? s->tune_bh_bucket : w.bucket;
             const float eps2 = s->eps > 0 ? (float)(s->eps * s->eps) : 0.0f;
             auto wave = s->eps > 0 ? k_walk_wave<true> : k_walk_wave<false>;
-            if (s->tune_bh_walk != 1) {  // the frontier walk is switched off (nb_set_tuning refuses 1)
+            if (s->tune_bh_walk == 2) {
                 hipLaunchKernelGGL(wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wnode.p,
                                    bucket, eps2, w.accs.p, w.icount.p, nullptr);
             } else {
-                // persistent grid: eight workgroups per CU, a multiple of 8
-                // so that each XCD gets its own range of groups
-                if (!w.cus) NB_HIP(hipDeviceGetAttribute(&w.cus, hipDeviceAttributeMultiprocessorCount, s->device));
+                // one wave per group; a multiple of 8 workgroups so that
+                // each XCD gets one range of `span` groups
                 const int64_t ngroups = (n + 63) / 64;
-                const int64_t G = std::max<int64_t>(8, std::min<int64_t>((int64_t)blocks(n), (int64_t)8 * w.cus) / 8 * 8);
+                const int64_t G = std::max<int64_t>(8, ((ngroups + kFWaves - 1) / kFWaves + 7) / 8 * 8);
+                const int64_t span = G / 8 * kFWaves;
                 const int32_t cap = s->tune_bh_stack > 0 ? s->tune_bh_stack : kFStackDefault;
                 w.fstack.ensure((size_t)G * kFWaves * (size_t)cap);
-                w.fctr.ensure(8); w.werr.ensure(1); w.icount2.ensure(1);
+                w.werr.ensure(1); w.icount2.ensure(1);
                 if (w.gcost_n != ngroups) {
                     w.gcost.ensure((size_t)ngroups);
                     w.gorder.ensure((size_t)ngroups);
                     NB_HIP(hipMemsetAsync(w.gcost.p, 0, (size_t)ngroups * sizeof(uint32_t), st));
                     w.gcost_n = ngroups;
                 }
-                hipLaunchKernelGGL(k_frontier_reset, dim3(1), dim3(64), 0, st, w.fctr.p, w.werr.p, w.icount2.p);
-                hipLaunchKernelGGL(k_group_order, dim3(8), dim3(1024), 0, st, ngroups, w.gcost.p, w.gorder.p);
+                hipLaunchKernelGGL(k_frontier_reset, dim3(1), dim3(64), 0, st, w.werr.p, w.icount2.p);
+                hipLaunchKernelGGL(k_group_order, dim3(8), dim3(1024), 0, st, ngroups, span, w.gcost.p, w.gorder.p);
                 auto front = s->eps > 0 ? k_walk_frontier<true> : k_walk_frontier<false>;
                 hipLaunchKernelGGL(front, dim3((unsigned)G), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wgrp.p,
-                                   bucket, eps2, w.accs.p, w.icount.p, w.fstack.p, cap, w.fctr.p, w.gorder.p,
-                                   w.gcost.p, w.werr.p);
+                                   bucket, eps2, w.accs.p, w.icount.p, w.fstack.p, cap, w.gorder.p, w.gcost.p,
+                                   w.werr.p);
                 hipLaunchKernelGGL(wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wnode.p,
                                    bucket, eps2, w.accs.p, w.icount2.p, w.werr.p);
                 frontier = true;

@@ -53,7 +53,7 @@ struct nb_system {
     int tune_direct_kernel = 0;  // 1: scalar-load j stream, 2: LDS j tiles (default)
     int tune_direct_waves = 0;   // target waves per SIMD for the direct split
     int tune_step_graphs = 0;    // 0: graphs for n <= kGraphMaxBodies, 1: always, 2: never
-    int tune_bh_walk = 0;        // 0/1: frontier walk, 2: per-wave DFS walk
+    int tune_bh_walk = 0;        // 0/1: frontier walk (FP32 path), 2: per-wave DFS walk
     int tune_bh_stack = 0;       // frontier walk stack entries per wave (0: engine default)
 
     // captured direct step loop (kGraphSteps steps) and what it was captured for

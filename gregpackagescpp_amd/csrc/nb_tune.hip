@@ -103,13 +103,12 @@ int nb_set_tuning(nb_system *s, const char *key, int64_t value) {
         s->err = "step_graphs must be 1 (on) or 2 (off)";
         return NB_ERR_INVALID_ARGUMENT;
     }
-    if (!std::strcmp(key, "bh_walk") && value != 0 && value != 2) {
-        s->err = value == 1 ? "bh_walk 1 (frontier walk) is switched off in this build"
-                            : "bh_walk must be 2 (per-wave DFS walk)";
-        return value == 1 ? NB_ERR_NOT_SUPPORTED : NB_ERR_INVALID_ARGUMENT;
+    if (!std::strcmp(key, "bh_walk") && value != 0 && value != 1 && value != 2) {
+        s->err = "bh_walk must be 1 (frontier walk) or 2 (per-wave DFS walk)";
+        return NB_ERR_INVALID_ARGUMENT;
     }
-    if (!std::strcmp(key, "bh_stack") && value != 0 && (value < 16 || value > 4096)) {
-        s->err = "bh_stack must be between 16 and 4096";
+    if (!std::strcmp(key, "bh_stack") && value != 0 && (value < 64 || value > 4096)) {
+        s->err = "bh_stack must be between 64 and 4096";
         return NB_ERR_INVALID_ARGUMENT;
     }
     if (value < 0 || value > 4096) {

@@ -139,10 +139,11 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
 // in FP32 as well, since rounding is monotonic) and otherwise body by body.
 // Accepted cells and bucket ranges go to per-wave LDS lists that every lane
 // then sums with VGPR operands; opened cells push their children, which the
-// grouped record layout keeps adjacent.  Groups are handed out per XCD from
-// a contiguous Morton range, heaviest first by the previous walk's counts.
-// A stack that would overflow sets *werr; the depth-first wave walk then
-// redoes the whole step (its launch follows, gated on that flag).
+// grouped record layout keeps adjacent.  Each XCD takes one contiguous range
+// of the group order, which lists the groups of that range heaviest first by
+// the previous walk's counts.  A stack that would overflow sets *werr; the
+// depth-first wave walk then redoes the whole step (its launch follows,
+// gated on that flag).
 constexpr int kIL = 256;        // accepted cells buffered per wave
 constexpr int kBL = 64;         // bucket ranges buffered per wave
 constexpr int kFWaves = kBlock / 64;
@@ -159,7 +160,7 @@ __device__ __forceinline__ float wave_max(float v) {
 __device__ __forceinline__ int wave_incl_scan(int v, unsigned lane) {
     for (int o = 1; o < 64; o <<= 1) {
         const int t = __shfl_up(v, o);
-        if ((int)lane >= o) v += t;
+        v += (int)lane >= o ? t : 0;
     }
     return v;
 }
@@ -171,33 +172,33 @@ template <bool SOFT>
 __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(6) void k_walk_frontier(
     int64_t n, int32_t total, const float4 *__restrict__ pos, const WalkNode *__restrict__ wgrp, int bucket,
     float eps2, double *__restrict__ accs, unsigned long long *__restrict__ icount, int32_t *__restrict__ stacks,
-    int32_t cap, uint32_t *__restrict__ ctr, const int32_t *__restrict__ order, uint32_t *__restrict__ cost,
-    int *__restrict__ werr) {
+    int32_t cap, const int32_t *__restrict__ order, uint32_t *__restrict__ cost, int *__restrict__ werr) {
 #pragma clang fp contract(fast)
     __shared__ float4 s_il[kFWaves][kIL];
     __shared__ int2 s_bl[kFWaves][kBL];
     __shared__ float4 s_body[kFWaves][64];
     __shared__ float4 s_grp[kFWaves][64];
     const unsigned lane = __lane_id(), wv = threadIdx.x >> 6;
-    const bool split = gridDim.x % 8u == 0u;
-    const uint32_t xcd = split ? blockIdx.x % 8u : 0u;
+    // One wave per group, no loop over groups and no lane-0-only code in any
+    // loop: a group handed out by one lane's atomic and broadcast by
+    // readfirstlane inside a persistent loop was compiled into a loop that
+    // lanes 1-63 never left.  Workgroups are renumbered so that each XCD
+    // takes one contiguous range of the order (heaviest groups first).
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    const uint32_t lb = (g % 8u == 0u) ? (b % 8u) * (g / 8u) + b / 8u : b;
     const int64_t ngroups = (n + 63) / 64;
-    const int64_t g_lo = split ? ngroups * xcd / 8 : 0, g_hi = split ? ngroups * (xcd + 1) / 8 : ngroups;
-    int32_t *const stk = stacks + ((size_t)blockIdx.x * kFWaves + wv) * (size_t)cap;
+    const int64_t gi = (int64_t)lb * kFWaves + wv;
+    if (gi >= ngroups) return;
+    int32_t *const stk = stacks + ((size_t)b * kFWaves + wv) * (size_t)cap;
     const int4 r0 = wgrp[0].r;
     const int32_t cs0 = r0.x, nc0 = min(8, r0.z & 15);
-    for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(&ctr[xcd], 1u);
-        const int64_t gi = g_lo + (int64_t)__builtin_amdgcn_readfirstlane(t);
-        if (gi >= g_hi) break;
-        const int64_t grp = order ? (int64_t)__builtin_amdgcn_readfirstlane(order[gi]) : gi;
+    {
+        const int64_t grp = order ? (int64_t)order[gi] : gi;
         const int64_t s = grp * 64 + lane;
         const bool live = s < n;
         const float4 p = live ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
         const uint64_t active = __ballot(live);
         const int nlive = __popcll(active);
-        wave_lds_sync();  // the previous group's reads of s_grp are done
         s_grp[wv][lane] = p;
         wave_lds_sync();
         const float lox = wave_min(live ? p.x : INFINITY), hix = wave_max(live ? p.x : -INFINITY);
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(6) void k_walk_frontier(
             }
             bl = 0;
         };
-        if ((int32_t)lane < nc0) stk[lane] = cs0 + (int32_t)lane;
+        stk[lane] = cs0 + (int32_t)lane;  // entries past nc0 are never popped (cap >= 64)
         int32_t sp = nc0;
         bool over = false;
         int32_t steps = 0;  // each step pops at least one cell of the tree
@@ -307,16 +308,14 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(6) void k_walk_frontier(
         }
         if (il) flush_il();
         if (bl) flush_bl();
-        if (over && lane == 0) werr[0] = 1;
+        if (over) werr[0] = 1;
         if (live) {
             accs[s] = ax;
             accs[n + s] = ay;
             accs[2 * n + s] = az;
         }
-        if (lane == 0) {
-            atomicAdd(icount, (unsigned long long)cnt * (unsigned long long)nlive);
-            if (cost) cost[grp] = cnt;
-        }
+        if (cost) cost[grp] = cnt;  // every lane stores the same value
+        if (lane == 0) atomicAdd(icount, (unsigned long long)cnt * (unsigned long long)nlive);
     }
 }
 

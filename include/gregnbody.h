@@ -235,11 +235,10 @@ int nb_time_elapsed(const nb_system *sys, double *t);
  *   "direct_waves" target waves per SIMD for the direct kernel's j split
  *   "bh_bucket"    rejected tree nodes with at most this many bodies are summed
  *                  body by body in the FP32 Barnes-Hut walk
- *   "bh_walk"      FP32 Barnes-Hut walk: 2 the per-wave depth-first walk (the
- *                  default); 1 the frontier walk (64 cell tests per step, the
- *                  same cells accepted), switched off in this build:
- *                  nb_set_tuning returns NB_ERR_NOT_SUPPORTED for it
- *   "bh_stack"     node entries of each wave's frontier stack (16..4096); when
+ *   "bh_walk"      FP32 Barnes-Hut walk: 1 the frontier walk (64 cell tests per
+ *                  step, the default), 2 the per-wave depth-first walk; both
+ *                  accept the same cells and differ only in summation order
+ *   "bh_stack"     node entries of each wave's frontier stack (64..4096); when
  *                  a group would overflow it, that step's walk is redone
  *                  depth-first (same cells accepted)
  *   "step_graphs"  1 capture step loops of small systems in HIP graphs, 2 off */

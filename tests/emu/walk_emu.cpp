@@ -10,8 +10,7 @@
 // when one lane ends while others wait (on the GPU, an intrinsic reached by
 // part of a wave).  Global buffers sit against PROT_NONE guard pages, so an
 // access past a buffer's end names the buffer.  Workgroups run one after
-// another, each as one wave (blockDim 64); a persistent kernel's first
-// workgroups therefore take every group of their range.  A step budget per
+// another, each wave of a workgroup in turn.  A step budget per
 // launch (--max-rounds) turns a walk that does not end into an error that
 // names the lane state.
 //
@@ -349,33 +348,33 @@ int main(int argc, char **argv) {
         fwrite(&zero, 8, 1, o);
         fwrite(accs, 8, 3 * (size_t)n, o);
     }
-    // frontier walk: a persistent grid of 8 workgroups per XCD range
+    // frontier walk: one wave per group, workgroups in a multiple of 8
     {
         const int64_t ngroups = (n + 63) / 64;
-        const unsigned G = 8;
+        const unsigned G = (unsigned)std::max<int64_t>(8, ((ngroups + nbe::kFWaves - 1) / nbe::kFWaves + 7) / 8 * 8);
         double *accs = galloc<double>("accs (frontier)", 3 * (size_t)n);
         unsigned long long *icount = galloc<unsigned long long>("icount (frontier)", 1);
         int32_t *stacks = galloc<int32_t>("stacks", (size_t)G * nbe::kFWaves * (size_t)cap);
-        uint32_t *ctr = galloc<uint32_t>("ctr", 8);
         int32_t *order = galloc<int32_t>("order", (size_t)ngroups);
         uint32_t *cost = galloc<uint32_t>("cost", (size_t)ngroups);
         int *werr = galloc<int>("werr", 1);
         for (int64_t g = 0; g < ngroups; ++g) order[g] = (int32_t)g;
         gridDim.x = G;
-        for (unsigned b = 0; b < G; ++b) {
-            blockIdx.x = b;
-            g_block = b;
-            g_lane_base = 0;
-            w.rounds = 0;
-            run_wave(w, [&]() {
-                if (soft)
-                    nbe::k_walk_frontier<true>(n, (int32_t)total, pos, wgrp, (int)bucket, eps2, accs, icount, stacks,
-                                               cap, ctr, order, cost, werr);
-                else
-                    nbe::k_walk_frontier<false>(n, (int32_t)total, pos, wgrp, (int)bucket, eps2, accs, icount,
-                                                stacks, cap, ctr, order, cost, werr);
-            });
-        }
+        for (unsigned b = 0; b < G; ++b)
+            for (unsigned wv = 0; wv < (unsigned)nbe::kFWaves; ++wv) {
+                blockIdx.x = b;
+                g_block = b;
+                g_lane_base = wv * 64;
+                w.rounds = 0;
+                run_wave(w, [&]() {
+                    if (soft)
+                        nbe::k_walk_frontier<true>(n, (int32_t)total, pos, wgrp, (int)bucket, eps2, accs, icount,
+                                                   stacks, cap, order, cost, werr);
+                    else
+                        nbe::k_walk_frontier<false>(n, (int32_t)total, pos, wgrp, (int)bucket, eps2, accs, icount,
+                                                    stacks, cap, order, cost, werr);
+                });
+            }
         printf("frontier walk: interactions %llu overflow %d\n", *icount, *werr);
         const int64_t cnt = (int64_t)*icount, ov = *werr;
         fwrite(&cnt, 8, 1, o);

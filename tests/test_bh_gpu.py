@@ -208,15 +208,14 @@ def test_bh_autotune():
 
 
 def test_walk_knobs():
-    """bh_walk 2 (depth-first) is accepted; the frontier walk (1) is switched
-    off and refused; bh_stack is range-checked."""
+    """bh_walk takes 1 (frontier) or 2 (depth-first); bh_stack is
+    range-checked."""
     m, pos, _ = capi.ic_two_plummer(14, 4096)
     s = system(m, pos, prec=capi.PREC_MIXED)
     s.set_tuning("bh_walk", 2)
     assert s.tuning("bh_walk") == 2
-    with pytest.raises(capi.NBodyError) as e:
-        s.set_tuning("bh_walk", 1)
-    assert e.value.code == capi.NB_ERR_NOT_SUPPORTED
+    with pytest.raises(capi.NBodyError):
+        s.set_tuning("bh_walk", 3)
     with pytest.raises(capi.NBodyError):
         s.set_tuning("bh_stack", 8)
     s.set_tuning("bh_stack", 64)

@@ -77,5 +77,5 @@ def test_far_outliers_ragged_and_unsoftened(emu, tmp_path):
 
 def test_small_stack_reports_overflow(emu, tmp_path):
     m, pos, _ = capi.ic_two_plummer(53, 4096)
-    (cw, _, _), (cf, ov, _) = run(emu, tmp_path, m, pos, 1, cap=16)
+    (cw, _, _), (cf, ov, _) = run(emu, tmp_path, m, pos, 1, cap=64)
     assert ov == 1
