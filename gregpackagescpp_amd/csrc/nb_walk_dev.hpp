@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
 // the previous walk's counts.  A stack that would overflow sets *werr; the
 // depth-first wave walk then redoes the whole step (its launch follows,
 // gated on that flag).
-constexpr int kIL = 256;        // accepted cells buffered per wave
+constexpr int kIL = 128;        // accepted cells buffered per wave (LDS fits 8 waves per SIMD)
 constexpr int kBL = 64;         // bucket ranges buffered per wave
 constexpr int kFWaves = kBlock / 64;
 constexpr int kFStackDefault = 4096;
@@ -169,7 +169,7 @@ __device__ __forceinline__ int lanes_below(uint64_t m, unsigned lane) {
 }
 
 template <bool SOFT>
-__global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(6) void k_walk_frontier(
+__global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
     int64_t n, int32_t total, const float4 *__restrict__ pos, const WalkNode *__restrict__ wgrp, int bucket,
     float eps2, double *__restrict__ accs, unsigned long long *__restrict__ icount, int32_t *__restrict__ stacks,
     int32_t cap, const int32_t *__restrict__ order, uint32_t *__restrict__ cost, int *__restrict__ werr) {
