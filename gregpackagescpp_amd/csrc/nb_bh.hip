@@ -103,6 +103,8 @@ struct BHWork {
     DBuf<uint32_t> bc_cnt, bc_blk, bc_total;  // overlaps per sorted body, then their exclusive scan
     DBuf<int32_t> e_own, e_nb;                // overlaps: owner body, neighbour body
     DBuf<double> e_geo, e_mom;                // distance and normal; axis momenta
+    DBuf<int32_t> c_nb;                       // first kCollCache overlaps per body: neighbour,
+    DBuf<double> c_geo;                       //   distance and normal (from the counting pass)
     DBuf<double> rmax;                        // largest radius
     DBuf<int32_t> cc_par;                     // union-find parents over bodies
     DBuf<uint64_t> cc_keys;                   // (component, entry) sort keys
@@ -472,6 +474,12 @@ __global__ void k_leaf_of(int32_t total, const int32_t *__restrict__ body, const
 // above the rounding of |.| and of the gaps) holds no body that can overlap o, so
 // it is skipped: only non-overlapping leaves drop out of the walk, and the
 // overlaps and their order are those of the full walk.
+// The counting pass also keeps each body's first kCollCache overlaps (in
+// visit order, with the same arithmetic as the writing pass); the writing
+// pass copies them for bodies with at most that many and walks again only
+// for the others.
+constexpr int kCollCache = 4;
+
 template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, const uint32_t *__restrict__ idx,
                                                      const int32_t *__restrict__ leaf_of,
@@ -481,12 +489,25 @@ __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, c
                                                      const double *__restrict__ r, const double *__restrict__ rmax_p,
                                                      uint32_t *__restrict__ cnt, uint32_t nent,
                                                      int32_t *__restrict__ e_own, int32_t *__restrict__ e_nb,
-                                                     double *__restrict__ e_geo) {
+                                                     double *__restrict__ e_geo, int32_t *__restrict__ c_nb,
+                                                     double *__restrict__ c_geo) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
-    if (FILL && (s + 1 < n ? cnt[s + 1] : nent) == cnt[s]) return;  // no overlaps: nothing to write
-    const int32_t own = leaf_of[s];
     const int32_t o = (int32_t)idx[s];
+    if (FILL) {
+        const uint32_t c0 = cnt[s], c1 = s + 1 < n ? cnt[s + 1] : nent;
+        if (c1 == c0) return;  // no overlaps: nothing to write
+        if (c1 - c0 <= (uint32_t)kCollCache) {
+            for (uint32_t q = 0; q < c1 - c0; ++q) {
+                const size_t src = (size_t)s * kCollCache + q, dst = (size_t)c0 + q;
+                e_own[dst] = o;
+                e_nb[dst] = c_nb[src];
+                for (int c = 0; c < 4; ++c) e_geo[4 * dst + c] = c_geo[4 * src + c];
+            }
+            return;
+        }
+    }
+    const int32_t own = leaf_of[s];
     double bb[6];
     for (int k = 0; k < 6; ++k) bb[k] = nbox[(size_t)6 * own + k];
     const double xo = x[o], yo = x[n + o], zo = x[2 * n + o], ro = r[o];
@@ -547,6 +568,13 @@ __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, c
                 e_geo[(size_t)4 * k + 1] = dx / dist;
                 e_geo[(size_t)4 * k + 2] = dy / dist;
                 e_geo[(size_t)4 * k + 3] = dz / dist;
+            } else if (k < (uint32_t)kCollCache) {
+                const size_t c = (size_t)s * kCollCache + k;
+                c_nb[c] = j;
+                c_geo[4 * c + 0] = dist;
+                c_geo[4 * c + 1] = dx / dist;
+                c_geo[4 * c + 2] = dy / dist;
+                c_geo[4 * c + 3] = dz / dist;
             }
             ++k;
         }
@@ -1129,9 +1157,11 @@ static void bh_collision_pass(nb_system *s) {
     hipLaunchKernelGGL(k_leaf_of, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.body.p, w.first.p, w.leaf_of.p);
     NB_HIP(hipMemsetAsync(w.rmax.p, 0, sizeof(double), st));
     hipLaunchKernelGGL(k_rmax, dim3(std::min<unsigned>(1024u, blocks(n))), dim3(kBlock), 0, st, n, s->r.p, w.rmax.p);
+    w.c_nb.ensure((size_t)kCollCache * (size_t)w.cap_n);
+    w.c_geo.ensure((size_t)4 * kCollCache * (size_t)w.cap_n);
     hipLaunchKernelGGL(k_bhc_scan<false>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.leaf_of.p,
                        w.nbox.p, w.skip.p, w.body.p, w.parent.p, s->x.p, s->r.p, w.rmax.p, w.bc_cnt.p, 0u, nullptr, nullptr,
-                       nullptr);
+                       nullptr, w.c_nb.p, w.c_geo.p);
     const int nb = (int)((n + 1023) / 1024);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.bc_cnt.p, w.bc_blk.p);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nb, w.bc_blk.p, w.bc_total.p);
@@ -1146,7 +1176,7 @@ static void bh_collision_pass(nb_system *s) {
     }
     hipLaunchKernelGGL(k_bhc_scan<true>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.leaf_of.p,
                        w.nbox.p, w.skip.p, w.body.p, w.parent.p, s->x.p, s->r.p, w.rmax.p, w.bc_cnt.p, nent, w.e_own.p,
-                       w.e_nb.p, w.e_geo.p);
+                       w.e_nb.p, w.e_geo.p, w.c_nb.p, w.c_geo.p);
     // components of the overlap graph, entries grouped by component in
     // (owner, visit) order, then one lane per component
     w.cc_par.ensure((size_t)w.cap_n);

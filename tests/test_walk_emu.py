@@ -26,17 +26,16 @@ sys.path.insert(0, os.path.join(HERE, "emu"))
 import make_walk_input as M  # noqa: E402
 from gregpackagescpp_amd import capi  # noqa: E402
 
-EXE = os.path.join(HERE, "emu", "_build", "walk_emu")
 SRC = os.path.join(HERE, "emu", "walk_emu.cpp")
 INC = os.path.join(HERE, "..", "gregpackagescpp_amd", "csrc")
 
 
 @pytest.fixture(scope="module")
-def emu():
-    os.makedirs(os.path.dirname(EXE), exist_ok=True)
+def emu(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("emu") / "walk_emu")  # one build per test process
     subprocess.check_call(["g++", "-O2", "-std=c++17", "-U_FORTIFY_SOURCE", "-D_FORTIFY_SOURCE=0",
-                           "-ffp-contract=off", "-Wno-unknown-pragmas", "-I", INC, "-o", EXE, SRC])
-    return EXE
+                           "-ffp-contract=off", "-Wno-unknown-pragmas", "-I", INC, "-o", exe, SRC])
+    return exe
 
 
 def run(emu, tmp_path, m, pos, bucket, eps=0.01, theta=0.5, cap=None):
