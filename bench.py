@@ -113,7 +113,7 @@ def bench_bh(capi, steps):
     s.set_precision(capi.PREC_FP64)
     _, i_alg = s.bh_accelerations()  # reference per-body MAC count on the initial positions
     s.set_precision(capi.PREC_MIXED)
-    s.autotune(capi.LEAPFROG_KDK | capi.BARNES_HUT)  # project autotuner: bucket size of the wave walk
+    s.autotune(capi.LEAPFROG_KDK | capi.BARNES_HUT)  # project autotuner: bucket size of the FP32 walk
     s.set_iterations(1)
     s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)  # warm-up
     t0 = time.perf_counter()
@@ -124,15 +124,15 @@ def bench_bh(capi, steps):
     b_ms, nb = s.profile_read("bh_build")
     w_ms, nw = s.profile_read("bh_walk")
     s.profile(False)
-    _, wave_cnt = s.bh_accelerations()
+    _, walk_cnt = s.bh_accelerations()
     bucket = s.tuning("bh_bucket")
     s.close()
     return {"value": i_alg / t, "unit": "interactions/s", "bodies": BH_BODIES, "theta": BH_THETA,
-            "ms_per_step": t * 1e3, "interactions_per_step": i_alg, "wave_walk_interactions": wave_cnt,
+            "ms_per_step": t * 1e3, "interactions_per_step": i_alg, "walk_interactions": walk_cnt,
             "tuned": {"bh_bucket": bucket},
             "build_ms": b_ms / max(nb, 1), "walk_ms": w_ms / max(nw, 1), "steps": steps,
             "workload": "BH leapfrog KDK, two Plummer spheres of 524288 at x=+-10, vrel=0.5, theta=0.5, "
-                        "eps=0.01, dt=1e-3, FP32 wave walk, FP64 state and node COM"}
+                        "eps=0.01, dt=1e-3, FP32 frontier walk, FP64 state and node COM"}
 
 
 def host_info():

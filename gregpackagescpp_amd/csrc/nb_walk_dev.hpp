@@ -18,7 +18,7 @@ struct __align__(32) WalkNode {
     int4 r;
 };
 
-// ---- 6a. per-wave walk, FP32 monopoles ----
+// ---- per-wave depth-first walk, FP32 monopoles ----
 // The wave shares one node sequence.  A node is accepted when every lane's MAC
 // holds; a rejected node with at most `bucket` bodies is not descended but
 // summed body by body (its bodies are contiguous in sorted order), which
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
         atomicAdd(icount, (unsigned long long)cnt * (unsigned long long)__popcll(active));
 }
 
-// ---- 6c. frontier walk, FP32 monopoles ----
+// ---- frontier walk, FP32 monopoles (the default) ----
 // One wave per 64 Morton-adjacent bodies (a group), as in the wave walk, and
 // the same cells accepted: a cell is taken as a monopole when every body of
 // the group satisfies d^2 > r^2_open, a rejected cell with at most `bucket`
