@@ -10,7 +10,8 @@ monopole only when every body of the group passes the opening test), so:
     fall back to the depth-first walk for the whole step: the result is then
     bit-identical to bh_walk = 2;
   * repeated walks are bit-identical (the heaviest-first group order changes
-    between calls; each group's sums do not depend on it).
+    between calls; each group's sums do not depend on it);
+  * theta = 0 accepts no cell: both walks reduce to the direct sum.
 """
 import numpy as np
 import pytest
@@ -114,3 +115,19 @@ def test_frontier_repeatable_and_evolves():
         out.append(t.state()["vel"])
     dv = np.linalg.norm(out[0] - out[1], axis=1) / np.linalg.norm(out[1] - vel, axis=1)
     assert np.median(dv) < 1e-4
+
+
+def test_frontier_theta_zero_is_direct_sum():
+    """theta = 0 accepts no cell: every body is summed body by body (counts
+    n*n with the own body), whichever walk runs or falls back."""
+    print("case frontier theta=0", flush=True)
+    n = 4096
+    m, pos, _ = capi.ic_plummer(47, n)
+    a, ca = system(m, pos, theta=0.0).bh_accelerations()
+    b, cb = system(m, pos, theta=0.0, walk=2).bh_accelerations()
+    assert ca == cb == n * n
+    d = pos[None, :, :] - pos[:, None, :]
+    r2 = (d * d).sum(2) + 0.01 ** 2
+    ref = (m[None, :, None] * d / (r2 * np.sqrt(r2))[:, :, None]).sum(1)
+    assert rel_diff(a, ref).max() < 1e-4
+    assert rel_diff(b, ref).max() < 1e-4
