@@ -23,6 +23,12 @@
 #include "nb_split64.hpp"
 #include "nb_system.hpp"
 
+// No contraction: the kick and drift recomputed here (v + h a, x + dt v) must
+// round as the forward step's update kernels do, so that recomputed segments
+// land on the forward trajectory bit for bit.  The pair passes below state
+// their fused multiply-adds explicitly.
+#pragma clang fp contract(off)
+
 namespace nbe {
 
 namespace {

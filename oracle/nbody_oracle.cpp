@@ -119,6 +119,31 @@ void direct_acc(int64_t n, int form, T G, T eps, const T *m, const T *pos, T *ac
             pair_acc<T>(form, G, eps2, pos + 3 * i, pos + 3 * j, m[i], m[j], acc + 3 * i, acc + 3 * j);
 }
 
+/* The same sums as direct_acc, one body per thread: body k takes its terms in
+ * the serial loop's order (partners j < k ascending, as the inner body of pair
+ * (j, k), then j > k ascending, as the outer body), so the result is bit for
+ * bit direct_acc's (tests/test_oracle.py checks it).  Used where the oracle
+ * runs long FP64 trajectories (the adjoint). */
+template <typename T>
+void direct_acc_rows(int64_t n, int form, T G, T eps, const T *m, const T *pos, T *acc) {
+    T eps2 = eps * eps;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) {
+        T a[3] = {T(0), T(0), T(0)}, other[3];
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == k) continue;
+            other[0] = other[1] = other[2] = T(0);
+            if (j < k)
+                pair_acc<T>(form, G, eps2, pos + 3 * j, pos + 3 * k, m[j], m[k], other, a);
+            else
+                pair_acc<T>(form, G, eps2, pos + 3 * k, pos + 3 * j, m[k], m[j], a, other);
+        }
+        acc[3 * k] = a[0];
+        acc[3 * k + 1] = a[1];
+        acc[3 * k + 2] = a[2];
+    }
+}
+
 /* Energy: set_ke (gregbod.hpp:917-920) KE = 0.5*m*|v|*|v|; cumulative_pe
  * (gregsys.hpp:358-363) / bh_pe (:376-379) pair PE added to both bodies, then
  * per-body halving and body-order totals (calc_energy :608-630). */
@@ -783,6 +808,11 @@ int orc_direct_acc_ld(int64_t n, int form, const long double *G, const long doub
     direct_acc<long double>(n, form, *G, *eps, m, pos, acc);
     return 0;
 }
+int orc_direct_acc_rows_d(int64_t n, int form, double G, double eps, const double *m, const double *pos,
+                          double *acc) {
+    direct_acc_rows<double>(n, form, G, eps, m, pos, acc);
+    return 0;
+}
 int orc_direct_acc_d(int64_t n, int form, double G, double eps, const double *m, const double *pos, double *acc) {
     direct_acc<double>(n, form, G, eps, m, pos, acc);
     return 0;
@@ -1108,6 +1138,7 @@ int orc_bh_kdk_coll_d(int64_t n, double G, double dt, double eps, double theta, 
  * (gx, gv at time T) and the gradients w.r.t. x0, v0, m come out. */
 static void vjp(int64_t n, double G, double eps2, const double *m, const double *x, const double *u, double *gx,
                 double *gm, double scale) {
+#pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < n; ++k) {
         double ax = 0, ay = 0, az = 0, am = 0;
         for (int64_t j = 0; j < n; ++j) {
@@ -1139,14 +1170,14 @@ int orc_adjoint_d(int64_t n, double G, double dt, double eps, int64_t steps, con
     std::vector<std::vector<double>> xs(steps + 1, std::vector<double>(3 * n));
     std::vector<double> v(v0, v0 + 3 * n), a(3 * n);
     xs[0].assign(x0, x0 + 3 * n);
-    direct_acc<double>(n, FORM_SOFT, G, eps, m, xs[0].data(), a.data());
+    direct_acc_rows<double>(n, FORM_SOFT, G, eps, m, xs[0].data(), a.data());
     for (int64_t s = 0; s < steps; ++s) {
         std::vector<double> &x = xs[s + 1];
         for (int64_t i = 0; i < 3 * n; ++i) {
             v[i] += h * a[i];
             x[i] = xs[s][i] + dt * v[i];
         }
-        direct_acc<double>(n, FORM_SOFT, G, eps, m, x.data(), a.data());
+        direct_acc_rows<double>(n, FORM_SOFT, G, eps, m, x.data(), a.data());
         for (int64_t i = 0; i < 3 * n; ++i) v[i] += h * a[i];
     }
     std::vector<double> lx(gxT, gxT + 3 * n), lv(gvT, gvT + 3 * n);

@@ -40,6 +40,7 @@ def lib():
         L = C.CDLL(LIB)
         L.orc_direct_acc_ld.argtypes = [C.c_int64, C.c_int, _LDP, _LDP, _LDP, _LDP, _LDP]
         L.orc_direct_acc_d.argtypes = [C.c_int64, C.c_int, C.c_double, C.c_double, _D, _D, _D]
+        L.orc_direct_acc_rows_d.argtypes = [C.c_int64, C.c_int, C.c_double, C.c_double, _D, _D, _D]
         L.orc_energy_ld.argtypes = [C.c_int64, C.c_int, _LDP, _LDP, _LDP, _LDP, _LDP, _LDP, _LDP]
         L.orc_energy_d.argtypes = [C.c_int64, C.c_int, C.c_double, C.c_double, _D, _D, _D, _D, _D]
         L.orc_kdk_ld.argtypes = [_I64, C.c_int, _LDP, _LDP, _LDP, C.c_int64, C.c_int, _LDP, _LDP, _LDP, _LDP, _LDP,
@@ -119,6 +120,14 @@ def direct_acc(m, pos, G, eps=0.0, form=FORM_NEWTON, dtype=LD):
         m, pos = _d(m), _d(pos)
         acc = np.zeros((n, 3))
         lib().orc_direct_acc_d(n, form, G, eps, _dp(m), _dp(pos), _dp(acc))
+    return acc
+
+
+def direct_acc_rows(m, pos, G, eps=0.0, form=FORM_NEWTON):
+    """FP64 direct_acc with one body per thread (the adjoint's forward)."""
+    m, pos = _d(m), _d(pos)
+    acc = np.zeros((len(m), 3))
+    lib().orc_direct_acc_rows_d(len(m), form, G, eps, _dp(m), _dp(pos), _dp(acc))
     return acc
 
 

@@ -6,6 +6,8 @@ Bars:
     finite differences in tests/test_oracle.py): every component of dL/dx0,
     dL/dv0, dL/dm within 1e-9 relative (max-norm).
   * checkpoint interval does not change the result (bit-identical).
+  * the 4096-body workload on a 128-step horizon: all three gradients
+    (dL/dm included) within 1e-9 of the oracle adjoint.
   * the 4096-body, 2000-step workload (checkpoints every 64): dL/dx0 and
     dL/dv0 components within 1e-5 (relative to the gradient's max-norm) of
     central differences with h = 1e-7 relative.  (dL/dm is not checked by
@@ -24,9 +26,9 @@ pytestmark = pytest.mark.gpu
 G_SI = 6.6743e-11
 
 
-def sys_fast(m, x, v, dt=10.0, eps=1e6, steps=1):
+def sys_fast(m, x, v, dt=10.0, eps=1e6, steps=1, G=G_SI):
     s = capi.System(0)
-    s.set_G(G_SI)
+    s.set_G(G)
     s.set_timestep(dt)
     s.set_softening(eps)
     s.set_precision(capi.PREC_FP64_FAST)
@@ -55,11 +57,23 @@ def test_adjoint_vs_oracle(n, steps, ckpt):
     assert rel(s.state()["pos"], xT) < 1e-13
 
 
-def test_adjoint_checkpoint_interval_invariant():
-    m, x, v = capi.ic_uniform_ball(22, 300, 1e20, 1e22, 1e9, 1e3)
+@pytest.mark.parametrize("ics", ["si_ball", "unit_plummer"])
+def test_adjoint_checkpoint_interval_invariant(ics):
+    """Recomputed segments must land on the forward trajectory bit for bit,
+    whatever the checkpoint interval.  The unit-scale Plummer case (G = 1,
+    dt = 1e-2, |v| ~ 0.5, |x| ~ 1) makes dt*|v| and h*|a| large against
+    ulp(x) and ulp(v), so a kick or drift rounded differently from the forward
+    step (a fused multiply-add on one side only) would change the gradients."""
+    print(f"case checkpoint invariance {ics}", flush=True)
+    if ics == "si_ball":
+        m, x, v = capi.ic_uniform_ball(22, 300, 1e20, 1e22, 1e9, 1e3)
+        kw = {}
+    else:
+        m, x, v = capi.ic_plummer(22, 300)
+        kw = dict(dt=1e-2, eps=0.05, G=1.0)
     rng = np.random.default_rng(2)
     gxT, gvT = rng.normal(size=(300, 3)), rng.normal(size=(300, 3))
-    outs = [sys_fast(m, x, v).adjoint(50, ck, gxT, gvT)[:3] for ck in (1, 7, 50, 64)]
+    outs = [sys_fast(m, x, v, **kw).adjoint(50, ck, gxT, gvT)[:3] for ck in (1, 7, 50, 64)]
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
             assert np.array_equal(a, b)
@@ -128,6 +142,36 @@ def test_config4_finite_differences():
             worst = max(worst, err)
             print(f"d/d{which}[{i},{k}]: adjoint {g:.10e} fd {fd:.10e} normwise {err:.1e}", flush=True)
     assert worst < 1e-5
+
+
+def test_config4_full_n_vs_oracle_adjoint():
+    """Config 4 at its full size (N = 4096 uniform ball, seed 4, eps = 1e6 m,
+    dt = 10 s, checkpoints every 64) on a shortened horizon of 128 steps (two
+    checkpoint segments): the config's loss, sum |x_T - x*_T|^2 with targets
+    from ICs perturbed by 1e-6 relative Gaussian noise, and all three
+    gradients -- dL/dx0, dL/dv0 and dL/dm -- within 1e-9 (max-norm relative)
+    of the oracle's FP64 reverse-mode adjoint on the same cotangents."""
+    n, steps, dt, eps = 4096, 128, 10.0, 1e6
+    print(f"case config4 full N n={n} steps={steps}", flush=True)
+    m, x, v = capi.ic_uniform_ball(4, n, 1e20, 1e22, 1e9, 1e3)
+    rng = np.random.default_rng(4)
+    xp = x * (1 + 1e-6 * rng.normal(size=x.shape))
+    vp = v * (1 + 1e-6 * rng.normal(size=v.shape))
+    t = sys_fast(m, xp, vp, dt, eps, steps)
+    t.evolve()
+    target = t.state()["pos"]
+    t.close()
+    s = sys_fast(m, x, v, dt, eps, steps)
+    s.evolve()
+    gxT = 2 * (s.state()["pos"] - target)
+    s.close()
+    gvT = np.zeros((n, 3))
+    s = sys_fast(m, x, v, dt, eps, steps)
+    gx, gv, gm, fns, bns = s.adjoint(steps, 64, gxT, gvT)
+    print("case config4 oracle adjoint", flush=True)
+    rx, rv, rm = O.adjoint(m, x, v, G_SI, dt, eps, steps, gxT, gvT)
+    print(f"rel err x {rel(gx, rx):.2e} v {rel(gv, rv):.2e} m {rel(gm, rm):.2e}", flush=True)
+    assert rel(gx, rx) < 1e-9 and rel(gv, rv) < 1e-9 and rel(gm, rm) < 1e-9
 
 
 def test_mass_gradient_vs_oracle_midsize():

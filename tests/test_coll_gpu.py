@@ -9,7 +9,8 @@ Bars:
     bodies survive, merged in the same steps, states within 1e-9 relative;
   * batched ensemble (one workgroup per system, many steps per launch):
     FP64 batch bit-identical to the oracle per system (also over the full
-    20,000-step comet workload for a sample of systems); the mixed (FP32 pair
+    20,000-step comet workload for a stratified sample of 16 systems, n from
+    8 to 512 fragments: merge log, ids, m, r, x, v); the mixed (FP32 pair
     forces) batch reproduces >= 98% of the FP64 batch's (system, outer,
     absorbed) merge records and its merge count within 0.1% -- FP32 forces
     shift the contact step of slow fragment approaches, so merge lists are
@@ -123,21 +124,32 @@ def test_batch_matches_single_system_api():
         k += c
 
 
-def test_comet_ensemble_mixed_merge_list_matches_fp64():
-    """4096 comet-string systems (central body + 8..512 fragments), dt = 5 s,
-    20,000 steps, merging on: the mixed batch's merge list equals the FP64
-    batch's, and a sample of small systems is bit-checked against the oracle."""
+COMET_STEPS = 20000
+
+
+@pytest.fixture(scope="module")
+def comet_runs():
+    """Config 3: 4096 comet-string systems (seed 3; central body + 8..512
+    fragments), dt = 5 s, 20,000 steps, merging on, run by the batch in the
+    mixed precision (the one the benchmark times) and in FP64."""
     offsets, m, r, pos, vel, e = capi.ic_comet_ensemble(3, 4096, 8, 512)
     logs, states = {}, {}
     for prec in (capi.PREC_MIXED, capi.PREC_FP64):
         print(f"case comet ensemble prec={prec}", flush=True)
         b = capi.Batch(offsets, m, r, pos, vel, e)
         b.set_params(G_SI, 5.0, 0.0, capi.OVERLAP_COLL_CHECK, True, 1e10, prec)
-        ns = b.evolve(20000)
+        ns = b.evolve(COMET_STEPS)
         print(f"  {ns / 1e9:.2f} s", flush=True)
         logs[prec] = b.merge_log()
         states[prec] = b.state()
         b.close()
+    return dict(ics=(offsets, m, r, pos, vel, e), logs=logs, states=states)
+
+
+def test_comet_ensemble_mixed_merge_list_matches_fp64(comet_runs):
+    """The mixed batch's merge list against the FP64 batch's: the same events
+    (who absorbs whom) to >= 98% and the same merge count to 0.1%."""
+    logs = comet_runs["logs"]
     a, b = logs[capi.PREC_MIXED], logs[capi.PREC_FP64]
     same = sum(np.array_equal(a[a[:, 0] == s], b[b[:, 0] == s]) for s in range(4096))
     pairs_a = {tuple(r) for r in a[:, [0, 2, 3]]}
@@ -151,17 +163,50 @@ def test_comet_ensemble_mixed_merge_list_matches_fp64():
     # rounding; the events themselves (who absorbs whom) must still agree.
     assert abs(len(a) - len(b)) <= 1e-3 * len(b)
     assert common >= 0.98
-    # oracle on the smallest systems, full 20,000 steps
+
+
+def comet_sample(sizes, k=16):
+    """k systems at evenly spaced ranks of the size order: the smallest, the
+    median region and the largest (n = 512 fragments) are all included."""
+    order = np.argsort(sizes, kind="stable")
+    ranks = np.unique(np.round(np.linspace(0, len(sizes) - 1, k)).astype(int))
+    return [int(order[q]) for q in ranks]
+
+
+def test_comet_ensemble_fp64_bit_exact_vs_oracle(comet_runs):
+    """The FP64 batch against the double restatement over all 20,000 steps,
+    for a stratified sample of 16 systems (smallest to largest n): the full
+    merge log (step, outer, absorbed, merged labels), the surviving ids and
+    every m, r, position and velocity, bit for bit."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    offsets, m, r, pos, vel, e = comet_runs["ics"]
     sizes = np.diff(offsets)
-    fp64 = states[capi.PREC_FP64]
-    starts = np.concatenate([[0], np.cumsum(fp64["counts"])])
-    for s in np.argsort(sizes, kind="stable")[:3]:
+    pick = comet_sample(sizes)
+    assert sizes[pick].max() == sizes.max() and sizes[pick].min() == sizes.min()
+    st = comet_runs["states"][capi.PREC_FP64]
+    log = comet_runs["logs"][capi.PREC_FP64]
+    starts = np.concatenate([[0], np.cumsum(st["counts"])])
+
+    def oracle_run(s):
         a, bnd = offsets[s], offsets[s + 1]
-        ref = O.kdk(m[a:bnd], pos[a:bnd], vel[a:bnd], G_SI, 5.0, 20000, coll=2, r=r[a:bnd], e=e[a:bnd],
-                    dtype=np.float64)
+        return O.kdk(m[a:bnd], pos[a:bnd], vel[a:bnd], G_SI, 5.0, COMET_STEPS, coll=2, r=r[a:bnd], e=e[a:bnd],
+                     dtype=np.float64)
+
+    print(f"case comet fp64 vs oracle systems={pick}", flush=True)
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 4)) as ex:
+        refs = list(ex.map(oracle_run, pick))
+    merges = 0
+    for s, ref in zip(pick, refs):
         k0, k1 = starts[s], starts[s + 1]
+        print(f"  system {s}: n0={sizes[s]} survivors={ref['n']} merges={len(ref['log'])}", flush=True)
         assert k1 - k0 == ref["n"]
-        assert np.array_equal(fp64["pos"][k0:k1], ref["pos"])
+        assert np.array_equal(st["ids"][k0:k1], ref["labels"])
+        for key in ("m", "r", "pos", "vel"):
+            assert np.array_equal(st[key][k0:k1], ref[key]), (s, key)
+        assert np.array_equal(log[log[:, 0] == s][:, 1:], ref["log"]), s
+        merges += len(ref["log"])
+    assert merges > 0
 
 
 @pytest.mark.parametrize("method", [capi.EULER, capi.MODIFIED_EULER, capi.MIDPOINT, capi.LEAPFROG_DKD])

@@ -96,6 +96,58 @@ def test_mixed_vs_long_double_oracle(n, eps):
     assert np.abs((st["pos"] - pos) - dx_ref).max() / np.abs(dx_ref).max() < 1e-5
 
 
+def test_config0_full_run():
+    """The stated config-0 workload: N = 1024 uniform ball (seed 0), masses
+    U[1e20, 1e22] kg, R = 1e9 m, sigma = 1e3 m/s, eps = 1e6 m, dt = 10 s,
+    1000 KDK steps; outputs final positions, velocities and total energy.
+
+    Bars:
+      * FP64 path: positions, velocities, accelerations and the (KE, PE) of
+        the final state bit-identical to the double restatement's;
+      * FP64 path vs the long-double oracle (only the arithmetic type
+        differs): displacement x_T - x_0 within 1e-9 and velocity increment
+        v_T - v_0 within 1e-9 relative (max-norm), energy within 1e-12 of |E|;
+      * mixed path (FP32 pair forces, Kahan sums) vs the long-double oracle:
+        displacement within 1e-5, velocity increment within 1e-5 relative
+        (FP32 terms carry ~1e-7 relative error; 1000 steps add them up), and
+        total energy within 1e-7 of |E| (the velocities are ~1e3 m/s against
+        increments of a few m/s, so a 1e-5 increment error moves KE by
+        ~1e-8 relative)."""
+    n, steps, dt, eps = 1024, 1000, 10.0, 1e6
+    m, pos, vel = capi.ic_uniform_ball(0, n, 1e20, 1e22, 1e9, 1e3)
+    print("case config0 fp64 vs double oracle", flush=True)
+    s = make_system(m, pos, vel, dt, steps, eps)
+    s.evolve(capi.LEAPFROG_KDK)
+    st = s.state()
+    ke, pe = s.energy()
+    ref = O.kdk(m, pos, vel, G_SI, dt, steps, eps=eps, form=O.FORM_SOFT, dtype=np.float64)
+    for k in ("pos", "vel", "acc"):
+        assert np.array_equal(st[k], ref[k]), k
+    rke, rpe = O.energy(m, ref["pos"], ref["vel"], G_SI, eps, O.FORM_SOFT, dtype=np.float64)
+    assert ke == rke and pe == rpe
+    print("case config0 long-double oracle", flush=True)
+    ld = O.kdk(m, pos, vel, G_SI, dt, steps, eps=eps, form=O.FORM_SOFT)
+    lke, lpe = O.energy(m, ld["pos"], ld["vel"], G_SI, eps, O.FORM_SOFT)
+    E_ld = float(lke + lpe)
+    dx_ref = (ld["pos"] - pos.astype(LD)).astype(np.float64)
+    dv_ref = (ld["vel"] - vel.astype(LD)).astype(np.float64)
+
+    def errs(state, E):
+        ex = np.abs((state["pos"] - pos) - dx_ref).max() / np.abs(dx_ref).max()
+        ev = np.abs((state["vel"] - vel) - dv_ref).max() / np.abs(dv_ref).max()
+        return ex, ev, abs(E - E_ld) / abs(E_ld)
+
+    ex, ev, ee = errs(st, ke + pe)
+    print(f"fp64 vs long double: x {ex:.2e} v {ev:.2e} E {ee:.2e}", flush=True)
+    assert ex < 1e-9 and ev < 1e-9 and ee < 1e-12
+    print("case config0 mixed", flush=True)
+    s = make_system(m, pos, vel, dt, steps, eps, prec=capi.PREC_MIXED)
+    s.evolve(capi.LEAPFROG_KDK)
+    ex, ev, ee = errs(s.state(), sum(s.energy()))
+    print(f"mixed vs long double: x {ex:.2e} v {ev:.2e} E {ee:.2e}", flush=True)
+    assert ex < 1e-5 and ev < 1e-5 and ee < 1e-7
+
+
 def test_advance_continues_evolve():
     m, pos, vel = capi.ic_uniform_ball(3, 100, 1e20, 1e22, 1e9, 1e3)
     a = make_system(m, pos, vel, 10.0, 20, 1e6)

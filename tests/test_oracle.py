@@ -152,6 +152,16 @@ def test_newton_third_law():
         assert np.abs(f).max() / scale < 1e-17
 
 
+def test_row_parallel_direct_sum_is_bit_identical():
+    """The adjoint's forward takes each body's pair terms on one thread in the
+    serial i < j loop's order, so its sums equal direct_acc's bit for bit
+    (both pair forms, several thread counts via OpenMP's default)."""
+    m, pos, _ = capi.ic_uniform_ball(9, 301, 1e20, 1e22, 1e9, 1e3)
+    for form, eps in ((O.FORM_NEWTON, 0.0), (O.FORM_SOFT, 1e6)):
+        ref = O.direct_acc(m, pos, G_SI, eps, form, dtype=np.float64)
+        assert np.array_equal(O.direct_acc_rows(m, pos, G_SI, eps, form), ref)
+
+
 def test_brute_force_pair_sum():
     """Per-body sums against an independent numpy long-double formulation."""
     m, pos, vel = capi.ic_uniform_ball(2, 8, 1e20, 1e22, 1e9, 1e3)
@@ -173,7 +183,9 @@ def test_softened_limit_equals_newton_form():
     a0 = O.direct_acc(m, pos, G_SI, 0.0, O.FORM_NEWTON)
     a1 = O.direct_acc(m, pos, G_SI, 1e-3, O.FORM_SOFT)  # eps^2 negligible vs r^2 ~ 1e17
     assert np.abs(a0 - a1).max() / np.abs(a0).max() < 1e-16
-    assert not np.array_equal(a0, a1) or True
+    # the two forms round differently (r^3 as |(d.d) d| vs (m/s)(d/sqrt s)):
+    # equal to 1e-16 but not the same bits, so the form switch is live
+    assert not np.array_equal(a0, a1)
 
 
 def test_kdk_time_reversible():
