@@ -79,6 +79,7 @@ SIGNATURES = {
     "nb_set_precision": (C.c_int, [_P, C.c_int]),
     "nb_evolve": (C.c_int, [_P, C.c_int, _I64]),
     "nb_advance": (C.c_int, [_P, C.c_uint64, _I64]),
+    "nb_advance_noalloc": (C.c_int, [_P, C.c_uint64, _I64]),
     "nb_energy": (C.c_int, [_P, _D, _D]),
     "nb_merge_log": (C.c_int, [_P, C.c_int64, _I64, _I64]),
     "nb_set_recording": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_int]),
@@ -257,6 +258,13 @@ class System:
     def advance(self, steps) -> int:
         ns = C.c_int64()
         self._ck(self._L.nb_advance(self.h, steps, C.byref(ns)))
+        return ns.value
+
+    def advance_noalloc(self, steps) -> int:
+        """nb_advance with the force pass's split partials in the library's
+        static device scratch (allocates nothing)."""
+        ns = C.c_int64()
+        self._ck(self._L.nb_advance_noalloc(self.h, steps, C.byref(ns)))
         return ns.value
 
     def energy(self):

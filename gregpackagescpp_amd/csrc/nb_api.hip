@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 
 #include "nb_system.hpp"
 
@@ -23,6 +24,33 @@ nb_system::~nb_system() {
 namespace {
 
 thread_local std::string g_static_err;
+
+// Device scratch of nb_advance_noalloc: the split partials of the direct
+// force pass live in this file-scope array instead of context buffers, so
+// that entry point allocates nothing.  Each device holds its own copy until
+// the process ends.  In host code `scratch` is only the symbol's host-side
+// stand-in: its device address comes from hipGetSymbolAddress, resolved once
+// per device at nb_create (with that device current) and kept in
+// g_scratch_ptr; host code passes that pointer, never the array's name.
+__device__ __align__(256) unsigned char scratch[NB_NOALLOC_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;  // per device index
+
+void resolve_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (g_scratch_ptr[(size_t)device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        throw DeviceError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[(size_t)device] = p;
+}
+
+void *scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[(size_t)device] : nullptr;
+}
 
 }  // namespace
 
@@ -236,46 +264,55 @@ GraphKey graph_key(const nb_system *s) {
     return k;
 }
 
+// One direct KDK step's launches from valid accelerations.  `open`: the
+// opening half kick and drift are still due; `last`: end with the closing
+// half kick instead of fusing it with the next step's opening kick and drift.
+// `partial` / `partial64` receive the split partials of the mixed / FP64-fast
+// force pass.  Returns whether the next step must open.
+bool kdk_step(nb_system *s, bool open, bool last, float *partial, double *partial64) {
+    const bool mixed = s->precision == NB_PREC_MIXED;
+    const bool fast64 = s->precision == NB_PREC_FP64_FAST;
+    const int64_t n = s->n;
+    if (open) {
+        launch_kick_drift(s->stream, n, s->a.p, s->v.p, s->x.p, s->half_dt, s->dt);
+        if (mixed) launch_pack(s->stream, n, s->x.p, s->m.p, s->G, s->dw.pos32.p);
+    }
+    if (fast64) {
+        {
+            ScopedTimer t(s->prof, "force", s->stream);
+            launch_force_fp64_split(s->stream, n, s->x.p, s->m.p, s->eps * s->eps, form_of(s) == 1, partial64,
+                                    s->dw.nsplit64);
+        }
+        launch_update_fp64_split(s->stream, n, partial64, s->dw.nsplit64, s->G, s->a.p, s->v.p, s->x.p, s->half_dt,
+                                 s->dt, last ? UPD_CLOSE : UPD_CLOSE_OPEN);
+        return last;
+    }
+    if (mixed) {
+        {
+            ScopedTimer t(s->prof, "force", s->stream);
+            launch_force_mixed(s->stream, n, s->dw.pos32.p, (float)(s->eps * s->eps), form_of(s) == 1, partial,
+                               s->dw.nsplit, s->dw.ipl, s->dw.variant);
+        }
+        ScopedTimer t(s->prof, "update", s->stream);
+        launch_update_mixed(s->stream, n, partial, s->dw.nsplit, s->a.p, s->v.p, s->x.p, s->m.p, s->G, s->half_dt,
+                            s->dt, last ? UPD_CLOSE : UPD_CLOSE_OPEN, s->dw.pos32.p);
+        return last;
+    }
+    {
+        ScopedTimer t(s->prof, "force", s->stream);
+        launch_force_fp64(s->stream, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->a.p,
+                          s->dw.terms.n >= (size_t)(fp64_terms_block(n) * 3 * n) ? s->dw.terms.p : nullptr);
+    }
+    launch_kick(s->stream, n, s->a.p, s->v.p, s->half_dt);
+    return true;
+}
+
 void direct_kdk_steps(nb_system *s, uint64_t steps) {
     const bool mixed = s->precision == NB_PREC_MIXED;
     const bool fast64 = s->precision == NB_PREC_FP64_FAST;
     const bool coll = s->coll_mode == NB_OVERLAP_COLL_CHECK;
-    // one step's launches; `open` = the opening half kick and drift are still
-    // due; returns whether the next step must open
     auto issue_step = [&](bool open, bool last) -> bool {
-        const int64_t n = s->n;
-        if (open) {
-            launch_kick_drift(s->stream, n, s->a.p, s->v.p, s->x.p, s->half_dt, s->dt);
-            if (mixed) launch_pack(s->stream, n, s->x.p, s->m.p, s->G, s->dw.pos32.p);
-        }
-        if (fast64) {
-            {
-                ScopedTimer t(s->prof, "force", s->stream);
-                launch_force_fp64_split(s->stream, n, s->x.p, s->m.p, s->eps * s->eps, form_of(s) == 1,
-                                        s->dw.partial64.p, s->dw.nsplit64);
-            }
-            launch_update_fp64_split(s->stream, n, s->dw.partial64.p, s->dw.nsplit64, s->G, s->a.p, s->v.p, s->x.p,
-                                     s->half_dt, s->dt, last ? UPD_CLOSE : UPD_CLOSE_OPEN);
-            return last;
-        }
-        if (mixed) {
-            {
-                ScopedTimer t(s->prof, "force", s->stream);
-                launch_force_mixed(s->stream, n, s->dw.pos32.p, (float)(s->eps * s->eps), form_of(s) == 1,
-                                   s->dw.partial.p, s->dw.nsplit, s->dw.ipl, s->dw.variant);
-            }
-            ScopedTimer t(s->prof, "update", s->stream);
-            launch_update_mixed(s->stream, n, s->dw.partial.p, s->dw.nsplit, s->a.p, s->v.p, s->x.p, s->m.p, s->G,
-                                s->half_dt, s->dt, last ? UPD_CLOSE : UPD_CLOSE_OPEN, s->dw.pos32.p);
-            return last;
-        }
-        {
-            ScopedTimer t(s->prof, "force", s->stream);
-            launch_force_fp64(s->stream, n, s->x.p, s->m.p, s->G, s->eps * s->eps, form_of(s), s->a.p,
-                              s->dw.terms.n >= (size_t)(fp64_terms_block(n) * 3 * n) ? s->dw.terms.p : nullptr);
-        }
-        launch_kick(s->stream, n, s->a.p, s->v.p, s->half_dt);
-        return true;
+        return kdk_step(s, open, last, s->dw.partial.p, s->dw.partial64.p);
     };
     // Interior steps of small systems replay a captured graph of kGraphSteps
     // steps (no collisions, no per-launch timing).
@@ -341,6 +378,12 @@ int nb_create(int device, nb_system **out) {
         if (e != hipSuccess) {
             delete s;
             throw DeviceError(std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+        }
+        try {
+            resolve_scratch(device);
+        } catch (...) {
+            delete s;
+            throw;
         }
         long double G;
         parse_units("M1:1,D1:1,T1:1", &G, s->G_vals);
@@ -611,6 +654,46 @@ int nb_advance(nb_system *s, uint64_t steps, int64_t *elapsed_ns) {
         if (other) direct_other_steps(s, s->acc_method, steps);
         else if (s->acc_method == NB_BARNES_HUT) bh_steps(s, steps);
         else direct_kdk_steps(s, steps);
+        s->time_elapsed += (long double)steps * (long double)s->dt;
+        NB_HIP(hipStreamSynchronize(s->stream));
+        s->prof.collect();
+        auto t1 = std::chrono::high_resolution_clock::now();
+        if (elapsed_ns) *elapsed_ns = (int64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+        return NB_OK;
+    });
+}
+
+int nb_advance_noalloc(nb_system *s, uint64_t steps, int64_t *elapsed_ns) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
+    if (elapsed_ns) *elapsed_ns = 0;
+    return guarded(s, [&]() {
+        if (!s->evolved || !s->acc_valid || s->acc_method != NB_LEAPFROG_KDK)
+            throw ApiError(NB_ERR_NO_EVOLUTION,
+                           "nb_advance_noalloc needs a previous direct-sum NB_LEAPFROG_KDK nb_evolve with unchanged "
+                           "parameters");
+        const bool mixed = s->precision == NB_PREC_MIXED;
+        if (!mixed && s->precision != NB_PREC_FP64_FAST)
+            throw ApiError(NB_ERR_NOT_SUPPORTED, "nb_advance_noalloc: precision must be NB_PREC_MIXED or "
+                                                 "NB_PREC_FP64_FAST");
+        if (s->coll_mode != NB_NO_COLL_CHECK || s->rec_every)
+            throw ApiError(NB_ERR_NOT_SUPPORTED, "nb_advance_noalloc: collisions and recording must be off");
+        const int64_t n = s->n;
+        const size_t need = mixed ? (size_t)s->dw.nsplit * 6 * (size_t)n * sizeof(float)
+                                  : (size_t)s->dw.nsplit64 * 3 * (size_t)n * sizeof(double);
+        if (need > (size_t)NB_NOALLOC_SCRATCH_BYTES)
+            throw ApiError(NB_ERR_CAPACITY, "nb_advance_noalloc: the split partials need " + std::to_string(need) +
+                                                " bytes, more than the " +
+                                                std::to_string((size_t)NB_NOALLOC_SCRATCH_BYTES) +
+                                                "-byte static scratch (use nb_advance)");
+        if (mixed && s->dw.pos32.n < (size_t)n)
+            throw ApiError(NB_ERR_NO_EVOLUTION, "nb_advance_noalloc: no packed positions for this body count");
+        void *p = scratch_ptr(s->device);
+        if (!p) throw DeviceError("nb_advance_noalloc: device scratch address not resolved");
+        auto t0 = std::chrono::high_resolution_clock::now();
+        bool open = true;
+        for (uint64_t st = 1; st <= steps && n > 0; ++st)
+            open = kdk_step(s, open, st == steps, mixed ? (float *)p : nullptr, mixed ? nullptr : (double *)p);
+        s->step_count += steps;
         s->time_elapsed += (long double)steps * (long double)s->dt;
         NB_HIP(hipStreamSynchronize(s->stream));
         s->prof.collect();

@@ -125,6 +125,22 @@ int nb_evolve(nb_system *sys, int method, int64_t *elapsed_ns);
  * accelerations are those of the current positions).  Returns
  * NB_ERR_NO_EVOLUTION if nothing was evolved. */
 int nb_advance(nb_system *sys, uint64_t steps, int64_t *elapsed_ns);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Continues a direct-sum NB_LEAPFROG_KDK evolve exactly as
+ * nb_advance does (same kernels, same split counts, bit-identical states), in
+ * NB_PREC_MIXED or NB_PREC_FP64_FAST, with collisions and recording off; the
+ * force pass's split partials live in a static device array of
+ * NB_NOALLOC_SCRATCH_BYTES per GPU instead of context buffers, and the call
+ * makes no hipMalloc / hipFree and captures no graph.  NB_ERR_CAPACITY when
+ * the partials (6 floats or 3 doubles per body per split) exceed the array,
+ * NB_ERR_NO_EVOLUTION without a previous direct KDK nb_evolve, and
+ * NB_ERR_NOT_SUPPORTED for other precisions or with collisions or recording.
+ * Every call on one GPU shares that GPU's array: calls that overlap in time
+ * on one GPU (contexts on several host threads) race and must be serialised
+ * by the caller. */
+#define NB_NOALLOC_SCRATCH_BYTES (32u << 20)
+int nb_advance_noalloc(nb_system *sys, uint64_t steps, int64_t *elapsed_ns);
 /* Total kinetic and potential energy of the current state in FP64, reference
  * formulae (set_ke gregbod.hpp:917-920, cumulative_pe gregsys.hpp:358-363,
  * calc_energy :608-630; bh_pe's softened form when eps > 0). */
