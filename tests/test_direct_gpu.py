@@ -105,14 +105,15 @@ def test_config0_full_run():
       * FP64 path: positions, velocities, accelerations and the (KE, PE) of
         the final state bit-identical to the double restatement's;
       * FP64 path vs the long-double oracle (only the arithmetic type
-        differs): displacement x_T - x_0 within 1e-9 and velocity increment
-        v_T - v_0 within 1e-9 relative (max-norm), energy within 1e-12 of |E|;
+        differs): displacement x_T - x_0 and velocity increment v_T - v_0
+        within 1e-11 relative (max-norm), energy within 1e-14 of |E|
+        (measured on MI355X: 1.1e-13, 3.0e-13, 1.7e-16);
       * mixed path (FP32 pair forces, Kahan sums) vs the long-double oracle:
-        displacement within 1e-5, velocity increment within 1e-5 relative
+        displacement within 1e-7, velocity increment within 1e-5 relative
         (FP32 terms carry ~1e-7 relative error; 1000 steps add them up), and
-        total energy within 1e-7 of |E| (the velocities are ~1e3 m/s against
-        increments of a few m/s, so a 1e-5 increment error moves KE by
-        ~1e-8 relative)."""
+        total energy within 1e-9 of |E| (the velocities are ~1e3 m/s against
+        increments of a few m/s, so increment errors move KE very little;
+        measured: 6.9e-9, 4.4e-7, 2.1e-11)."""
     n, steps, dt, eps = 1024, 1000, 10.0, 1e6
     m, pos, vel = capi.ic_uniform_ball(0, n, 1e20, 1e22, 1e9, 1e3)
     print("case config0 fp64 vs double oracle", flush=True)
@@ -139,13 +140,13 @@ def test_config0_full_run():
 
     ex, ev, ee = errs(st, ke + pe)
     print(f"fp64 vs long double: x {ex:.2e} v {ev:.2e} E {ee:.2e}", flush=True)
-    assert ex < 1e-9 and ev < 1e-9 and ee < 1e-12
+    assert ex < 1e-11 and ev < 1e-11 and ee < 1e-14
     print("case config0 mixed", flush=True)
     s = make_system(m, pos, vel, dt, steps, eps, prec=capi.PREC_MIXED)
     s.evolve(capi.LEAPFROG_KDK)
     ex, ev, ee = errs(s.state(), sum(s.energy()))
     print(f"mixed vs long double: x {ex:.2e} v {ev:.2e} E {ee:.2e}", flush=True)
-    assert ex < 1e-5 and ev < 1e-5 and ee < 1e-7
+    assert ex < 1e-7 and ev < 1e-5 and ee < 1e-9
 
 
 def test_advance_continues_evolve():

@@ -349,6 +349,38 @@ def test_merge_conserves_mass_and_momentum():
 
 
 # ---------------- adjoint ----------------
+def test_comet_merge_list_double_equals_long_double():
+    """Config 3 (comet ensemble, seed 3, dt = 5 s, merging on, 20,000 steps):
+    on the ensemble's smaller systems the double restatement merges the same
+    bodies in the same steps as the long-double one (the reference's
+    arithmetic type), with the same survivors.  So the FP64 path, which
+    equals the double restatement bit for bit, reproduces the long-double
+    merge list on them."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    G = float(LD(66743) / LD(10 ** 15))
+    offsets, m, r, pos, vel, e = capi.ic_comet_ensemble(3, 4096, 8, 512)
+    picks = [209, 523, 2311, 165, 78]  # 9, 42, 75, 108 and 209 bodies (129 merges in the last)
+
+    def run(args):
+        s, dtype = args
+        a, b = offsets[s], offsets[s + 1]
+        return O.kdk(m[a:b], pos[a:b], vel[a:b], G, 5.0, 20000, coll=2, r=r[a:b], e=e[a:b], dtype=dtype)
+
+    jobs = [(s, t) for s in picks for t in (np.float64, LD)]
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 2)) as ex:
+        res = list(ex.map(run, jobs))
+    merges = 0
+    for k in range(len(picks)):
+        d, ld = res[2 * k], res[2 * k + 1]
+        assert d["n"] == ld["n"]
+        assert np.array_equal(d["log"], ld["log"])
+        assert np.array_equal(d["labels"], ld["labels"])
+        assert np.abs(d["pos"] - ld["pos"].astype(np.float64)).max() < 1e-9 * np.abs(ld["pos"]).max()
+        merges += len(d["log"])
+    assert merges >= 150
+
+
 def test_adjoint_vs_central_differences():
     rng = np.random.default_rng(3)
     n, steps, dt, eps = 4, 30, 0.01, 0.1
