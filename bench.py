@@ -15,7 +15,9 @@ N(N-1)*K / t over the K timed steps.  The JSON line also carries:
   bh           Barnes-Hut KDK at 1,048,576 bodies (I_alg interactions/s), with
                its own single-thread and all-core CPU baselines;
   adjoint      backward/forward wall-time ratio of the adjoint at N = 4096;
-  batch        4096-system comet ensemble with merging (interactions/s).
+  batch        4096-system comet ensemble with merging (interactions/s), the
+               mixed batch and, beside it, the FP64 batch (bit-exact merge
+               list), each with its kernel roofline.
 """
 import argparse
 import json
@@ -49,14 +51,21 @@ PROFILED_FORCE_TRAFFIC = {
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
 
 
-def cpu_baseline(m, pos, vel):
-    """Time the reference CPU path on a bounded sample of the same workload."""
+def cpu_baseline(m, pos, vel, allow_port=False):
+    """Time the reference CPU path on a bounded sample of the same workload:
+    the compiled reference (oracle/_ref, built from the reference's sources
+    beside this repository).  Without it the run stops, unless allow_port
+    asks for the oracle's restatement instead (labelled kind "port")."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     ns_sample, steps = 8192, 2
     mm, pp, vv = m[:ns_sample].copy(), pos[:ns_sample].copy(), vel[:ns_sample].copy()
     pairs_per_pass = ns_sample * (ns_sample - 1)
+    if not O.ref_available() and not allow_port:
+        raise SystemExit("bench.py: oracle/_ref/ref_harness (the compiled reference) is missing; build it with "
+                         "__graft_entry__.build() where the reference sources are mounted, or pass "
+                         "--cpu-baseline-port to time the oracle's restatement instead")
     if O.ref_available():
         with tempfile.TemporaryDirectory() as td:
             inp, out = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
@@ -98,11 +107,31 @@ BH_BODIES = 1 << 20
 BH_THETA = 0.5
 
 
+RADIX_PASSES = 8  # 8-bit LSD passes over the 63-bit high key of the octree sort
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 summed over the build kernels
+# of one BH step at 1,048,576 bodies (profiles/bh_walk_build_two_plummer_1M.md),
+# and the same for the frontier walk launch; None when not profiled.
+PROFILED_BH_BUILD_BYTES = None
+PROFILED_BH_WALK_BYTES = None
+
+
+def build_bytes(n, nodes, passes=RADIX_PASSES):
+    """Algorithmic HBM bytes of one octree build: per body 32 B read (x, y,
+    z, m) and 12 B written (key, index); per radix pass 24 B per body (key
+    and index read and written); 48 B per node written; 32 B per body for
+    the reorder of the bodies into tree order."""
+    return n * (32 + 12) + passes * n * 24 + nodes * 48 + n * 32
+
+
 def bench_bh(capi, steps):
     """Barnes-Hut KDK on two 524,288-body Plummer spheres (theta = 0.5, eps =
     0.01, dt = 1e-3, full rebuild every step).  Interactions are I_alg: the
     cells accepted under the reference's per-body MAC, counted by the FP64
-    per-lane walk on the same positions."""
+    per-lane walk on the same positions.  Rooflines: the build against HBM
+    (algorithmic bytes over its HIP-event time), the walk against FP32 VALU
+    at 20 FLOP per I_alg interaction."""
     m, pos, vel = capi.ic_two_plummer(2, BH_BODIES)
     s = capi.System(0)
     s.set_G(1.0)
@@ -125,12 +154,24 @@ def bench_bh(capi, steps):
     w_ms, nw = s.profile_read("bh_walk")
     s.profile(False)
     _, walk_cnt = s.bh_accelerations()
+    nodes = s.tree_count()
     bucket = s.tuning("bh_bucket")
     s.close()
+    build_ms, walk_ms = b_ms / max(nb, 1), w_ms / max(nw, 1)
+    bb = build_bytes(BH_BODIES, nodes)
+    build_gbs = bb / (build_ms * 1e-3) / 1e9
+    walk_tf = FLOP_PER_INTERACTION * i_alg / (walk_ms * 1e-3) / 1e12
     return {"value": i_alg / t, "unit": "interactions/s", "bodies": BH_BODIES, "theta": BH_THETA,
             "ms_per_step": t * 1e3, "interactions_per_step": i_alg, "walk_interactions": walk_cnt,
-            "tuned": {"bh_bucket": bucket},
-            "build_ms": b_ms / max(nb, 1), "walk_ms": w_ms / max(nw, 1), "steps": steps,
+            "tuned": {"bh_bucket": bucket}, "nodes": nodes,
+            "build_ms": build_ms, "walk_ms": walk_ms, "steps": steps,
+            "build_roofline": {"bound": "hbm", "achieved": build_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                               "frac": build_gbs / PEAK_HBM_GBS, "algorithmic_bytes": bb,
+                               "radix_passes": RADIX_PASSES, "traffic": PROFILED_BH_BUILD_BYTES},
+            "walk_roofline": {"bound": "valu", "achieved": walk_tf, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                              "frac": walk_tf / PEAK_FP32_TFLOPS, "flop_per_interaction": FLOP_PER_INTERACTION,
+                              "evaluated_tflops": FLOP_PER_INTERACTION * walk_cnt / (walk_ms * 1e-3) / 1e12,
+                              "traffic": PROFILED_BH_WALK_BYTES},
             "workload": "BH leapfrog KDK, two Plummer spheres of 524288 at x=+-10, vrel=0.5, theta=0.5, "
                         "eps=0.01, dt=1e-3, FP32 frontier walk, FP64 state and node COM"}
 
@@ -275,19 +316,26 @@ def cpu_bh_allcore(m, pos):
                       f"full {len(m)}-body two-Plummer IC, one force evaluation ({cnt} interactions, {t:.1f} s)"}
 
 
-def bench_batch(capi, steps):
+def bench_batch(capi, steps, precision=None):
     """Batched ensemble: 4096 comet-string systems (central body of 1.9e27 kg
-    plus n ~ U{8..512} fragments, seed 3), dt = 5 s, merging on, mixed
-    precision, one persistent launch per 1000 steps.  Interactions are the
-    force pairs actually evaluated, sum over steps and systems of n(n-1) with
-    n reduced by each merge from the step after it; the collision scan's
-    n(n-1)/2 overlap tests per step are reported beside them."""
+    plus n ~ U{8..512} fragments, seed 3), dt = 5 s, merging on, one
+    persistent launch per 1000 steps; mixed precision (FP32 pair forces) by
+    default, or FP64 (the precision whose merge list is bit-exact against
+    the double restatement).  Interactions are the force pairs actually
+    evaluated, sum over steps and systems of n(n-1) with n reduced by each
+    merge from the step after it; the collision scan's n(n-1)/2 overlap tests
+    per step are reported beside them.  The roofline takes the stepping
+    launches' HIP-event time, at 20 FLOP per interaction, against the FP32
+    (mixed) or FP64 VALU peak."""
     G_SI = 6.6743e-11
+    if precision is None:
+        precision = capi.PREC_MIXED
     offsets, m, r, pos, vel, e = capi.ic_comet_ensemble(3, 4096, 8, 512)
     sizes = np.diff(offsets).astype(np.int64)
     b = capi.Batch(offsets, m, r, pos, vel, e)
-    b.set_params(G_SI, 5.0, 0.0, capi.OVERLAP_COLL_CHECK, True, 1e10, capi.PREC_MIXED)
+    b.set_params(G_SI, 5.0, 0.0, capi.OVERLAP_COLL_CHECK, True, 1e10, precision)
     ns = b.evolve(steps)
+    k_ms, k_launches = b.kernel_time()
     log = b.merge_log()
     b.close()
     t = ns * 1e-9
@@ -306,12 +354,19 @@ def bench_batch(capi, steps):
             span = int(bounds[k + 1] - bounds[k])
             inter += span * nk * (nk - 1)
             tests += span * nk * (nk - 1) // 2
+    mixed = precision == capi.PREC_MIXED
+    peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
+    achieved = FLOP_PER_INTERACTION * inter / (k_ms * 1e-3) / 1e12
     return {"value": inter / t, "unit": "interactions/s", "systems": len(sizes), "bodies": int(offsets[-1]),
             "steps": steps, "wall_s": t, "interactions": inter, "overlap_tests_per_s": tests / t,
-            "merges": int(len(log)),
+            "merges": int(len(log)), "kernel_ms": k_ms, "launches": k_launches,
+            "roofline": {"bound": "valu", "kernel": "k_batch_steps", "achieved": achieved, "peak": peak,
+                         "unit": "TFLOP/s", "frac": achieved / peak, "flop_per_interaction": FLOP_PER_INTERACTION,
+                         "avg_launch_ms": k_ms / max(k_launches, 1), "traffic": None},
             "workload": "4096 comet-string systems (seed 3): 1.9e27 kg central body + U{8..512} fragments, "
-                        "dt=5 s, inelastic merging on, FP32 pair forces from LDS, FP64 state, exact FP64 "
-                        "collision replay; persistent launch, one workgroup per system"}
+                        "dt=5 s, inelastic merging on, "
+                        + ("FP32 pair forces from LDS" if mixed else "FP64 pair forces in the reference's order")
+                        + ", FP64 state, exact FP64 collision replay; persistent launch, one workgroup per system"}
 
 
 def profile_run(capi, args):
@@ -350,6 +405,9 @@ def main():
     ap.add_argument("--no-bh", action="store_true")
     ap.add_argument("--bh-steps", type=int, default=5)
     ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--batch-fp64-steps", type=int, default=1000)
+    ap.add_argument("--cpu-baseline-port", action="store_true",
+                    help="time the oracle's restatement when the compiled reference (oracle/_ref) is absent")
     ap.add_argument("--hbm-copy", action="store_true", help="run only the HBM copy calibration kernel")
     ap.add_argument("--profile-direct", action="store_true",
                     help="for rocprofv3: the 65,536-body mixed direct step with fixed tuning, no events, no JSON")
@@ -449,11 +507,12 @@ def main():
         line["bh"] = bench_bh(capi, args.bh_steps)
         line["adjoint"] = bench_adjoint(capi)
         line["batch"] = bench_batch(capi, args.batch_steps)
+        line["batch"]["fp64"] = bench_batch(capi, args.batch_fp64_steps, capi.PREC_FP64)
         line["bh_coll"] = [bench_bh_coll(capi, 3, 0.002), bench_bh_coll(capi, 3, 0.005)]
         line["integrators"] = bench_integrators(capi)
     if not args.no_cpu_baseline:
         line["host"] = host_info()
-        line["cpu_baseline"] = cpu_baseline(m, pos, vel)
+        line["cpu_baseline"] = cpu_baseline(m, pos, vel, allow_port=args.cpu_baseline_port)
         line["cpu_baseline_allcore"] = cpu_direct_allcore(m, pos)
         if not args.no_bh:
             bm, bp, _ = capi.ic_two_plummer(2, 65536)

@@ -105,6 +105,7 @@ SIGNATURES = {
     "nb_batch_set_params": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double,
                                       C.c_int]),
     "nb_batch_evolve": (C.c_int, [_P, C.c_uint64, _I64]),
+    "nb_batch_kernel_time": (C.c_int, [_P, _D, _I64]),
     "nb_batch_get_state": (C.c_int, [_P, C.c_int64, _I64, _D, _D, _D, _D, _I64]),
     "nb_batch_merge_log": (C.c_int, [_P, C.c_int64, _I64, _I64]),
     "nb_ic_uniform_ball": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, _D, _D,
@@ -331,6 +332,14 @@ class System:
             return dict(depth=depth[:k], octant=octant[:k], body=body[:k], parent=parent[:k], mass=mass[:k],
                         com=com[:k], side=side[:k])
 
+    def tree_count(self):
+        """Node count of the octree of the current positions (builds it)."""
+        cnt = C.c_int64()
+        rc = self._L.nb_tree_topology(self.h, 0, None, None, None, None, None, None, None, C.byref(cnt))
+        if rc not in (NB_OK, NB_ERR_CAPACITY):
+            self._ck(rc)
+        return cnt.value
+
     def bh_accelerations(self):
         n = self.num_bodies()
         acc = np.zeros((n, 3))
@@ -422,6 +431,12 @@ class Batch:
         ns = C.c_int64()
         self._ck(self._L.nb_batch_evolve(self.h, steps, C.byref(ns)))
         return ns.value
+
+    def kernel_time(self):
+        """(total ms, launches) of the last evolve's stepping launches (HIP events)."""
+        ms, n = C.c_double(), C.c_int64()
+        self._ck(self._L.nb_batch_kernel_time(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def state(self):
         cap = self.total

@@ -140,7 +140,13 @@ struct nb_batch {
     DBuf<unsigned char> flags;
     BatchParams P{6.6743e-11, 1.0, 0.5, 0.0, 3, 1, 1e10, 1};
     int64_t step = 0;
+    // HIP events around the stepping launches of the last nb_batch_evolve
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double kernel_ms = 0.0;
+    int64_t kernel_launches = 0;
     ~nb_batch() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -293,6 +299,10 @@ int nb_batch_evolve(nb_batch *b, uint64_t steps, int64_t *elapsed_ns) {
                                b->step, (int64_t)1);
         }
         const int64_t chunk = 1000;  // bound the duration of one launch
+        if (!b->ev0) NB_HIP(hipEventCreate(&b->ev0));
+        if (!b->ev1) NB_HIP(hipEventCreate(&b->ev1));
+        NB_HIP(hipEventRecord(b->ev0, b->stream));
+        int64_t launches = 0;
         for (uint64_t done = 0; done < steps;) {
             const int64_t k = (int64_t)std::min<uint64_t>(chunk, steps - done);
             hipLaunchKernelGGL(k_batch_steps, dim3((unsigned)b->nsys), dim3(kThreads), 0, b->stream, all, b->ord.p,
@@ -301,15 +311,28 @@ int nb_batch_evolve(nb_batch *b, uint64_t steps, int64_t *elapsed_ns) {
             NB_HIP(hipGetLastError());
             b->step += k;
             done += (uint64_t)k;
+            ++launches;
         }
+        NB_HIP(hipEventRecord(b->ev1, b->stream));
         int errv = 0;
         NB_HIP(hipMemcpyAsync(&errv, b->errf.p, sizeof(int), hipMemcpyDeviceToHost, b->stream));
         NB_HIP(hipStreamSynchronize(b->stream));
+        float ms = 0.f;
+        NB_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->kernel_ms = ms;
+        b->kernel_launches = launches;
         if (errv) throw ApiError(NB_ERR_CAPACITY, "collision pass: more than 256 overlapping partners for one body");
         auto t1 = std::chrono::high_resolution_clock::now();
         if (elapsed_ns) *elapsed_ns = (int64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
         return NB_OK;
     });
+}
+
+int nb_batch_kernel_time(const nb_batch *b, double *total_ms, int64_t *launches) {
+    if (!b || !total_ms || !launches) return bfail(const_cast<nb_batch *>(b), NB_ERR_INVALID_ARGUMENT, "NULL argument");
+    *total_ms = b->kernel_ms;
+    *launches = b->kernel_launches;
+    return NB_OK;
 }
 
 int nb_batch_get_state(const nb_batch *cb, int64_t cap, int64_t *counts, double *m, double *r, double *pos,

@@ -283,6 +283,10 @@ const char *nb_batch_last_error(const nb_batch *b);
 int nb_batch_set_params(nb_batch *b, double G, double dt, double eps, int mode, int mergers, double min_tot_com_mom,
                         int precision);
 int nb_batch_evolve(nb_batch *b, uint64_t steps, int64_t *elapsed_ns);
+/* Engine extension: device time of the stepping launches of the last
+ * nb_batch_evolve (HIP events on the batch's stream around them, the a(x0)
+ * launch excluded) and the number of those launches. */
+int nb_batch_kernel_time(const nb_batch *b, double *total_ms, int64_t *launches);
 /* Per-system body counts (nsys) and the state packed per system in body
  * order; cap is in bodies. */
 int nb_batch_get_state(const nb_batch *b, int64_t cap, int64_t *counts, double *m, double *r, double *pos,
