@@ -30,6 +30,10 @@ NB_ERR_CAPACITY = -11
 NB_ERR_DEVICE = -12
 NB_ERR_NSYS_LOAD = -13
 NB_ERR_NSYS_FORMAT = -14
+NB_ERR_CALLBACK = -15
+HOOK_STEP = 1
+HOOK_BODIES = 2
+HOOK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int)
 
 TWO_BODY = 1
 EULER = 2
@@ -80,6 +84,9 @@ SIGNATURES = {
     "nb_evolve": (C.c_int, [_P, C.c_int, _I64]),
     "nb_advance": (C.c_int, [_P, C.c_uint64, _I64]),
     "nb_advance_noalloc": (C.c_int, [_P, C.c_uint64, _I64]),
+    "nb_merge_overlapping": (C.c_int, [_P, _I64]),
+    "nb_update_state": (C.c_int, [_P, C.c_int64, _D, _D, _D, _D, _D, _D]),
+    "nb_set_step_hook": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_int]),
     "nb_energy": (C.c_int, [_P, _D, _D]),
     "nb_merge_log": (C.c_int, [_P, C.c_int64, _I64, _I64]),
     "nb_set_recording": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_int]),
@@ -198,6 +205,26 @@ class System:
         n = m.shape[0]
         assert pos.size == 3 * n and vel.size == 3 * n
         self._ck(self._L.nb_set_bodies(self.h, n, dp(m), dp(r), dp(pos), dp(vel), dp(rest_c), int(check_overlap)))
+
+    def merge_overlapping(self) -> int:
+        """check_overlap with mergeOverlappingBodies; returns the bodies removed."""
+        k = C.c_int64()
+        self._ck(self._L.nb_merge_overlapping(self.h, C.byref(k)))
+        return k.value
+
+    def update_state(self, m=None, r=None, pos=None, vel=None, acc=None, rest_c=None):
+        m, r, pos, vel, acc, rest_c = map(_c64, (m, r, pos, vel, acc, rest_c))
+        self._ck(self._L.nb_update_state(self.h, self.num_bodies(), dp(m), dp(r), dp(pos), dp(vel), dp(acc),
+                                         dp(rest_c)))
+
+    def set_step_hook(self, fn, mask=HOOK_STEP | HOOK_BODIES):
+        """fn(step, where) -> int, called where evolve's callbacks run; None removes it."""
+        if fn is None:
+            self._hook = None
+            self._ck(self._L.nb_set_step_hook(self.h, None, None, 0))
+            return
+        self._hook = HOOK_FN(lambda user, sys, step, where: int(fn(step, where)))
+        self._ck(self._L.nb_set_step_hook(self.h, C.cast(self._hook, C.c_void_p), None, mask))
 
     def num_bodies(self) -> int:
         n = C.c_int64()

@@ -150,6 +150,21 @@ namespace nbe {
 
 int form_of(const nb_system *s) { return s->eps > 0 ? 1 : 0; }
 
+void run_hook(nb_system *s, uint64_t step, int where) {
+    if (!s->hook_fn || !(s->hook_mask & where)) return;
+    NB_HIP(hipStreamSynchronize(s->stream));
+    s->in_hook = true;
+    int rc;
+    try {
+        rc = s->hook_fn(s->hook_user, s, step, where);
+    } catch (...) {
+        rc = -1;  // a hook must not throw across the C ABI; treat it as a failure
+    }
+    s->in_hook = false;
+    if (rc != 0) throw ApiError(NB_ERR_CALLBACK, "step hook returned " + std::to_string(rc) + " at step " +
+                                                     std::to_string(step));
+}
+
 void direct_forces(nb_system *s) {
     const int64_t n = s->n;
     if (s->precision == NB_PREC_MIXED) {
@@ -247,7 +262,9 @@ void direct_other_steps(nb_system *s, int method, uint64_t steps) {
             launch_kick_drift(s->stream, n, s->a.p, s->v.p, s->x.p, s->dt, s->half_dt);
         }
         ++s->step_count;
+        run_hook(s, s->step_count, NB_HOOK_STEP);
         if (coll) collision_pass(s, s->step_count);
+        run_hook(s, s->step_count, NB_HOOK_BODIES);
         if (record_due(s, s->step_count)) record_state(s, s->step_count);
     }
 }
@@ -316,7 +333,8 @@ void direct_kdk_steps(nb_system *s, uint64_t steps) {
     };
     // Interior steps of small systems replay a captured graph of kGraphSteps
     // steps (no collisions, no per-launch timing).
-    const bool graphs = !coll && !s->prof.on && !s->rec_every &&
+    const bool hooks = hooks_on(s);
+    const bool graphs = !coll && !hooks && !s->prof.on && !s->rec_every &&
                         (s->tune_step_graphs == 1 || (s->tune_step_graphs == 0 && s->n <= kGraphMaxBodies));
     bool need_open = true;
     for (uint64_t st = 1; st <= steps;) {
@@ -343,10 +361,11 @@ void direct_kdk_steps(nb_system *s, uint64_t steps) {
         }
         if (fast64 && s->dw.partial64.n < (size_t)s->dw.nsplit64 * 3 * (size_t)n) direct_forces(s);
         if (mixed && s->dw.pos32.n < (size_t)n) direct_forces(s);  // defensive: sizes follow n
-        // a step that records or runs collisions ends with its closing kick
-        need_open = issue_step(need_open, st == steps || coll || record_due(s, s->step_count + 1));
+        // a step that records, runs collisions or calls hooks ends with its closing kick
+        need_open = issue_step(need_open, st == steps || coll || hooks || record_due(s, s->step_count + 1));
         ++st;
         ++s->step_count;
+        run_hook(s, s->step_count, NB_HOOK_STEP);
         if (coll) {
             collision_pass(s, s->step_count);
             if (mixed && s->n != n && s->n > 0) {
@@ -355,6 +374,7 @@ void direct_kdk_steps(nb_system *s, uint64_t steps) {
                 s->dw.partial.ensure((size_t)s->dw.nsplit * 6 * (size_t)s->n);
             }
         }
+        run_hook(s, s->step_count, NB_HOOK_BODIES);
         if (record_due(s, s->step_count)) record_state(s, s->step_count);
     }
 }
@@ -416,18 +436,7 @@ int nb_set_bodies(nb_system *s, int64_t n, const double *m, const double *r, con
             if (r && r[i] < 0)
                 throw ApiError(NB_ERR_NEGATIVE_RADIUS, "body " + std::to_string(i) + " has negative radius");
         }
-        if (check_overlap && r) {
-            // system::check_overlap (gregsys.hpp:307-342) without merging
-            for (int64_t i = 0; i < n; ++i)
-                for (int64_t j = i + 1; j < n; ++j) {
-                    double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
-                           dz = pos[3 * j + 2] - pos[3 * i + 2];
-                    if (std::sqrt(dx * dx + dy * dy + dz * dz) < r[i] + r[j])
-                        throw ApiError(NB_ERR_OVERLAPPING_BODIES, "The bodies with id=" + std::to_string(i) +
-                                                                      " and id=" + std::to_string(j) +
-                                                                      " that were added to this system object overlap.");
-                }
-        }
+        if (s->in_hook) throw ApiError(NB_ERR_INVALID_ARGUMENT, "nb_set_bodies cannot be called from a step hook");
         NB_HIP(hipStreamSynchronize(s->stream));
         s->n = n;
         size_t nn = (size_t)std::max<int64_t>(n, 1);
@@ -455,8 +464,68 @@ int nb_set_bodies(nb_system *s, int64_t n, const double *m, const double *r, con
         s->rec_total = 0;  // records of the previous bodies are dropped
         s->rec_n0 = std::max<int64_t>(s->rec_n0, n);
         s->time_elapsed = 0;
+        int64_t oi = 0, oj = 0;
+        if (check_overlap && n > 1 && first_overlap(s, 0, &oi, &oj)) {
+            // system::check_overlap (gregsys.hpp:307-342) without merging:
+            // the constructor throws, so no bodies are kept
+            s->n = 0;
+            s->labels.clear();
+            s->next_label = 0;
+            throw ApiError(NB_ERR_OVERLAPPING_BODIES, "The bodies with id=" + std::to_string(oi) + " and id=" +
+                                                          std::to_string(oj) +
+                                                          " that were added to this system object overlap.");
+        }
         return NB_OK;
     });
+}
+
+int nb_merge_overlapping(nb_system *s, int64_t *removed) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "nb_merge_overlapping: NULL system");
+    if (removed) *removed = 0;
+    return guarded(s, [&]() {
+        if (s->in_hook) throw ApiError(NB_ERR_INVALID_ARGUMENT, "nb_merge_overlapping cannot be called from a step hook");
+        const int64_t k = merge_overlapping(s);
+        if (removed) *removed = k;
+        return NB_OK;
+    });
+}
+
+int nb_update_state(nb_system *s, int64_t n, const double *m, const double *r, const double *pos, const double *vel,
+                    const double *acc, const double *rest_c) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "nb_update_state: NULL system");
+    return guarded(s, [&]() {
+        if (n != s->n) throw ApiError(NB_ERR_INVALID_ARGUMENT, "nb_update_state: n differs from the body count");
+        for (int64_t i = 0; i < n; ++i) {
+            if (m && m[i] < 0) throw ApiError(NB_ERR_NEGATIVE_MASS, "body " + std::to_string(i) + " has negative mass");
+            if (r && r[i] < 0)
+                throw ApiError(NB_ERR_NEGATIVE_RADIUS, "body " + std::to_string(i) + " has negative radius");
+        }
+        NB_HIP(hipStreamSynchronize(s->stream));
+        if (n == 0) return NB_OK;
+        if (m) copy_wait(s->m.p, m, (size_t)n * 8, hipMemcpyHostToDevice, s->stream);
+        if (r) copy_wait(s->r.p, r, (size_t)n * 8, hipMemcpyHostToDevice, s->stream);
+        if (rest_c) copy_wait(s->e.p, rest_c, (size_t)n * 8, hipMemcpyHostToDevice, s->stream);
+        std::vector<double> tmp;
+        const double *srcs[3] = {pos, vel, acc};
+        double *dsts[3] = {s->x.p, s->v.p, s->a.p};
+        for (int q = 0; q < 3; ++q) {
+            if (!srcs[q]) continue;
+            to_soa(n, srcs[q], tmp);
+            copy_wait(dsts[q], tmp.data(), (size_t)(3 * n) * 8, hipMemcpyHostToDevice, s->stream);
+        }
+        return NB_OK;
+    });
+}
+
+int nb_set_step_hook(nb_system *s, nb_step_hook fn, void *user, int where_mask) {
+    if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "nb_set_step_hook: NULL system");
+    if (where_mask & ~(NB_HOOK_STEP | NB_HOOK_BODIES))
+        return fail(s, NB_ERR_INVALID_ARGUMENT, "nb_set_step_hook: unknown bits in where_mask");
+    if (s->in_hook) return fail(s, NB_ERR_INVALID_ARGUMENT, "nb_set_step_hook cannot be called from a step hook");
+    s->hook_fn = fn;
+    s->hook_user = fn ? user : nullptr;
+    s->hook_mask = fn ? where_mask : 0;
+    return NB_OK;
 }
 
 int nb_get_state(const nb_system *cs, int64_t cap, double *m, double *r, double *pos, double *vel, double *acc,
@@ -583,6 +652,7 @@ int nb_evolve(nb_system *s, int method, int64_t *elapsed_ns) {
     if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
     if (elapsed_ns) *elapsed_ns = 0;
     return guarded(s, [&]() {
+        if (s->in_hook) throw ApiError(NB_ERR_INVALID_ARGUMENT, "nb_evolve cannot be called from a step hook");
         if (!s->n || !check_option(method)) return NB_OK;  // evolve returns 0 ns (:2611-2612)
         auto t0 = std::chrono::high_resolution_clock::now();
         if ((method & NB_TWO_BODY) == NB_TWO_BODY) {
@@ -646,6 +716,7 @@ int nb_advance(nb_system *s, uint64_t steps, int64_t *elapsed_ns) {
     if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
     if (elapsed_ns) *elapsed_ns = 0;
     return guarded(s, [&]() {
+        if (s->in_hook) throw ApiError(NB_ERR_INVALID_ARGUMENT, "nb_advance cannot be called from a step hook");
         const bool other = s->evolved && (s->acc_method == NB_EULER || s->acc_method == NB_MODIFIED_EULER ||
                                            s->acc_method == NB_MIDPOINT || s->acc_method == NB_LEAPFROG_DKD);
         if (!s->evolved || (!other && !s->acc_valid))
@@ -667,6 +738,7 @@ int nb_advance_noalloc(nb_system *s, uint64_t steps, int64_t *elapsed_ns) {
     if (!s) return fail(nullptr, NB_ERR_INVALID_ARGUMENT, "NULL system");
     if (elapsed_ns) *elapsed_ns = 0;
     return guarded(s, [&]() {
+        if (s->in_hook) throw ApiError(NB_ERR_INVALID_ARGUMENT, "nb_advance_noalloc cannot be called from a step hook");
         if (!s->evolved || !s->acc_valid || s->acc_method != NB_LEAPFROG_KDK)
             throw ApiError(NB_ERR_NO_EVOLUTION,
                            "nb_advance_noalloc needs a previous direct-sum NB_LEAPFROG_KDK nb_evolve with unchanged "

@@ -183,6 +183,7 @@ struct DirectWork {
     DBuf<double> partial64;  // [split][3][n] for the FP64 split kernels
     DBuf<double> terms;      // [j block][3][n] pair terms of the FP64 reference-order path
     DBuf<double> px, pv, pa; // predicted state of modified Euler / midpoint (SoA, 3n)
+    DBuf<unsigned long long> ovl_best;  // first overlapping pair of the construction check
     int nsplit64 = 1;
 };
 

@@ -60,6 +60,12 @@ struct nb_system {
     nbe::StepGraph step_graph;
     nbe::GraphKey step_graph_key;
 
+    // step hooks (nb_set_step_hook): called where evolve's callbacks run
+    nb_step_hook hook_fn = nullptr;
+    void *hook_user = nullptr;
+    int hook_mask = 0;
+    bool in_hook = false;
+
     nbe::DirectWork dw;
     nbe::BHWork *bh = nullptr;
     nbe::CollWork *coll = nullptr;
@@ -75,6 +81,12 @@ void direct_kdk_steps(nb_system *s, uint64_t steps); // steps of KDK from valid 
 int form_of(const nb_system *s);                     // 0 = unsoftened reference form, 1 = softened
 GraphKey graph_key(const nb_system *s);
 void set_static_error(const std::string &msg);  // message for nb_last_error(NULL)
+// step hooks: call the hook for `where` when set (synchronises the stream)
+void run_hook(nb_system *s, uint64_t step, int where);
+inline bool hooks_on(const nb_system *s) { return s->hook_fn != nullptr && s->hook_mask != 0; }
+// construction-time overlap check (nb_overlap.hip)
+bool first_overlap(nb_system *s, int64_t from, int64_t *outer, int64_t *inner);
+int64_t merge_overlapping(nb_system *s);
 // state recording (nb_record.hip): a record of the current state when `step`
 // is a multiple of the recording interval
 void record_state(nb_system *s, uint64_t step);

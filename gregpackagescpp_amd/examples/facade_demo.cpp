@@ -12,7 +12,7 @@ int main() {
     using sys_t = gtd::system<double, double, double, false, false, 0, 0, 0, false>;
     const double M = 1.89813e27, R = 4.217e8, G = 6.6743e-11;  // Jupiter and Io's orbital radius
     const double v = std::sqrt(G * M / R);
-    std::vector<gtd::body> bods;
+    std::vector<gtd::body<double, double, double, 0>> bods;
     bods.emplace_back(M, 6.9911e7);
     bods.emplace_back(8.93e22, 1.8216e6, gtd::vector3D<double>{R, 0, 0}, gtd::vector3D<double>{0, v, 0});
     const double period = 2 * M_PI * std::sqrt(R * R * R / (G * M));
@@ -27,6 +27,32 @@ int main() {
                 (unsigned long long)steps, ns.count() / 1e6, r / R - 1, std::atan2(dy, dx));
     if (std::fabs(r / R - 1) > 1e-6 || std::fabs(std::atan2(dy, dx)) > 1e-3) return 1;
 
+    // memFreq = 10: energies and body histories every 10 steps (the
+    // reference's pe_extrema / prev_pos_at accessors), in long double
+    {
+        using ld = long double;
+        std::vector<gtd::body<ld, ld, ld, 10>> lb;
+        lb.emplace_back((ld)M, (ld)6.9911e7);
+        lb.emplace_back((ld)8.93e22, (ld)1.8216e6, gtd::vector3D<ld>{(ld)R, 0, 0}, gtd::vector3D<ld>{0, (ld)v, 0});
+        gtd::system<ld, ld, ld, false, false, 0, 10, 0, false> rec{lb, (ld)(period / steps), 200, "M1:1,D1:1,T1:1",
+                                                                   true};
+        rec.evolve(decltype(rec)::leapfrog_kdk);
+        const auto *pe = rec.pe_extrema();
+        const auto *te = rec.tot_e_extrema();
+        std::printf("memFreq 10: %zu records, PE in [%.6Le, %.6Le], total energy spread %.3Le\n",
+                    rec.total_pe().size(), pe[0].first[0], pe[1].first[0], te[1].first[0] - te[0].first[0]);
+        if (rec.total_pe().size() != 21 || rec[1].num_recorded() != 21 || rec[1].prev_pos_at(0).get_x() != (ld)R ||
+            !(pe[0].first[0] <= pe[1].first[0]) || std::fabs((double)((te[1].first[0] - te[0].first[0]) /
+                                                                      te[0].first[0])) > 1e-6)
+            return 8;
+    }
+    // prog: the reference's progress lines and closing summary
+    {
+        gtd::system<double, double, double, true, false, 0, 0, 0, false> p{bods, period / steps, 300,
+                                                                           "M1:1,D1:1,T1:1", true};
+        p.evolve(decltype(p)::leapfrog_kdk);
+    }
+
     // continue_ev_if: stop after 100 polled steps
     int polls = 0;
     sys.evolve(sys_t::leapfrog_kdk, [&]() { return polls++ < 100; });
@@ -36,7 +62,7 @@ int main() {
     const int64_t n = 20000;
     std::vector<double> m(n), p(3 * n), w(3 * n);
     if (nb_ic_plummer(7, n, 1.0, nullptr, nullptr, m.data(), p.data(), w.data()) != NB_OK) return 2;
-    std::vector<gtd::body> pl;
+    std::vector<gtd::body<double, double, double, 0>> pl;
     pl.reserve(n);
     for (int64_t i = 0; i < n; ++i)
         pl.emplace_back(m[i], 0.0, gtd::vector3D<double>{p[3 * i], p[3 * i + 1], p[3 * i + 2]},
@@ -49,7 +75,7 @@ int main() {
     auto bns = bh.evolve(sys_t::leapfrog_kdk | sys_t::barnes_hut);
     std::printf("Barnes-Hut: %lld bodies, 10 steps in %.3f ms\n", (long long)bh.num_bodies(), bns.count() / 1e6);
     // body_tracker over the inner bodies (the reference's COM-of-a-subset helper)
-    gtd::body_tracker<> core{bh, [](const gtd::body &b) {
+    gtd::body_tracker<double, double, double, 0> core{bh, [](const auto &b) {
         return b.pos().get_x() * b.pos().get_x() + b.pos().get_y() * b.pos().get_y() < 0.25;
     }};
     const auto c = core.com();
@@ -59,7 +85,7 @@ int main() {
     try {
         sys.set_timestep(-1);
         if (sys.set_timestep(-1)) return 3;  // setters answer false, as the reference's do
-        std::vector<gtd::body> bad{gtd::body(1.0, 1.0), gtd::body(1.0, 1.0)};
+        std::vector<gtd::body<double, double, double, 0>> bad{{1.0, 1.0}, {1.0, 1.0}};
         sys_t overlapping{bad, 1.0, 1, "M1:1,D1:1,T1:1", false};
         return 4;
     } catch (const gtd::overlapping_bodies_error &e) {

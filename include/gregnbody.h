@@ -41,6 +41,7 @@ extern "C" {
 #define NB_ERR_DEVICE (-12)               /* HIP runtime failure                   */
 #define NB_ERR_NSYS_LOAD (-13)            /* nsys_load_error                       */
 #define NB_ERR_NSYS_FORMAT (-14)          /* nsys_format_error                     */
+#define NB_ERR_CALLBACK (-15)             /* a step hook returned nonzero          */
 
 /* ---- integration method bit flags: gtd::system constants gregsys.hpp:156-165 ---- */
 #define NB_TWO_BODY 1
@@ -77,9 +78,48 @@ const char *nb_last_error(const nb_system *sys);
  * (gregbod.hpp:329-336).  r and rest_c may be NULL (radius 1, restitution 1, the
  * body<> defaults gregbod.hpp:907-916).  When check_overlap != 0 the O(N^2)
  * construction-time overlap test of system::check_overlap (gregsys.hpp:307-342)
- * runs and NB_ERR_OVERLAPPING_BODIES is returned on the first overlap. */
+ * runs on the GPU and NB_ERR_OVERLAPPING_BODIES is returned on the first
+ * overlapping pair in (outer, inner > outer) order, the context then holding
+ * no bodies (as the reference's constructor throws). */
 int nb_set_bodies(nb_system *sys, int64_t n, const double *m, const double *r, const double *pos,
                   const double *vel, const double *rest_c, int check_overlap);
+/* system::check_overlap with mergeOverlappingBodies (gregsys.hpp:307-342)
+ * on the current bodies: the reference's construction-time loop, which
+ * merges an overlapping pair (body::operator+=, gregbod.hpp:1065-1082), erases
+ * the inner body and restarts, with its exact visiting order after a merge.
+ * The first overlapping outer body is found on the GPU; the loop is followed
+ * on the host from there.  *removed (may be NULL) receives the number of
+ * bodies removed; labels follow the surviving bodies (a merged body keeps the
+ * outer body's label). */
+int nb_merge_overlapping(nb_system *sys, int64_t *removed);
+/* Overwrite the state of the current bodies in place (engine extension; the
+ * per-body setters of gtd::body -- set_mass, set_radius, set_restitution,
+ * set_acc, update, shift -- gregbod.hpp:498-600, applied from the host).  n
+ * must equal the body count; any pointer may be NULL (that field is kept).
+ * Labels, time elapsed, the merge log and the evolved state are kept, and
+ * acc is the acceleration the next opening half kick uses, as the body's
+ * stored acceleration is in the reference.  Negative masses or radii return
+ * NB_ERR_NEGATIVE_MASS / NB_ERR_NEGATIVE_RADIUS with nothing changed. */
+int nb_update_state(nb_system *sys, int64_t n, const double *m, const double *r, const double *pos,
+                    const double *vel, const double *acc, const double *rest_c);
+/* Step hooks: where evolve's callbacks run in the reference
+ * (gregsys.hpp:2593-2607, 2946-2990).  For the direct-sum integrators, after
+ * each step's update and before the collision pass (for_all_bods, passed into
+ * leapfrog_kdk_acc_e_and_step :527-539 and FUNC_TEMPL_SELECT :16-57), and
+ * after the collision pass (for_each_bod, :2961-2963), the engine
+ * synchronises its stream and calls fn(user, sys, step, where) with where =
+ * NB_HOOK_STEP or NB_HOOK_BODIES (selected by where_mask).  Inside the hook
+ * the caller may read the state (nb_get_state, nb_body_ids, nb_num_bodies)
+ * and change it (nb_update_state); nb_evolve, nb_advance and nb_set_bodies
+ * are refused there.  A nonzero return stops the evolve with
+ * NB_ERR_CALLBACK.  Steps that run hooks end with their closing half kick and
+ * replay no captured graph.  Barnes-Hut steps call no hook, as the
+ * reference's BH branch calls neither callback (:2893-2945).  fn = NULL
+ * removes the hook. */
+#define NB_HOOK_STEP 1
+#define NB_HOOK_BODIES 2
+typedef int (*nb_step_hook)(void *user, nb_system *sys, uint64_t step, int where);
+int nb_set_step_hook(nb_system *sys, nb_step_hook fn, void *user, int where_mask);
 /* State out (any pointer may be NULL).  Replaces iteration / operator[] over
  * bodies (gregsys.hpp:3178-3213, 3383-3387) with body::pos()/vel()/mass()/rad()
  * (gregbod.hpp:444-459).  cap is the capacity in bodies. */

@@ -215,6 +215,56 @@ void merge_into(CBody<T> &self, const CBody<T> &other) {
 
 struct MergeRec { int64_t step, outer_label, absorbed_label, new_label; };
 
+/* ---------------- construction-time overlap check ----------------
+ * system::check_overlap (gregsys.hpp:307-342), restated loop for loop,
+ * including how it walks after a merge: `continue` advances the inner index
+ * but not the inner pointer, so after the first reset the body tested at
+ * inner index k is bods[k - 1]; the skip at inner == outer then leaves
+ * bods[outer - 1] untested and the next test meets the outer body itself
+ * (distance 0, merged with itself -- a no-op while body::allow_self_addition
+ * is false, gregbod.hpp:1068-1070 -- and the body at that inner index is
+ * erased); a prefix merge erases the body after the merged one.  The overlap
+ * test is distance(p_o, p_i) < r_o + r_i with distance = sqrt of the ordered
+ * sum of squares of (p_i - p_o) (vec_ops::distance, gregvec.hpp:4701-4706).
+ * merge = false returns at the first overlap: *first_outer / *first_inner
+ * receive its indices (-1 when none). */
+template <typename T>
+void check_overlap(std::vector<CBody<T>> &bods, bool merge, int64_t *first_outer, int64_t *first_inner) {
+    *first_outer = *first_inner = -1;
+    int64_t num = (int64_t)bods.size();
+    for (int64_t outer = 0; outer < num; ++outer) {
+        int64_t ob = outer;          // outer_b as an index
+        int64_t ib = outer + 1;      // inner_b as an index
+        for (int64_t inner = outer + 1; inner < num; ++inner) {
+            if (merge && inner == outer) {
+                ++ib;
+                continue;
+            }
+            const CBody<T> &o = bods[ob], &b = bods[ib];
+            T dx = b.p[0] - o.p[0], dy = b.p[1] - o.p[1], dz = b.p[2] - o.p[2];
+            if (t_sqrt<T>(dx * dx + dy * dy + dz * dz) < o.r + b.r) {
+                if (!merge) {
+                    *first_outer = outer;
+                    *first_inner = inner;
+                    return;
+                }
+                if (ib != ob) merge_into<T>(bods[ob], bods[ib]);  // self-addition is a no-op
+                bods.erase(bods.begin() + inner);
+                if (inner < outer) {
+                    --outer;
+                    --ob;
+                }
+                inner = 0;  // the loop's ++inner makes it 1; the pointer stays at element 0
+                ib = 0;
+                --num;
+                continue;
+            }
+            ++ib;
+        }
+    }
+}
+
+
 template <typename T>
 struct Overlap { int64_t idx; T mo, mi; T n[3]; };
 
@@ -883,6 +933,40 @@ int orc_s_coll_d(int64_t *n_io, int merge, double min_mom, int64_t step, double 
         log[4 * q + 2] = lg[q].absorbed_label; log[4 * q + 3] = lg[q].new_label;
     }
     return 0;
+}
+
+/* check_overlap: merge = 0 -> *first = (outer, inner) of the first overlap or
+ * (-1, -1); merge = 1 -> the merged bodies replace the arrays, *n_io the new
+ * count, labels follow the surviving bodies. */
+}  // extern "C"
+template <typename T>
+static int check_overlap_arrays(int64_t *n_io, int merge, T *m, T *r, T *e, T *pos, T *vel, int64_t *labels,
+                                int64_t *first) {
+    int64_t n = *n_io;
+    std::vector<CBody<T>> bods(n);
+    for (int64_t i = 0; i < n; ++i) {
+        CBody<T> &b = bods[i];
+        b.label = labels[i]; b.m = m[i]; b.r = r[i]; b.e = e[i];
+        for (int k = 0; k < 3; ++k) { b.p[k] = pos[3 * i + k]; b.v[k] = vel[3 * i + k]; b.a[k] = T(0); }
+    }
+    check_overlap<T>(bods, merge != 0, first, first + 1);
+    n = (int64_t)bods.size();
+    *n_io = n;
+    for (int64_t i = 0; i < n; ++i) {
+        const CBody<T> &b = bods[i];
+        labels[i] = b.label; m[i] = b.m; r[i] = b.r; e[i] = b.e;
+        for (int k = 0; k < 3; ++k) { pos[3 * i + k] = b.p[k]; vel[3 * i + k] = b.v[k]; }
+    }
+    return 0;
+}
+extern "C" {
+int orc_check_overlap_d(int64_t *n_io, int merge, double *m, double *r, double *e, double *pos, double *vel,
+                        int64_t *labels, int64_t *first) {
+    return check_overlap_arrays<double>(n_io, merge, m, r, e, pos, vel, labels, first);
+}
+int orc_check_overlap_ld(int64_t *n_io, int merge, long double *m, long double *r, long double *e, long double *pos,
+                         long double *vel, int64_t *labels, int64_t *first) {
+    return check_overlap_arrays<long double>(n_io, merge, m, r, e, pos, vel, labels, first);
 }
 
 /* ---------------- Barnes-Hut ---------------- */

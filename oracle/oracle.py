@@ -47,6 +47,8 @@ def lib():
                                  _LDP, _LDP, _I64, _I64, C.c_int64, _I64]
         L.orc_kdk_d.argtypes = [_I64, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int, C.c_double,
                                 _D, _D, _D, _D, _D, _D, _I64, _I64, C.c_int64, _I64]
+        L.orc_check_overlap_d.argtypes = [_I64, C.c_int, _D, _D, _D, _D, _D, _I64, _I64]
+        L.orc_check_overlap_ld.argtypes = [_I64, C.c_int, _LDP, _LDP, _LDP, _LDP, _LDP, _I64, _I64]
         L.orc_s_coll_d.argtypes = [_I64, C.c_int, C.c_double, C.c_int64, _D, _D, _D, _D, _D, _D, _I64, _I64, _I64,
                                    C.c_int64, _I64]
         L.orc_bh_tree_d.restype = C.c_int64
@@ -205,6 +207,26 @@ def s_coll(m, r, e, pos, vel, acc, labels, next_label, merge=True, min_mom=1e10,
     k = nn.value
     return dict(n=k, m=m[:k], r=r[:k], e=e[:k], pos=pos[:k], vel=vel[:k], acc=acc[:k], labels=lab[:k],
                 next_label=nl.value, log=log[: loglen.value])
+
+
+def check_overlap(m, r, pos, vel, e=None, merge=False, dtype=LD):
+    """system::check_overlap (gregsys.hpp:307-342).  merge=False: (outer,
+    inner) of the first overlap, or None.  merge=True: the bodies after the
+    construction-time merges, dict(n, m, r, e, pos, vel, labels)."""
+    n = len(m)
+    conv = _ld if dtype == LD else _d
+    ptr = _lp if dtype == LD else _dp
+    m, r, pos, vel = conv(m).copy(), conv(r).copy(), conv(pos).copy(), conv(vel).copy()
+    e = conv(e if e is not None else np.ones(n)).copy()
+    lab = np.arange(n, dtype=np.int64)
+    first = np.zeros(2, np.int64)
+    nn = C.c_int64(n)
+    fn = lib().orc_check_overlap_ld if dtype == LD else lib().orc_check_overlap_d
+    fn(C.byref(nn), int(merge), ptr(m), ptr(r), ptr(e), ptr(pos), ptr(vel), _ip(lab), _ip(first))
+    if not merge:
+        return None if first[0] < 0 else (int(first[0]), int(first[1]))
+    k = nn.value
+    return dict(n=k, m=m[:k], r=r[:k], e=e[:k], pos=pos[:k], vel=vel[:k], labels=lab[:k])
 
 
 def bh_tree(m, pos, cap=None):

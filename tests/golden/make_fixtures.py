@@ -24,9 +24,20 @@ generators; outputs are the reference's own results, stored as .npz:
                          a dense rubble cluster: inputs and the velocities
                          after the pass (x87 long double).
 
-Usage: python tests/golden/make_fixtures.py [integ|bhcoll|track]   (only the
-       integrator / bh_s_coll / body_tracker fixtures)
+  ref_dropin_{cor,cor_pred}_seed1.npz  examples/drop_in_driver.cpp built
+                         against the reference (oracle/_ref/drop_in_driver_ref):
+                         long double system with collisions = 3 whose
+                         for_all_bods rewrites every restitution coefficient
+                         the way random_comet does; every body after every
+                         step (rubble cluster, seed 1, 300 steps).
+  ref_dropin_merge_{case}.npz / ref_dropin_check_{case}.npz  the same driver's
+                         construction with mergeOverlappingBodies (surviving
+                         bodies) and the plain overlap check (throws or not).
+
+Usage: python tests/golden/make_fixtures.py [integ|bhcoll|track|dropin]   (only
+       the integrator / bh_s_coll / body_tracker / drop-in driver fixtures)
 """
+import subprocess
 import os
 import sys
 import tempfile
@@ -109,6 +120,59 @@ def track_fixtures(tmp):
         print("track", n, seed, cnt)
 
 
+def overlap_cases():
+    """Inputs of the construction-time overlap check: (name, m, r, e, pos, vel)."""
+    cases = []
+    # a merge whose grown body then reaches a prefix body (q <= o - 2): the
+    # reference erases the body after q, shifts the outer index down, and
+    # erases every body after the outer through self-additions
+    m = np.array([1.0, 1.0, 1.0, 1.0, 2.0, 3.0])
+    r = np.ones(6)
+    pos = np.array([[0.0, 0, 0], [100.0, 0, 0], [2.5, 0, 0], [2.0, 0, 0], [50.0, 0, 0], [70.0, 0, 0]])
+    vel = np.arange(18, dtype=float).reshape(6, 3) * 0.1
+    cases.append(("prefix", m, r, np.full(6, 0.5), pos, vel))
+    # an outer of zero radius: its self-test fails, so the loop keeps testing
+    # (one element behind) the bodies after it
+    m = np.array([1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 4.0])
+    r = np.array([1.0, 0.0, 1.0, 1.0, 1.0, 1.0, 1.0])
+    pos = np.array([[0.0, 0, 0], [10.0, 0, 0], [10.5, 0, 0], [30.0, 0, 0], [31.0, 0, 0], [60.0, 0, 0],
+                    [61.5, 0, 0]])
+    vel = np.arange(21, dtype=float).reshape(7, 3) * -0.2
+    cases.append(("zero_radius", m, r, np.linspace(0.2, 0.8, 7), pos, vel))
+    # rubble clusters with inflated radii (several overlapping pairs) and one
+    # without any overlap
+    for seed, rfac, n in ((11, 0.3, 60), (12, 0.3, 60), (20, 0.5, 60), (26, 0.3, 60), (13, 3.0, 200)):
+        m, r, e, pos, vel = coll_ics(n, seed)
+        cases.append((f"rubble{seed}", m, r * rfac, e, pos, vel))
+    return cases
+
+
+def dropin_fixtures(tmp):
+    exe = os.path.join(O.REF_DIR, "drop_in_driver_ref")
+    inp, out = os.path.join(tmp, "d_in.bin"), os.path.join(tmp, "d_out.bin")
+    m, r, e, pos, vel = coll_ics(24, 1)
+    for mode in ("cor", "cor_pred"):
+        O.write_ref_input(inp, m, pos, vel, r=r, e=e, steps=300, dt=5.0)
+        subprocess.run([exe, mode, inp, out], check=True, timeout=600)
+        rec = np.fromfile(out, dtype=np.float64).reshape(-1, 11)
+        rec = rec[rec[:, 0] % 25 == 0]  # every body at every 25th step
+        np.savez_compressed(os.path.join(OUT, f"ref_dropin_{mode}_seed1.npz"), m=m, r=r, e=e, pos=pos, vel=vel,
+                            dt=5.0, steps=300, rec=rec)
+        print("dropin", mode, len(rec))
+    for name, m, r, e, pos, vel in overlap_cases():
+        O.write_ref_input(inp, m, pos, vel, r=r, e=e, steps=1, dt=1.0)
+        subprocess.run([exe, "merge", inp, out], check=True, timeout=600)
+        res = np.fromfile(out, dtype=np.float64)
+        k = int(res[0])
+        np.savez_compressed(os.path.join(OUT, f"ref_dropin_merge_{name}.npz"), m=m, r=r, e=e, pos=pos, vel=vel,
+                            bodies=res[1:].reshape(k, 9))
+        subprocess.run([exe, "check", inp, out], check=True, timeout=600, capture_output=True)
+        flag = np.fromfile(out, dtype=np.float64)[0]
+        np.savez_compressed(os.path.join(OUT, f"ref_dropin_check_{name}.npz"), m=m, r=r, e=e, pos=pos, vel=vel,
+                            throws=flag)
+        print("dropin merge", name, len(m), "->", k, "check throws", flag)
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref is not built (needs /root/reference): nothing generated")
@@ -123,8 +187,12 @@ def main():
     if sys.argv[1:] == ["track"]:
         track_fixtures(tmp)
         return
+    if sys.argv[1:] == ["dropin"]:
+        dropin_fixtures(tmp)
+        return
     bhcoll_fixtures(tmp)
     track_fixtures(tmp)
+    dropin_fixtures(tmp)
     integ_fixtures(inp, out)
     for n, steps in KDK_CASES:
         m, pos, vel = capi.ic_uniform_ball(0, n, 1e20, 1e22, 1e9, 1e3)
