@@ -67,16 +67,24 @@ int fail(nb_system *s, int code, const std::string &msg) {
     return code;
 }
 
+// Runs an entry point's body, turning library exceptions into status codes.
+// Barnes-Hut steps keep the bodies in the tree's order between steps; every
+// entry point returns with them back in input order.
 template <typename F>
 int guarded(nb_system *s, F &&f) {
     try {
         if (s) NB_HIP(hipSetDevice(s->device));
-        return f();
+        const int rc = f();
+        if (s) bh_restore_order(s);
+        return rc;
     } catch (const ApiError &e) {
+        if (s) restore_order_quietly(s);
         return fail(s, e.code, e.what);
     } catch (const DeviceError &e) {
+        if (s) restore_order_quietly(s);
         return fail(s, NB_ERR_DEVICE, e.what);
     } catch (const std::bad_alloc &) {
+        if (s) restore_order_quietly(s);
         return fail(s, NB_ERR_CAPACITY, "host allocation failed");
     }
 }
@@ -438,6 +446,7 @@ int nb_set_bodies(nb_system *s, int64_t n, const double *m, const double *r, con
         }
         if (s->in_hook) throw ApiError(NB_ERR_INVALID_ARGUMENT, "nb_set_bodies cannot be called from a step hook");
         NB_HIP(hipStreamSynchronize(s->stream));
+        bh_forget_order(s);
         s->n = n;
         size_t nn = (size_t)std::max<int64_t>(n, 1);
         s->m.ensure(nn); s->r.ensure(nn); s->e.ensure(nn); s->pe.ensure(nn);
@@ -919,6 +928,7 @@ int nb_bh_accelerations(nb_system *s, double *acc, int64_t *interactions) {
     return guarded(s, [&]() {
         if (s->n == 0) return NB_OK;
         bh_forces(s, interactions);
+        bh_restore_order(s);   // accelerations back in input order before they are read
         s->acc_valid = false;  // cached a no longer matches an evolve in progress
         NB_HIP(hipStreamSynchronize(s->stream));
         s->prof.collect();

@@ -19,20 +19,24 @@
 //     maximal run sharing L >= 1 digits; nodes starting at s are levels
 //     (c[s-1], c[s]] plus the leaf of s, so an exclusive scan of
 //     1 + max(0, c[s]-c[s-1]) numbers every node in DFS pre-order with octants
-//     0..7 -- the order cell_iterator visits them.  A node's last position
-//     (first e >= s with c[e] < L) comes from a sparse-table range minimum;
-//     its skip pointer (next node outside its subtree) is the first node
-//     starting at e + 1;
-//  5. node mass and COM bottom up in FP64, internal nodes grouped by depth
-//     (one radix pass); the sparse deep depths in one single-workgroup
-//     launch, then one launch per depth up to the root, children summed in
-//     reverse octant order as calc_com;
-//  6. the walk is stackless over (node, skip): descend = node + 1, accept or
-//     skip = skip[node].  Two walks: per lane with the reference's per-body
-//     MAC in FP64 (parity), and per wave with FP32 monopoles where a node is
-//     opened for all 64 lanes when any lane needs it (node data is
-//     wave-uniform, so it arrives through scalar loads); a rejected node with
-//     at most `bucket` bodies is summed body by body instead of descended.
+//     0..7 -- the order cell_iterator visits them.  One thread per sorted
+//     position writes the nodes starting there: a node's last position (the
+//     end of its run) and the start of its parent's run come from doubling
+//     and bisection searches over the sorted keys, its skip pointer (next
+//     node outside its subtree) is the first node starting at last + 1;
+//  5. the bodies move into the tree's order in the same pass and stay in it
+//     from step to step (the next step's gathers then read nearly sequential
+//     addresses, and the walk writes accelerations in place); the input order
+//     is restored when an evolve returns;
+//  6. node mass and COM bottom up in FP64, children summed in reverse octant
+//     order as calc_com: one workgroup per subtree of at most kSubtree nodes
+//     (depth by depth under workgroup barriers), then one workgroup for the
+//     nodes above those subtrees; the walk records are written on the way;
+//  7. the walk is stackless over (node, skip): descend = node + 1, accept or
+//     skip = skip[node].  Walks: per lane with the reference's per-body MAC
+//     in FP64 (parity), per wave with FP32 monopoles where a node is opened
+//     for all 64 lanes when any lane needs it, and the frontier walk (the
+//     default FP32 walk, nb_walk_dev.hpp).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -48,9 +52,6 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kLevels = 42;   // octant digits per body
 constexpr int kHiLevels = 21; // digits in the high key
-constexpr int kMaxLog = 32;
-constexpr int kComGrid = 512;  // workgroups of a per-depth COM launch
-constexpr int kComDeep = 20;   // depths >= this are summed by one workgroup in one launch
 
 enum { BH_ERR_NONE = 0, BH_ERR_PLACEMENT = 1, BH_ERR_INDIST = 2, BH_ERR_DEPTH = 3 };
 }  // namespace
@@ -63,26 +64,35 @@ struct BHWork {
     DBuf<double> part;      // bbox partials
     DBuf<double> box;       // btm[3], top[3]
     DBuf<int> err;          // BH_ERR_*
-    DBuf<uint64_t> khi, klo, klo_tmp;
-    DBuf<uint32_t> idx;
-    DBuf<uint8_t> st;       // sparse table [log][n]
+    DBuf<uint64_t> khi, klo, klo_slot;  // high keys and low keys in sorted order; low keys by body slot
+    DBuf<uint32_t> idx;     // sorted position -> body slot (the identity once the bodies are in tree order)
+    DBuf<uint8_t> c;        // shared digits of sorted neighbours
     DBuf<uint32_t> cnt;     // nodes starting at s, then their exclusive scan
     DBuf<uint32_t> blk;     // scan block sums
     DBuf<uint32_t> total;   // node count
+    // the bodies in tree order: slot -> input index, and the other copy of
+    // every per-body array
+    bool permuted = false;
+    bool acc_permuted = false;  // the accelerations too (a walk wrote them)
+    DBuf<int32_t> perm, perm_alt;
+    DBuf<double> xa, va, aa;
+    // masses, radii and restitution in tree order (the context's own arrays
+    // stay in input order: Barnes-Hut steps do not change them)
+    DBuf<double> mt, rt, et, mt_alt, rt_alt, et_alt;
     // nodes (DFS pre-order)
     DBuf<int32_t> first, last, skip, parent, body;
     DBuf<uint8_t> depth, nchild;
     DBuf<int8_t> octant;
-    DBuf<double> mass, com, side;
-    DBuf<uint32_t> ready;
-    DBuf<uint64_t> lvl_key; // node depth (leaves: 63), radix-sorted with
-    DBuf<uint32_t> lvl_node;//   the node index: internal nodes grouped by depth
-    DBuf<uint32_t> lvl_off; // first position of each depth in the sorted list
+    DBuf<double> mass, com, side;  // com: SoA with stride cap_nodes
     DBuf<WalkNode> wnode;   // walk record per node (one 32-byte line), DFS order
     DBuf<WalkNode> wgrp;    // the same records with each node's children adjacent
     DBuf<uint8_t> rank;     // index of a node among its parent's children
     DBuf<uint32_t> cbase;   // exclusive scan of nchild (children of x at 1 + cbase[x])
+    DBuf<uint32_t> slot;    // position of each node in the grouped layout
+    DBuf<int32_t> kid;      // node at each position of the grouped layout (children of x: 1 + cbase[x] ...)
     DBuf<uint32_t> cblk, ctot;
+    DBuf<int32_t> roots, top;  // subtree roots and the nodes above them (bottom-up pass)
+    DBuf<uint32_t> nroots;     // their counts
     DBuf<int32_t> fstack;   // frontier walk: one node stack per wave
     DBuf<int> werr;         // frontier walk: a stack would have overflowed
     DBuf<uint32_t> gcost;   // interactions of each 64-body group in the last walk
@@ -90,8 +100,7 @@ struct BHWork {
     int64_t gcost_n = 0;
     DBuf<unsigned long long> icount2;  // count of the gated wave walk
     int bucket = 32;        // rejected nodes with at most this many bodies are summed directly
-    DBuf<float4> pos32s;    // sorted body positions (FP32) and G*m
-    DBuf<double> accs;      // sorted accelerations, SoA
+    DBuf<float4> pos32s;    // body positions (FP32) and G*m in tree order
     DBuf<unsigned long long> icount;
     RadixSorter sorter;
     int64_t nodes = 0;
@@ -253,30 +262,29 @@ __device__ __forceinline__ int shared_digits(uint64_t h0, uint64_t l0, uint64_t 
 }
 
 // ---- 4. shared digits, node counts ----
-__global__ void k_lcp(int64_t n, const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo,
-                      const uint32_t *__restrict__ idx, const double *__restrict__ x, uint8_t *__restrict__ c,
-                      int *__restrict__ err) {
+// c[s] = octant digits shared by sorted neighbours s and s+1 (0 for the
+// last); the nodes starting at s are the levels (c[s-1], c[s]] plus the leaf
+// of s, so cnt[s] = 1 + max(0, c[s] - c[s-1]).  Each thread also recomputes
+// c[s-1] (the same function of the same keys) so that one launch does both.
+__global__ void k_lcp_counts(int64_t n, const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo,
+                             const uint32_t *__restrict__ idx, const double *__restrict__ x, uint8_t *__restrict__ c,
+                             uint32_t *__restrict__ cnt, int *__restrict__ err) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
-    if (s == n - 1) {
-        c[s] = 0;
-        return;
+    int cs = 0;
+    if (s < n - 1) {
+        cs = shared_digits(khi[s], klo[s], khi[s + 1], klo[s + 1]);
+        if (cs >= kLevels) {
+            const uint32_t a = idx[s], b = idx[s + 1];
+            const bool same = x[a] == x[b] && x[n + a] == x[n + b] && x[2 * n + a] == x[2 * n + b];
+            atomicMax(err, same ? BH_ERR_INDIST : BH_ERR_DEPTH);  // indistinguishable_bodies (greg8tree.hpp:171)
+            cs = kLevels - 1;
+        }
     }
-    int d = shared_digits(khi[s], klo[s], khi[s + 1], klo[s + 1]);
-    if (d >= kLevels) {
-        uint32_t a = idx[s], b = idx[s + 1];
-        bool same = x[a] == x[b] && x[n + a] == x[n + b] && x[2 * n + a] == x[2 * n + b];
-        atomicMax(err, same ? BH_ERR_INDIST : BH_ERR_DEPTH);  // indistinguishable_bodies (greg8tree.hpp:171)
-        d = kLevels - 1;
-    }
-    c[s] = (uint8_t)d;
-}
-
-__global__ void k_node_counts(int64_t n, const uint8_t *__restrict__ c, uint32_t *__restrict__ cnt) {
-    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    int prev = s > 0 ? c[s - 1] : 0;
-    cnt[s] = 1u + (uint32_t)max(0, (int)c[s] - prev);
+    int cp = 0;
+    if (s > 0) cp = min(kLevels - 1, shared_digits(khi[s - 1], klo[s - 1], khi[s], klo[s]));
+    c[s] = (uint8_t)cs;
+    cnt[s] = 1u + (uint32_t)max(0, cs - cp);
 }
 
 // Exclusive scan in three launches (block scan, scan of block sums, add).
@@ -325,33 +333,83 @@ __global__ __launch_bounds__(1024) void k_scan_add(int64_t n, uint32_t *__restri
     if (i < n) a[i] += blk[blockIdx.x];
 }
 
-// ---- sparse table of range minima over c ----
-__global__ void k_st_level(int64_t n, const uint8_t *__restrict__ prev, uint8_t *__restrict__ cur, int64_t half) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint8_t a = prev[i];
-    uint8_t b = i + half < n ? prev[i + half] : 0;
-    cur[i] = a < b ? a : b;
-}
-
 __device__ __forceinline__ unsigned digit_at(uint64_t hi, uint64_t lo, int level) {
     return level <= kHiLevels ? (unsigned)(hi >> (3 * (kHiLevels - level))) & 7u
                               : (unsigned)(lo >> (3 * (kLevels - level))) & 7u;
 }
 
-// ---- nodes in DFS pre-order ----
-__global__ void k_nodes(int64_t n, int logn, const uint8_t *__restrict__ st, const uint32_t *__restrict__ base,
-                        const uint32_t *__restrict__ total_p, const uint64_t *__restrict__ khi,
-                        const uint64_t *__restrict__ klo, const uint32_t *__restrict__ idx,
-                        int32_t *__restrict__ first, int32_t *__restrict__ last, int32_t *__restrict__ skip,
-                        int32_t *__restrict__ body, uint8_t *__restrict__ depth, int8_t *__restrict__ octant) {
+// Over sorted keys the digits shared by positions a < b are the minimum of c
+// over [a, b), so the bodies sharing at least L digits with s form one run.
+// Its last position, searched in [s, ub] (ub: a position known to lie at or
+// past the end), by doubling steps then bisection:
+__device__ int64_t run_end(const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo, int64_t s, int L,
+                           int64_t ub) {
+    const uint64_t h = khi[s], l = klo[s];
+    int64_t good = s, bad = ub + 1, step = 1;
+    while (good + step < bad) {
+        const int64_t j = good + step;
+        if (shared_digits(h, l, khi[j], klo[j]) >= L) {
+            good = j;
+            step <<= 1;
+        } else {
+            bad = j;
+            break;
+        }
+    }
+    while (bad - good > 1) {
+        const int64_t mid = good + (bad - good) / 2;
+        if (shared_digits(h, l, khi[mid], klo[mid]) >= L) good = mid;
+        else bad = mid;
+    }
+    return good;
+}
+// ---- nodes in DFS pre-order, and the bodies moved into tree order ----
+// Node numbering: the root is 0 and the nodes starting at sorted position s
+// are 1 + base[s] + j, internal levels (c[s-1], c[s]] first, then the leaf
+// of s, so an exclusive scan of cnt numbers every node in pre-order with
+// octants 0..7 (the order cell_iterator visits them).  One thread per s
+// writes the nodes starting at s:
+//  * first = s, last = the end of the run sharing L digits (a node's bodies
+//    are contiguous), skip = the first node starting at last + 1 (the next
+//    node outside its subtree);
+//  * depth, octant, side (top.x - btm.x replayed by set_corners' halving on
+//    x along the key's digits);
+//  * for the leaf: body = s and the leaf's mass and COM.
+// The bodies themselves move into tree order in the same pass (every
+// per-body array, gathered through idx), with the FP32 (x, y, z, G m)
+// stream of the walk; idx becomes the identity.  Between steps the bodies
+// stay in the previous step's tree order, so these gathers read nearly
+// sequential addresses.
+__global__ __launch_bounds__(kBlock) void k_build_nodes(
+    int64_t n, const uint8_t *__restrict__ c, const uint32_t *__restrict__ base, const uint32_t *__restrict__ total_p,
+    const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo, const double *__restrict__ box,
+    uint32_t *__restrict__ idx, const double *__restrict__ x, const double *__restrict__ v,
+    const double *__restrict__ m, const double *__restrict__ r, const double *__restrict__ e,
+    const int32_t *__restrict__ perm, double *__restrict__ xo, double *__restrict__ vo, double *__restrict__ mo,
+    double *__restrict__ ro, double *__restrict__ eo, int32_t *__restrict__ permo, float4 *__restrict__ pos32,
+    double G, int32_t *__restrict__ first, int32_t *__restrict__ last, int32_t *__restrict__ skip,
+    int32_t *__restrict__ body, uint8_t *__restrict__ depth,
+    int8_t *__restrict__ octant, double *__restrict__ side, double *__restrict__ mass, double *__restrict__ com,
+    int64_t cstride) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
-    const uint8_t *c = st;  // level 0 of the sparse table
+    // bodies into tree order
+    const uint32_t b = idx[s];
+    const double px = x[b], py = x[n + b], pz = x[2 * n + b], pm = m[b];
+    xo[s] = px; xo[n + s] = py; xo[2 * n + s] = pz;
+    vo[s] = v[b]; vo[n + s] = v[n + b]; vo[2 * n + s] = v[2 * n + b];
+    mo[s] = pm;
+    ro[s] = r[b];
+    eo[s] = e[b];
+    permo[s] = perm ? perm[b] : (int32_t)b;
+    pos32[s] = make_float4((float)px, (float)py, (float)pz, (float)(G * pm));
+    idx[s] = (uint32_t)s;
+    // nodes starting at s
     const int32_t total = (int32_t)*total_p;
-    const int cs = c[s];
-    const int cp = s > 0 ? c[s - 1] : 0;
+    const int cs = c[s], cp = s > 0 ? c[s - 1] : 0;
     const uint64_t h = khi[s], l = klo[s];
+    const int32_t n0 = 1 + (int32_t)base[s];
+    double lo = box[0], hi = box[3];
     if (s == 0) {
         first[0] = 0;
         last[0] = (int32_t)(n - 1);
@@ -359,40 +417,59 @@ __global__ void k_nodes(int64_t n, int logn, const uint8_t *__restrict__ st, con
         body[0] = -1;
         depth[0] = 0;
         octant[0] = -1;
+        side[0] = hi - lo;
     }
-    int32_t node = 1 + (int32_t)base[s];
-    for (int L = cp + 1; L <= cs; ++L, ++node) {
-        // last position: first e >= s with c[e] < L (c[n-1] = 0)
-        int64_t e = s;
-        for (int k = logn - 1; k >= 0; --k) {
-            int64_t len = (int64_t)1 << k;
-            if (e + len - 1 <= n - 1 && st[(size_t)k * n + e] >= L) e += len;
+    int lev = 0;
+    int64_t ub = n - 1;
+    for (int L = cp + 1; L <= cs; ++L) {
+        for (; lev < L; ++lev) {
+            const double mid = (lo + hi) * 0.5;
+            const bool up = digit_at(h, l, lev + 1) & 1u;
+            lo = up ? mid : lo;
+            hi = up ? hi : mid;
         }
+        const int32_t node = n0 + (L - cp - 1);
+        const int64_t end = run_end(khi, klo, s, L, ub);
+        ub = end;
         first[node] = (int32_t)s;
-        last[node] = (int32_t)e;
-        skip[node] = e >= n - 1 ? total : 1 + (int32_t)base[e + 1];
+        last[node] = (int32_t)end;
+        skip[node] = end >= n - 1 ? total : 1 + (int32_t)base[end + 1];
         body[node] = -1;
         depth[node] = (uint8_t)L;
         octant[node] = (int8_t)digit_at(h, l, L);
+        side[node] = hi - lo;
     }
-    const int ld = (cs > cp ? cs : cp) + 1;
-    first[node] = (int32_t)s;
-    last[node] = (int32_t)s;
-    skip[node] = node + 1;
-    body[node] = (int32_t)idx[s];
-    depth[node] = (uint8_t)ld;
-    octant[node] = (int8_t)digit_at(h, l, ld);
+    const int ld = max(cs, cp) + 1;
+    for (; lev < ld; ++lev) {
+        const double mid = (lo + hi) * 0.5;
+        const bool up = digit_at(h, l, lev + 1) & 1u;
+        lo = up ? mid : lo;
+        hi = up ? hi : mid;
+    }
+    const int32_t leaf = n0 + max(0, cs - cp);
+    first[leaf] = (int32_t)s;
+    last[leaf] = (int32_t)s;
+    skip[leaf] = leaf + 1;
+    body[leaf] = (int32_t)s;
+    depth[leaf] = (uint8_t)ld;
+    octant[leaf] = (int8_t)digit_at(h, l, ld);
+    side[leaf] = hi - lo;
+    mass[leaf] = pm;
+    com[leaf] = px;
+    com[cstride + leaf] = py;
+    com[2 * cstride + leaf] = pz;
 }
 
+// Children of each internal node, in octant order along the skip chain:
+// their count, each child's index among its siblings, and its parent.
 __global__ void k_children(int32_t total, const int32_t *__restrict__ skip, const int32_t *__restrict__ body,
-                           int32_t *__restrict__ parent, uint8_t *__restrict__ nchild, uint32_t *__restrict__ ready,
-                           uint8_t *__restrict__ rank, uint32_t *__restrict__ cbase) {
+                           uint8_t *__restrict__ nchild, uint8_t *__restrict__ rank, uint32_t *__restrict__ cbase,
+                           int32_t *__restrict__ parent) {
     const int32_t x = blockIdx.x * kBlock + threadIdx.x;
     if (x >= total) return;
-    ready[x] = 0;
     if (x == 0) {
-        parent[0] = -1;
         rank[0] = 0;
+        parent[0] = -1;
     }
     if (body[x] >= 0) {
         nchild[x] = 0;
@@ -400,34 +477,15 @@ __global__ void k_children(int32_t total, const int32_t *__restrict__ skip, cons
         return;
     }
     int32_t ch = x + 1, k = 0;
-    const int32_t end = skip[x];
-    while (ch < end) {
-        parent[ch] = x;
+    const int32_t end = min(skip[x], total);
+    while (ch < end && k < 8) {  // an octree node has at most 8 children
         rank[ch] = (uint8_t)k;
-        ch = skip[ch];
+        parent[ch] = x;
+        ch = max(skip[ch], ch + 1);
         ++k;
     }
     nchild[x] = (uint8_t)k;
     cbase[x] = (uint32_t)k;  // scanned into the first child's slot
-}
-
-// side = top.x - btm.x of the node, replaying set_corners' recursion on x.
-__global__ void k_side(int32_t total, const double *__restrict__ box, const int32_t *__restrict__ first,
-                       const uint8_t *__restrict__ depth, const uint64_t *__restrict__ khi,
-                       const uint64_t *__restrict__ klo, double *__restrict__ side) {
-    const int32_t x = blockIdx.x * kBlock + threadIdx.x;
-    if (x >= total) return;
-    const int32_t s = first[x];
-    const uint64_t h = khi[s], l = klo[s];
-    double lo = box[0], hi = box[3];
-    const int dep = depth[x];
-    for (int L = 1; L <= dep; ++L) {
-        double mid = (lo + hi) * 0.5;
-        bool up = digit_at(h, l, L) & 1u;
-        lo = up ? mid : lo;
-        hi = up ? hi : mid;
-    }
-    side[x] = hi - lo;
 }
 
 // ---- collision pass on the tree: system::bh_s_coll (gregsys.hpp:631-686) ----
@@ -743,131 +801,263 @@ __global__ void k_bhc_replay(int64_t n, uint32_t nent, const uint64_t *__restric
     }
 }
 
-// ---- 5. mass and COM bottom up, one launch per depth (deepest first) ----
-__global__ void k_com_leaves(int64_t n, int32_t total, const uint32_t *__restrict__ base,
-                             const uint8_t *__restrict__ c, const int32_t *__restrict__ body,
-                             const double *__restrict__ m, const double *__restrict__ x, double *__restrict__ mass,
-                             double *__restrict__ com) {
-    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    const int cs = c[s], cp = s > 0 ? c[s - 1] : 0;
-    const int32_t leaf = 1 + (int32_t)base[s] + max(0, cs - cp);
-    const int32_t b = body[leaf];
-    mass[leaf] = m[b];
-    for (int k = 0; k < 3; ++k) com[(size_t)k * total + leaf] = x[k * n + b];
-}
-
-// Nodes grouped by depth: key = depth for internal nodes, 63 for leaves.
-__global__ void k_level_keys(int32_t total, const uint8_t *__restrict__ depth, const int32_t *__restrict__ body,
-                             uint64_t *__restrict__ key, uint32_t *__restrict__ node) {
-    const int32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= total) return;
-    key[p] = body[p] >= 0 ? 63u : depth[p];
-    node[p] = (uint32_t)p;
-}
-
-// off[L] = first sorted position with depth >= L, for L = 0..kLevels.
-__global__ void k_level_bounds(int32_t total, const uint64_t *__restrict__ key, uint32_t *__restrict__ off) {
-    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= total) return;
-    const int prev = i == 0 ? -1 : (int)key[i - 1];
-    const int cur = (int)key[i];
-    for (int L = prev + 1; L <= cur && L <= kLevels; ++L) off[L] = (uint32_t)i;
-    if (i == total - 1)
-        for (int L = cur + 1; L <= kLevels; ++L) off[L] = (uint32_t)total;
-}
-
-// calc_com (greg8tree.hpp:211-282) for internal node p from its children,
-// summed in reverse octant order (:262-266).
-__device__ __forceinline__ void com_node(int32_t total, int32_t p, const int32_t *__restrict__ skip,
-                                         double *__restrict__ mass, double *__restrict__ com) {
-    int32_t kids[8];
-    int nk = 0;
-    for (int32_t ch = p + 1; ch < skip[p] && nk < 8; ch = skip[ch]) kids[nk++] = ch;
-    double ms = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int q = nk - 1; q >= 0; --q) {  // reverse octant order (calc_com :262-266)
-        const int32_t ch = kids[q];
-        const double mc = mass[ch];
-        ms += mc;
-        sx += mc * com[ch];
-        sy += mc * com[(size_t)total + ch];
-        sz += mc * com[2 * (size_t)total + ch];
-    }
-    mass[p] = ms;
-    com[p] = sx / ms;
-    com[(size_t)total + p] = sy / ms;
-    com[2 * (size_t)total + p] = sz / ms;
-}
-
-// One depth: the nodes list[off[L] .. off[L+1]) over a fixed grid.
-__global__ void k_com_level(int32_t total, int level, const uint32_t *__restrict__ off,
-                            const uint32_t *__restrict__ list, const int32_t *__restrict__ skip,
-                            double *__restrict__ mass, double *__restrict__ com) {
-    const uint32_t i0 = off[level], i1 = off[level + 1];
-    for (uint32_t i = i0 + blockIdx.x * kBlock + threadIdx.x; i < i1; i += gridDim.x * kBlock)
-        com_node(total, (int32_t)list[i], skip, mass, com);
-}
-
-// The deep, sparsely populated depths kLevels-1 .. lo in one workgroup, a
-// barrier between depths (one launch instead of one per depth).
-__global__ __launch_bounds__(1024) void k_com_deep(int32_t total, int lo, const uint32_t *__restrict__ off,
-                                                   const uint32_t *__restrict__ list,
-                                                   const int32_t *__restrict__ skip, double *__restrict__ mass,
-                                                   double *__restrict__ com) {
-    for (int L = kLevels - 1; L >= lo; --L) {
-        const uint32_t i0 = off[L], i1 = off[L + 1];
-        for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) com_node(total, (int32_t)list[i], skip, mass, com);
-        __syncthreads();
-    }
-}
-
-// Walk record per node: q = (com, G*mass) in FP32 and r = (skip, first
+// ---- 5. mass and COM bottom up ----
+// Walk records of node x: q = (com, G*mass) in FP32 and r = (skip, first
 // sorted body, body count, r^2_open as float bits; -1 for leaves).  The
 // grouped copy puts the root at 0 and the children of y at 1 + cbase[y] +
 // rank, in octant order, with r = (first child's slot, first sorted body,
 // min(count, 2^27 - 1) << 4 | child count, r^2_open).
-__global__ void k_walk_prep(int32_t total, const double *__restrict__ mass, const double *__restrict__ com,
-                            const double *__restrict__ side, const int32_t *__restrict__ body,
-                            const int32_t *__restrict__ skip, const int32_t *__restrict__ first,
-                            const int32_t *__restrict__ last, double G, double theta,
-                            WalkNode *__restrict__ wnode, const int32_t *__restrict__ parent,
-                            const uint8_t *__restrict__ rank, const uint8_t *__restrict__ nchild,
-                            const uint32_t *__restrict__ cbase, WalkNode *__restrict__ wgrp) {
-    const int32_t x = blockIdx.x * kBlock + threadIdx.x;
-    if (x >= total) return;
+struct WalkRecs {
+    WalkNode *wnode, *wgrp;
+    const uint32_t *slot, *cbase;
+    const uint8_t *nchild;
+    double G, theta;
+    int32_t total;
+};
+__device__ void write_walk_recs(const WalkRecs &W, int32_t x, int64_t cstride, const double *__restrict__ mass,
+                                const double *__restrict__ com, const double *__restrict__ side,
+                                const int32_t *__restrict__ body, const int32_t *__restrict__ skip,
+                                const int32_t *__restrict__ first, const int32_t *__restrict__ last) {
     WalkNode w;
-    w.q = make_float4((float)com[x], (float)com[(size_t)total + x], (float)com[2 * (size_t)total + x],
-                      (float)(G * mass[x]));
+    w.q = make_float4((float)com[x], (float)com[cstride + x], (float)com[2 * cstride + x], (float)(W.G * mass[x]));
     float r2o;
     if (body[x] >= 0) r2o = -1.0f;
-    else if (theta > 0) {
-        double r = side[x] / theta;
+    else if (W.theta > 0) {
+        const double r = side[x] / W.theta;
         r2o = (float)(r * r);
     } else r2o = INFINITY;
     w.r = make_int4(skip[x], first[x], last[x] - first[x] + 1, __float_as_int(r2o));
-    wnode[x] = w;
-    if (!wgrp) return;
-    const int32_t at = x == 0 ? 0 : 1 + (int32_t)cbase[parent[x]] + (int32_t)rank[x];
-    const int nc = nchild[x];
+    W.wnode[x] = w;
+    if (!W.wgrp) return;
+    const int nc = W.nchild[x];
     const int32_t cnt = min(last[x] - first[x] + 1, (1 << 27) - 1);
     WalkNode g;
     g.q = w.q;
-    g.r = make_int4(nc ? 1 + (int32_t)cbase[x] : 0, first[x], (cnt << 4) | nc, __float_as_int(r2o));
-    wgrp[at] = g;
+    g.r = make_int4(nc ? 1 + (int32_t)W.cbase[x] : 0, first[x], (cnt << 4) | nc, __float_as_int(r2o));
+    const uint32_t at = W.slot[x];
+    if (at < (uint32_t)W.total) W.wgrp[at] = g;
 }
 
-__global__ void k_sorted_pos(int64_t n, const uint32_t *__restrict__ idx, const double *__restrict__ x,
-                             const double *__restrict__ m, double G, float4 *__restrict__ out) {
+// The bottom-up pass is split at subtrees of at most kSubtree nodes: one
+// workgroup sums such a subtree entirely in LDS, depth by depth under
+// workgroup barriers (its internal nodes sorted by depth with an LDS counting
+// sort), and one workgroup then sums the nodes above those subtrees.  Each
+// node's slot in the grouped walk layout (kid[] lists the children of x at
+// 1 + cbase[x] in octant order), the subtree roots (internal nodes of at
+// most kSubtree nodes under a parent of more) and the nodes above them are
+// found first; the leaves' walk records are written on the way.
+constexpr int32_t kSubtree = 1024;
+constexpr int kDepths = kLevels + 1;  // node depths 0..42
+constexpr uint32_t kTopLds = 8192;    // nodes above the subtrees sorted in LDS (else scanned per depth)
+
+__global__ void k_link(int32_t total, const int32_t *__restrict__ parent, const int32_t *__restrict__ skip,
+                       const uint8_t *__restrict__ rank, const uint32_t *__restrict__ cbase,
+                       uint32_t *__restrict__ slot, int32_t *__restrict__ kid, int32_t *__restrict__ roots,
+                       uint32_t *__restrict__ nroots, int32_t *__restrict__ top, uint32_t *__restrict__ ntop,
+                       WalkRecs W, int64_t cstride, const double *__restrict__ mass, const double *__restrict__ com,
+                       const double *__restrict__ side, const int32_t *__restrict__ body,
+                       const int32_t *__restrict__ first, const int32_t *__restrict__ last) {
+    const int32_t x = blockIdx.x * kBlock + threadIdx.x;
+    if (x >= total) return;
+    const int32_t p = parent[x];
+    const uint32_t at = x == 0 ? 0u : (p >= 0 && p < total ? 1u + cbase[p] + rank[x] : 0u);
+    slot[x] = at;
+    if (at < (uint32_t)total && (x == 0 || at > 0)) kid[at] = x;
+    if (body[x] >= 0) {
+        if (W.wnode) write_walk_recs(W, x, cstride, mass, com, side, body, skip, first, last);
+        return;
+    }
+    const int32_t size = skip[x] - x;
+    if (size > kSubtree) top[atomicAdd(ntop, 1u)] = x;
+    else if (x == 0 || p < 0 || p >= total || skip[p] - p > kSubtree) roots[atomicAdd(nroots, 1u)] = x;
+}
+
+// calc_com (greg8tree.hpp:211-282) for internal node p from its children
+// (kid[1 + cbase[p] ...], octant order), summed in reverse octant order
+// (:262-266), with the data in global memory.
+__device__ __forceinline__ void com_node_kids(int64_t cstride, int32_t p, const uint8_t *__restrict__ nchild,
+                                              const uint32_t *__restrict__ cbase, const int32_t *__restrict__ kid,
+                                              int32_t total, double *__restrict__ mass, double *__restrict__ com) {
+    const int nc = min((int)nchild[p], 8);
+    const uint32_t c0 = 1u + cbase[p];
+    double ms = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int k = nc - 1; k >= 0; --k) {
+        const int32_t ch = kid[c0 + k];
+        if (ch <= p || ch >= total) continue;
+        const double mc = mass[ch];
+        ms += mc;
+        sx += mc * com[ch];
+        sy += mc * com[cstride + ch];
+        sz += mc * com[2 * cstride + ch];
+    }
+    mass[p] = ms;
+    com[p] = sx / ms;
+    com[cstride + p] = sy / ms;
+    com[2 * cstride + p] = sz / ms;
+}
+
+__global__ __launch_bounds__(kBlock) void k_com_subtrees(
+    int32_t total, const int32_t *__restrict__ roots, const uint32_t *__restrict__ nroots,
+    const int32_t *__restrict__ skip, const int32_t *__restrict__ body, const uint8_t *__restrict__ depth,
+    const uint8_t *__restrict__ nchild, const uint32_t *__restrict__ cbase, const int32_t *__restrict__ kid,
+    double *__restrict__ mass, double *__restrict__ com, int64_t cstride, WalkRecs W, const double *__restrict__ side,
+    const int32_t *__restrict__ first, const int32_t *__restrict__ last) {
+    __shared__ double s_m[kSubtree], s_x[kSubtree], s_y[kSubtree], s_z[kSubtree];
+    __shared__ int16_t s_kid[kSubtree];   // children of the subtree's nodes, local indices, grouped layout order
+    __shared__ uint32_t s_list[kSubtree]; // internal nodes by depth: local index | first child << 11 | count << 22
+    __shared__ uint32_t s_cur[kDepths], s_off[kDepths + 1];
+    const uint32_t cnt = *nroots;
+    for (uint32_t q = blockIdx.x; q < cnt; q += gridDim.x) {
+        const int32_t r = roots[q];
+        const int32_t size = min(skip[r] - r, kSubtree);
+        const uint32_t cb0 = cbase[r];
+        for (int t = threadIdx.x; t < kDepths; t += kBlock) s_cur[t] = 0;
+        // the children of the subtree's nodes fill slots 1 + cbase[r] .. in the
+        // grouped layout (cbase is a prefix sum over pre-order, and every node
+        // of the subtree but its root is a child of one of its nodes)
+        for (int32_t t = threadIdx.x; t < size - 1; t += kBlock) {
+            const int32_t ch = kid[1 + cb0 + t] - r;
+            s_kid[t] = (int16_t)(ch > 0 && ch < size ? ch : 0);
+        }
+        __syncthreads();
+        // the leaves' mass and COM into LDS; internal nodes counted per depth
+        for (int32_t i = threadIdx.x; i < size; i += kBlock) {
+            const int32_t p = r + i;
+            if (body[p] >= 0) {
+                s_m[i] = mass[p];
+                s_x[i] = com[p];
+                s_y[i] = com[cstride + p];
+                s_z[i] = com[2 * cstride + p];
+            } else {
+                atomicAdd(&s_cur[min((int)depth[p], kDepths - 1)], 1u);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t run = 0;
+            for (int L = 0; L < kDepths; ++L) {
+                s_off[L] = run;
+                run += s_cur[L];
+                s_cur[L] = s_off[L];
+            }
+            s_off[kDepths] = run;
+        }
+        __syncthreads();
+        for (int32_t i = threadIdx.x; i < size; i += kBlock) {
+            const int32_t p = r + i;
+            if (body[p] < 0) {
+                const uint32_t c0 = cbase[p] - cb0, nc = min((uint32_t)nchild[p], 8u);
+                s_list[atomicAdd(&s_cur[min((int)depth[p], kDepths - 1)], 1u)] =
+                    (uint32_t)i | (min(c0, (uint32_t)kSubtree - 1u) << 11) | (nc << 22);
+            }
+        }
+        __syncthreads();
+        const int d0 = depth[r];
+        for (int L = kDepths - 1; L >= d0; --L) {
+            const uint32_t b = s_off[L], e = s_off[L + 1];
+            if (b == e) continue;  // the same for every thread (LDS values after a barrier)
+            for (uint32_t j = b + threadIdx.x; j < e; j += kBlock) {
+                const uint32_t rec = s_list[j];
+                const int32_t i = (int32_t)(rec & 2047u);
+                const int32_t c0 = (int32_t)((rec >> 11) & 2047u), nc = (int32_t)(rec >> 22);
+                double ms = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+                for (int k = nc - 1; k >= 0; --k) {  // reverse octant order (calc_com :262-266)
+                    const int32_t ch = s_kid[min(c0 + k, kSubtree - 1)];
+                    if (ch <= i) continue;
+                    const double mc = s_m[ch];
+                    ms += mc;
+                    sx += mc * s_x[ch];
+                    sy += mc * s_y[ch];
+                    sz += mc * s_z[ch];
+                }
+                s_m[i] = ms;
+                s_x[i] = sx / ms;
+                s_y[i] = sy / ms;
+                s_z[i] = sz / ms;
+            }
+            __syncthreads();  // this depth's sums before the next one up reads them
+        }
+        for (uint32_t j = threadIdx.x; j < s_off[kDepths]; j += kBlock) {
+            const int32_t i = (int32_t)(s_list[j] & 2047u), p = r + i;
+            mass[p] = s_m[i];
+            com[p] = s_x[i];
+            com[cstride + p] = s_y[i];
+            com[2 * cstride + p] = s_z[i];
+            if (W.wnode) write_walk_recs(W, p, cstride, mass, com, side, body, skip, first, last);
+        }
+        __syncthreads();  // LDS reused by the next subtree
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *__restrict__ top,
+                                                  const uint32_t *__restrict__ ntop, const int32_t *__restrict__ skip,
+                                                  const int32_t *__restrict__ body, const uint8_t *__restrict__ depth,
+                                                  const uint8_t *__restrict__ nchild, const uint32_t *__restrict__ cbase,
+                                                  const int32_t *__restrict__ kid, double *__restrict__ mass,
+                                                  double *__restrict__ com, int64_t cstride, WalkRecs W,
+                                                  const double *__restrict__ side, const int32_t *__restrict__ first,
+                                                  const int32_t *__restrict__ last) {
+    __shared__ int32_t s_list[kTopLds];
+    __shared__ uint32_t s_cur[kDepths], s_off[kDepths + 1];
+    const uint32_t cnt = *ntop;
+    const bool sorted = cnt <= kTopLds;
+    for (int t = threadIdx.x; t < kDepths; t += blockDim.x) s_cur[t] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x)
+        atomicAdd(&s_cur[min((int)depth[top[i]], kDepths - 1)], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int L = 0; L < kDepths; ++L) {
+            s_off[L] = run;
+            run += s_cur[L];
+            s_cur[L] = s_off[L];
+        }
+        s_off[kDepths] = run;
+    }
+    __syncthreads();
+    if (sorted)
+        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+            const int32_t p = top[i];
+            s_list[atomicAdd(&s_cur[min((int)depth[p], kDepths - 1)], 1u)] = p;
+        }
+    __syncthreads();
+    for (int L = kDepths - 1; L >= 0; --L) {
+        const uint32_t b = s_off[L], e = s_off[L + 1];
+        if (b == e) continue;  // the same for every thread
+        if (sorted) {
+            for (uint32_t j = b + threadIdx.x; j < e; j += blockDim.x) {
+                const int32_t p = s_list[j];
+                com_node_kids(cstride, p, nchild, cbase, kid, total, mass, com);
+                if (W.wnode) write_walk_recs(W, p, cstride, mass, com, side, body, skip, first, last);
+            }
+        } else {
+            for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+                const int32_t p = top[i];
+                if (depth[p] != L) continue;
+                com_node_kids(cstride, p, nchild, cbase, kid, total, mass, com);
+                if (W.wnode) write_walk_recs(W, p, cstride, mass, com, side, body, skip, first, last);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Positions, velocities (and accelerations when a walk wrote them) back in
+// input order, through perm (slot -> input index).
+__global__ void k_restore(int64_t n, const int32_t *__restrict__ perm, const double *__restrict__ x,
+                          const double *__restrict__ v, const double *__restrict__ a, double *__restrict__ xo,
+                          double *__restrict__ vo, double *__restrict__ ao) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
-    const uint32_t b = idx[s];
-    out[s] = make_float4((float)x[b], (float)x[n + b], (float)x[2 * n + b], (float)(G * m[b]));
+    const int64_t b = perm[s];
+    for (int k = 0; k < 3; ++k) {
+        xo[k * n + b] = x[k * n + s];
+        vo[k * n + b] = v[k * n + s];
+        if (a) ao[k * n + b] = a[k * n + s];
+    }
 }
 
-// Heaviest groups first within each XCD's range of `span` groups (one
-// workgroup per range): a counting sort of the previous walk's per-group
-// interactions into 64 bins.  The order within a bin is arbitrary; each
-// group's sums do not depend on it.
 __global__ __launch_bounds__(1024) void k_group_order(int64_t ngroups, int64_t span, const uint32_t *__restrict__ cost,
                                                       int32_t *__restrict__ order) {
     __shared__ uint32_t s_max;
@@ -909,7 +1099,8 @@ __global__ void k_frontier_reset(int *__restrict__ werr, unsigned long long *__r
 __global__ __launch_bounds__(kBlock) void k_walk_exact(int64_t n, int32_t total, const uint32_t *__restrict__ idx,
                                                        const uint32_t *__restrict__ base, const uint8_t *__restrict__ c,
                                                        const double *__restrict__ x, const double *__restrict__ mass,
-                                                       const double *__restrict__ com, const double *__restrict__ side,
+                                                       const double *__restrict__ com, int64_t cstride,
+                                                       const double *__restrict__ side,
                                                        const int32_t *__restrict__ body,
                                                        const int32_t *__restrict__ skip, double G, double eps2,
                                                        double theta, double *__restrict__ accs,
@@ -928,7 +1119,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_exact(int64_t n, int32_t total,
             node = node + 1;
             continue;
         }
-        const double dx = com[node] - px, dy = com[(size_t)total + node] - py, dz = com[2 * (size_t)total + node] - pz;
+        const double dx = com[node] - px, dy = com[cstride + node] - py, dz = com[2 * cstride + node] - pz;
         bool accept = body[node] >= 0;
         if (!accept) accept = side[node] / __builtin_sqrt(dx * dx + dy * dy + dz * dz) < theta;  // is_source_cell
         if (accept) {
@@ -950,15 +1141,6 @@ __global__ __launch_bounds__(kBlock) void k_walk_exact(int64_t n, int32_t total,
     atomicAdd(icount, cnt);
 }
 
-__global__ void k_unsort_acc(int64_t n, const uint32_t *__restrict__ idx, const double *__restrict__ accs,
-                             double *__restrict__ a) {
-    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t b = idx[s];
-    a[b] = accs[s];
-    a[n + b] = accs[n + s];
-    a[2 * n + b] = accs[2 * n + s];
-}
 
 inline unsigned blocks(int64_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -971,10 +1153,17 @@ void raise_bh_error(int e) {
         throw ApiError(NB_ERR_CAPACITY, "Barnes-Hut: two bodies share more than 42 octant levels");
 }
 
+template <typename T>
+void swap_buf(DBuf<T> &a, DBuf<T> &b) {
+    std::swap(a.p, b.p);
+    std::swap(a.n, b.n);
+}
+
 }  // namespace
 
-// Builds the octree of the current positions into s->bh (synchronises once to
-// size the node arrays and to surface build errors).
+// Builds the octree of the current positions into s->bh and leaves the bodies
+// in the tree's order (synchronises once, after the node count scan, to size
+// the node arrays and to surface build errors).
 static void bh_build(nb_system *s) {
     const int64_t n = s->n;
     if (!s->bh) s->bh = new BHWork();
@@ -984,14 +1173,18 @@ static void bh_build(nb_system *s) {
     if (n >= (int64_t)1 << 31) throw ApiError(NB_ERR_CAPACITY, "Barnes-Hut: too many bodies");
     ScopedTimer tm(s->prof, "bh_build", st);
     if (w.cap_n < n) {
-        w.khi.ensure(n); w.klo.ensure(n); w.klo_tmp.ensure(n); w.idx.ensure(n);
-        w.cnt.ensure(n); w.pos32s.ensure(n); w.accs.ensure(3 * (size_t)n);
-        w.st.ensure((size_t)kMaxLog * n);
+        w.khi.ensure(n); w.klo.ensure(n); w.klo_slot.ensure(n); w.idx.ensure(n);
+        w.cnt.ensure(n); w.pos32s.ensure(n); w.c.ensure(n);
         w.blk.ensure((size_t)(n + 1023) / 1024 + 1);
         w.sorter.ensure(n);
+        w.perm.ensure(n); w.perm_alt.ensure(n);
         w.cap_n = n;
     }
+    // the other copy of every per-body array, for the move into tree order
+    w.xa.ensure(3 * (size_t)n); w.va.ensure(3 * (size_t)n); w.aa.ensure(3 * (size_t)n);
+    for (DBuf<double> *b : {&w.mt, &w.rt, &w.et, &w.mt_alt, &w.rt_alt, &w.et_alt}) b->ensure((size_t)n);
     w.part.ensure(6 * 1024); w.box.ensure(6); w.err.ensure(1); w.total.ensure(1); w.icount.ensure(1);
+    w.nroots.ensure(2);
     if (!w.h_total) NB_HIP(hipHostMalloc((void **)&w.h_total, sizeof(uint32_t), hipHostMallocDefault));
     if (!w.h_err) NB_HIP(hipHostMalloc((void **)&w.h_err, sizeof(int), hipHostMallocDefault));
 
@@ -999,22 +1192,15 @@ static void bh_build(nb_system *s) {
     hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, st, n, s->x.p, w.part.p);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, st, nparts, w.part.p, w.box.p);
     NB_HIP(hipMemsetAsync(w.err.p, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kBlock), 0, st, n, s->x.p, w.box.p, w.khi.p, w.klo.p, w.idx.p,
-                       w.err.p);
-    w.sorter.sort(st, w.khi.p, w.idx.p, n, 3 * kHiLevels);
-    hipLaunchKernelGGL(k_gather_lo, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.klo.p, w.idx.p, w.klo_tmp.p);
-    std::swap(w.klo.p, w.klo_tmp.p);
-    hipLaunchKernelGGL(k_fix_ties, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p);
-    // sparse table level 0 is c itself
-    hipLaunchKernelGGL(k_lcp, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p, s->x.p, w.st.p,
-                       w.err.p);
-    int logn = 1;
-    while (((int64_t)1 << logn) <= n) ++logn;
-    for (int k = 1; k < logn; ++k)
-        hipLaunchKernelGGL(k_st_level, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.st.p + (size_t)(k - 1) * n,
-                           w.st.p + (size_t)k * n, (int64_t)1 << (k - 1));
-    hipLaunchKernelGGL(k_node_counts, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.st.p, w.cnt.p);
+    hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kBlock), 0, st, n, s->x.p, w.box.p, w.khi.p, w.klo_slot.p,
+                       w.idx.p, w.err.p);
+    // the sort by the high key, then ties by the low key, and the node counts
     const int nb = (int)((n + 1023) / 1024);
+    w.sorter.sort(st, w.khi.p, w.idx.p, n, 3 * kHiLevels);
+    hipLaunchKernelGGL(k_gather_lo, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.klo_slot.p, w.idx.p, w.klo.p);
+    hipLaunchKernelGGL(k_fix_ties, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p);
+    hipLaunchKernelGGL(k_lcp_counts, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p, s->x.p,
+                       w.c.p, w.cnt.p, w.err.p);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nb, w.blk.p, w.total.p);
     hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
@@ -1027,41 +1213,68 @@ static void bh_build(nb_system *s) {
         size_t t = (size_t)total + total / 4;
         w.first.ensure(t); w.last.ensure(t); w.skip.ensure(t); w.parent.ensure(t); w.body.ensure(t);
         w.depth.ensure(t); w.nchild.ensure(t); w.octant.ensure(t); w.mass.ensure(t); w.com.ensure(3 * t);
-        w.side.ensure(t); w.ready.ensure(t); w.wnode.ensure(t);
+        w.side.ensure(t); w.wnode.ensure(t); w.slot.ensure(t); w.kid.ensure(t); w.roots.ensure(t); w.top.ensure(t);
         w.wgrp.ensure(t); w.rank.ensure(t); w.cbase.ensure(t); w.cblk.ensure(t / 1024 + 2); w.ctot.ensure(1);
-        w.lvl_key.ensure(t); w.lvl_node.ensure(t); w.lvl_off.ensure(kLevels + 1);
-        w.sorter.ensure((int64_t)t);
         w.cap_nodes = (int64_t)t;
     }
     w.nodes = total;
-    hipLaunchKernelGGL(k_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, logn, w.st.p, w.cnt.p, w.total.p, w.khi.p,
-                       w.klo.p, w.idx.p, w.first.p, w.last.p, w.skip.p, w.body.p, w.depth.p, w.octant.p);
-    hipLaunchKernelGGL(k_children, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.skip.p, w.body.p, w.parent.p,
-                       w.nchild.p, w.ready.p, w.rank.p, w.cbase.p);
-    if (s->precision == NB_PREC_MIXED && s->tune_bh_walk != 2) {
-        // first-child slots of the grouped walk layout (frontier walk only)
-        const int nbn = (int)((total + 1023) / 1024);
-        hipLaunchKernelGGL(k_scan_blocks, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
-        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nbn, w.cblk.p, w.ctot.p);
-        hipLaunchKernelGGL(k_scan_add, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
-    }
-    hipLaunchKernelGGL(k_side, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.box.p, w.first.p, w.depth.p,
-                       w.khi.p, w.klo.p, w.side.p);
-    // com is SoA with stride `total`
-    hipLaunchKernelGGL(k_com_leaves, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.cnt.p, w.st.p, w.body.p,
-                       s->m.p, s->x.p, w.mass.p, w.com.p);
-    // internal nodes grouped by depth (one 8-bit radix pass), then one launch
-    // per depth that touches only that depth's nodes
-    hipLaunchKernelGGL(k_level_keys, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.depth.p, w.body.p,
-                       w.lvl_key.p, w.lvl_node.p);
-    w.sorter.sort(st, w.lvl_key.p, w.lvl_node.p, total, 6);
-    hipLaunchKernelGGL(k_level_bounds, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.lvl_key.p, w.lvl_off.p);
-    hipLaunchKernelGGL(k_com_deep, dim3(1), dim3(1024), 0, st, total, kComDeep, w.lvl_off.p, w.lvl_node.p, w.skip.p,
-                       w.mass.p, w.com.p);
-    for (int L = kComDeep - 1; L >= 0; --L)
-        hipLaunchKernelGGL(k_com_level, dim3(kComGrid), dim3(kBlock), 0, st, total, L, w.lvl_off.p, w.lvl_node.p,
-                           w.skip.p, w.mass.p, w.com.p);
+    const int64_t cs = w.cap_nodes;  // stride of the SoA com array
+    // nodes, and the bodies into tree order
+    hipLaunchKernelGGL(k_build_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.c.p, w.cnt.p, w.total.p, w.khi.p,
+                       w.klo.p, w.box.p, w.idx.p, s->x.p, s->v.p, w.permuted ? w.mt.p : s->m.p,
+                       w.permuted ? w.rt.p : s->r.p, w.permuted ? w.et.p : s->e.p, w.permuted ? w.perm.p : nullptr,
+                       w.xa.p, w.va.p, w.mt_alt.p, w.rt_alt.p, w.et_alt.p, w.perm_alt.p,
+                       w.pos32s.p, s->G, w.first.p, w.last.p, w.skip.p, w.body.p, w.depth.p, w.octant.p,
+                       w.side.p, w.mass.p, w.com.p, cs);
+    swap_buf(s->x, w.xa); swap_buf(s->v, w.va);
+    swap_buf(w.mt, w.mt_alt); swap_buf(w.rt, w.rt_alt); swap_buf(w.et, w.et_alt);
+    swap_buf(w.perm, w.perm_alt);
+    w.permuted = true;
+    w.acc_permuted = false;  // until a walk writes accelerations in tree order
+    // children, their slots in the grouped walk layout, subtree lists
+    hipLaunchKernelGGL(k_children, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.skip.p, w.body.p,
+                       w.nchild.p, w.rank.p, w.cbase.p, w.parent.p);
+    const int nbn = (int)((total + 1023) / 1024);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nbn, w.cblk.p, w.ctot.p);
+    hipLaunchKernelGGL(k_scan_add, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
+    NB_HIP(hipMemsetAsync(w.nroots.p, 0, 2 * sizeof(uint32_t), st));
+    const bool records = s->precision == NB_PREC_MIXED;
+    const WalkRecs W{records ? w.wnode.p : nullptr, records && s->tune_bh_walk != 2 ? w.wgrp.p : nullptr, w.slot.p,
+                     w.cbase.p, w.nchild.p, s->G, s->theta, total};
+    hipLaunchKernelGGL(k_link, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.parent.p, w.skip.p, w.rank.p,
+                       w.cbase.p, w.slot.p, w.kid.p, w.roots.p, w.nroots.p, w.top.p, w.nroots.p + 1, W, cs, w.mass.p,
+                       w.com.p, w.side.p, w.body.p, w.first.p, w.last.p);
+    // mass, COM and the internal nodes' walk records, bottom up
+    const unsigned sub_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, total / 128 + 1));
+    hipLaunchKernelGGL(k_com_subtrees, dim3(sub_grid), dim3(kBlock), 0, st, total, w.roots.p, w.nroots.p, w.skip.p,
+                       w.body.p, w.depth.p, w.nchild.p, w.cbase.p, w.kid.p, w.mass.p, w.com.p, cs, W, w.side.p,
+                       w.first.p, w.last.p);
+    hipLaunchKernelGGL(k_com_top, dim3(1), dim3(1024), 0, st, total, w.top.p, w.nroots.p + 1, w.skip.p, w.body.p,
+                       w.depth.p, w.nchild.p, w.cbase.p, w.kid.p, w.mass.p, w.com.p, cs, W, w.side.p, w.first.p,
+                       w.last.p);
     NB_HIP(hipGetLastError());
+}
+
+// The bodies back in their input order after Barnes-Hut steps (no-op when
+// they already are).
+void bh_restore_order(nb_system *s) {
+    if (!s->bh || !s->bh->permuted) return;
+    BHWork &w = *s->bh;
+    const int64_t n = s->n;
+    const bool acc = w.acc_permuted;
+    hipLaunchKernelGGL(k_restore, dim3(blocks(n)), dim3(kBlock), 0, s->stream, n, w.perm.p, s->x.p, s->v.p,
+                       acc ? s->a.p : nullptr, w.xa.p, w.va.p, w.aa.p);
+    NB_HIP(hipGetLastError());
+    swap_buf(s->x, w.xa); swap_buf(s->v, w.va);
+    if (acc) swap_buf(s->a, w.aa);
+    w.permuted = false;
+    w.acc_permuted = false;
+}
+
+// Forgets the tree order (the bodies were replaced).
+void bh_forget_order(nb_system *s) {
+    if (s->bh) s->bh->permuted = false;
 }
 
 void bh_forces(nb_system *s, int64_t *interactions) {
@@ -1070,23 +1283,18 @@ void bh_forces(nb_system *s, int64_t *interactions) {
     BHWork &w = *s->bh;
     hipStream_t st = s->stream;
     const int32_t total = (int32_t)w.nodes;
+    const int64_t cs = w.cap_nodes;
     NB_HIP(hipMemsetAsync(w.icount.p, 0, sizeof(unsigned long long), st));
     bool frontier = false;
     {
         ScopedTimer tm(s->prof, "bh_walk", st);
         if (s->precision == NB_PREC_MIXED) {
-            hipLaunchKernelGGL(k_walk_prep, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.mass.p, w.com.p,
-                               w.side.p, w.body.p, w.skip.p, w.first.p, w.last.p, s->G, s->theta, w.wnode.p,
-                               w.parent.p, w.rank.p, w.nchild.p, w.cbase.p,
-                               s->tune_bh_walk != 2 ? w.wgrp.p : nullptr);
-            hipLaunchKernelGGL(k_sorted_pos, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.idx.p, s->x.p, s->m.p, s->G,
-                               w.pos32s.p);
             const int bucket = s->tune_bh_bucket > 0 ? s->tune_bh_bucket : w.bucket;
             const float eps2 = s->eps > 0 ? (float)(s->eps * s->eps) : 0.0f;
             auto wave = s->eps > 0 ? k_walk_wave<true> : k_walk_wave<false>;
             if (s->tune_bh_walk == 2) {
                 hipLaunchKernelGGL(wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wnode.p,
-                                   bucket, eps2, w.accs.p, w.icount.p, nullptr);
+                                   bucket, eps2, s->a.p, w.icount.p, nullptr);
             } else {
                 // one wave per group; a multiple of 8 workgroups so that
                 // each XCD gets one range of `span` groups
@@ -1106,20 +1314,20 @@ void bh_forces(nb_system *s, int64_t *interactions) {
                 hipLaunchKernelGGL(k_group_order, dim3(8), dim3(1024), 0, st, ngroups, span, w.gcost.p, w.gorder.p);
                 auto front = s->eps > 0 ? k_walk_frontier<true> : k_walk_frontier<false>;
                 hipLaunchKernelGGL(front, dim3((unsigned)G), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wgrp.p,
-                                   bucket, eps2, w.accs.p, w.icount.p, w.fstack.p, cap, w.gorder.p, w.gcost.p,
+                                   bucket, eps2, s->a.p, w.icount.p, w.fstack.p, cap, w.gorder.p, w.gcost.p,
                                    w.werr.p);
                 hipLaunchKernelGGL(wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wnode.p,
-                                   bucket, eps2, w.accs.p, w.icount2.p, w.werr.p);
+                                   bucket, eps2, s->a.p, w.icount2.p, w.werr.p);
                 frontier = true;
             }
         } else {
-            hipLaunchKernelGGL(k_walk_exact, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.cnt.p, w.st.p,
-                               s->x.p, w.mass.p, w.com.p, w.side.p, w.body.p, w.skip.p, s->G, s->eps * s->eps,
-                               s->theta, w.accs.p, w.icount.p);
+            hipLaunchKernelGGL(k_walk_exact, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.cnt.p, w.c.p,
+                               s->x.p, w.mass.p, w.com.p, cs, w.side.p, w.body.p, w.skip.p, s->G, s->eps * s->eps,
+                               s->theta, s->a.p, w.icount.p);
         }
     }
-    hipLaunchKernelGGL(k_unsort_acc, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.idx.p, w.accs.p, s->a.p);
     NB_HIP(hipGetLastError());
+    w.acc_permuted = true;
     if (interactions) {
         // after a frontier overflow the gated wave walk's count is the one
         unsigned long long c = 0, c2 = 0;
@@ -1135,10 +1343,6 @@ void bh_forces(nb_system *s, int64_t *interactions) {
     s->acc_valid = true;
 }
 
-// bh_s_coll (gregsys.hpp:631-686) on the tree this step built: overlaps of
-// touching leaves counted per body in parallel, then (when there are any)
-// written in leaf order and bounced in order by one lane.  Synchronises once
-// to size the overlap list.
 static void bh_collision_pass(nb_system *s) {
     BHWork &w = *s->bh;
     const int64_t n = s->n;
@@ -1156,11 +1360,11 @@ static void bh_collision_pass(nb_system *s) {
                        w.khi.p, w.klo.p, w.nbox.p);
     hipLaunchKernelGGL(k_leaf_of, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.body.p, w.first.p, w.leaf_of.p);
     NB_HIP(hipMemsetAsync(w.rmax.p, 0, sizeof(double), st));
-    hipLaunchKernelGGL(k_rmax, dim3(std::min<unsigned>(1024u, blocks(n))), dim3(kBlock), 0, st, n, s->r.p, w.rmax.p);
+    hipLaunchKernelGGL(k_rmax, dim3(std::min<unsigned>(1024u, blocks(n))), dim3(kBlock), 0, st, n, w.rt.p, w.rmax.p);
     w.c_nb.ensure((size_t)kCollCache * (size_t)w.cap_n);
     w.c_geo.ensure((size_t)4 * kCollCache * (size_t)w.cap_n);
     hipLaunchKernelGGL(k_bhc_scan<false>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.leaf_of.p,
-                       w.nbox.p, w.skip.p, w.body.p, w.parent.p, s->x.p, s->r.p, w.rmax.p, w.bc_cnt.p, 0u, nullptr, nullptr,
+                       w.nbox.p, w.skip.p, w.body.p, w.parent.p, s->x.p, w.rt.p, w.rmax.p, w.bc_cnt.p, 0u, nullptr, nullptr,
                        nullptr, w.c_nb.p, w.c_geo.p);
     const int nb = (int)((n + 1023) / 1024);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.bc_cnt.p, w.bc_blk.p);
@@ -1175,7 +1379,7 @@ static void bh_collision_pass(nb_system *s) {
         w.e_own.ensure(cap); w.e_nb.ensure(cap); w.e_geo.ensure(4 * cap); w.e_mom.ensure(2 * cap);
     }
     hipLaunchKernelGGL(k_bhc_scan<true>, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.leaf_of.p,
-                       w.nbox.p, w.skip.p, w.body.p, w.parent.p, s->x.p, s->r.p, w.rmax.p, w.bc_cnt.p, nent, w.e_own.p,
+                       w.nbox.p, w.skip.p, w.body.p, w.parent.p, s->x.p, w.rt.p, w.rmax.p, w.bc_cnt.p, nent, w.e_own.p,
                        w.e_nb.p, w.e_geo.p, w.c_nb.p, w.c_geo.p);
     // components of the overlap graph, entries grouped by component in
     // (owner, visit) order, then one lane per component
@@ -1195,13 +1399,14 @@ static void bh_collision_pass(nb_system *s) {
     hipLaunchKernelGGL(k_cc_gather, dim3(eb), dim3(kBlock), 0, st, nent, w.cc_vals.p, w.e_own.p, w.e_nb.p, w.e_geo.p,
                        w.g_own.p, w.g_nb.p, w.g_geo.p);
     hipLaunchKernelGGL(k_bhc_replay, dim3(eb), dim3(kBlock), 0, st, n, nent, w.cc_keys.p, w.g_own.p, w.g_nb.p,
-                       w.g_geo.p, w.e_mom.p, s->m.p, s->e.p, s->v.p);
+                       w.g_geo.p, w.e_mom.p, w.mt.p, w.et.p, s->v.p);
     NB_HIP(hipGetLastError());
 }
 
 // evolve's KDK + Barnes-Hut loop body (gregsys.hpp:2914-2939): drift a full
 // step, rebuild, walk, kick a full step (velocities stay half a step ahead),
-// then bh_s_coll when collisions are on.
+// then bh_s_coll when collisions are on.  The bodies stay in tree order from
+// step to step; a record is taken in input order.
 void bh_steps(nb_system *s, uint64_t steps) {
     for (uint64_t k = 0; k < steps; ++k) {
         launch_drift(s->stream, s->n, s->v.p, s->x.p, s->dt);
@@ -1209,7 +1414,10 @@ void bh_steps(nb_system *s, uint64_t steps) {
         launch_kick(s->stream, s->n, s->a.p, s->v.p, s->dt);
         if (s->coll_mode == NB_OVERLAP_COLL_CHECK) bh_collision_pass(s);
         ++s->step_count;
-        if (record_due(s, s->step_count)) record_state(s, s->step_count);
+        if (record_due(s, s->step_count)) {
+            bh_restore_order(s);
+            record_state(s, s->step_count);
+        }
     }
 }
 
@@ -1217,29 +1425,35 @@ int bh_topology(nb_system *s, int64_t cap, int32_t *depth, int32_t *octant, int6
                 double *mass, double *com, double *side, int64_t *count) {
     bh_build(s);
     BHWork &w = *s->bh;
-    const int64_t t = w.nodes;
+    const int64_t t = w.nodes, n = s->n, cs = w.cap_nodes;
     *count = t;
-    if (cap < t) throw ApiError(NB_ERR_CAPACITY, "nb_tree_topology: capacity smaller than the node count");
-    std::vector<uint8_t> d8((size_t)t), nc((size_t)t);
+    if (cap < t) {
+        bh_restore_order(s);
+        throw ApiError(NB_ERR_CAPACITY, "nb_tree_topology: capacity smaller than the node count");
+    }
+    std::vector<uint8_t> d8((size_t)t);
     std::vector<int8_t> o8((size_t)t);
-    std::vector<int32_t> b32((size_t)t), p32((size_t)t);
-    std::vector<double> cm(3 * (size_t)t);
+    std::vector<int32_t> b32((size_t)t), p32((size_t)t), perm((size_t)n);
+    std::vector<double> cm(3 * (size_t)cs);
     hipStream_t st = s->stream;
     NB_HIP(hipMemcpyAsync(d8.data(), w.depth.p, (size_t)t, hipMemcpyDeviceToHost, st));
     NB_HIP(hipMemcpyAsync(o8.data(), w.octant.p, (size_t)t, hipMemcpyDeviceToHost, st));
     NB_HIP(hipMemcpyAsync(b32.data(), w.body.p, (size_t)t * 4, hipMemcpyDeviceToHost, st));
     NB_HIP(hipMemcpyAsync(p32.data(), w.parent.p, (size_t)t * 4, hipMemcpyDeviceToHost, st));
+    NB_HIP(hipMemcpyAsync(perm.data(), w.perm.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     if (mass) NB_HIP(hipMemcpyAsync(mass, w.mass.p, (size_t)t * 8, hipMemcpyDeviceToHost, st));
     if (side) NB_HIP(hipMemcpyAsync(side, w.side.p, (size_t)t * 8, hipMemcpyDeviceToHost, st));
-    NB_HIP(hipMemcpyAsync(cm.data(), w.com.p, (size_t)t * 24, hipMemcpyDeviceToHost, st));
+    NB_HIP(hipMemcpyAsync(cm.data(), w.com.p, (size_t)cs * 24, hipMemcpyDeviceToHost, st));
     NB_HIP(hipStreamSynchronize(st));
+    bh_restore_order(s);
     for (int64_t k = 0; k < t; ++k) {
         if (depth) depth[k] = d8[(size_t)k];
         if (octant) octant[k] = o8[(size_t)k];
-        if (body) body[k] = b32[(size_t)k];
+        // leaves name the body by its input index (the tree's slots are in tree order)
+        if (body) body[k] = b32[(size_t)k] >= 0 ? perm[(size_t)b32[(size_t)k]] : -1;
         if (parent) parent[k] = p32[(size_t)k];
         if (com)
-            for (int q = 0; q < 3; ++q) com[3 * k + q] = cm[(size_t)q * t + k];
+            for (int q = 0; q < 3; ++q) com[3 * k + q] = cm[(size_t)q * cs + k];
     }
     return NB_OK;
 }

@@ -99,6 +99,16 @@ void collision_pass(nb_system *s, uint64_t step);
 // Barnes-Hut (nb_bh.hip)
 void bh_forces(nb_system *s, int64_t *interactions);  // a(x) into s->a by the tree walk
 void bh_steps(nb_system *s, uint64_t steps);
+// Barnes-Hut steps leave the bodies in the tree's order; this puts them back
+// in input order (every entry point does so before it returns)
+void bh_restore_order(nb_system *s);
+void bh_forget_order(nb_system *s);  // the bodies were replaced
+inline void restore_order_quietly(nb_system *s) {
+    try {
+        bh_restore_order(s);
+    } catch (...) {
+    }
+}
 int bh_topology(nb_system *s, int64_t cap, int32_t *depth, int32_t *octant, int64_t *body, int64_t *parent,
                 double *mass, double *com, double *side, int64_t *count);
 // adjoint (nb_adjoint.hip)

@@ -195,13 +195,16 @@ int nb_autotune(nb_system *s, int method, const char *cache_path) {
         }
         s->prof.on = prof;
         s->acc_valid = false;  // accelerations were recomputed for timing only
+        bh_restore_order(s);   // the timed builds left the bodies in tree order
         NB_HIP(hipStreamSynchronize(st));
         return NB_OK;
     } catch (const ApiError &e) {
         s->err = e.what;
+        restore_order_quietly(s);
         return e.code;
     } catch (const DeviceError &e) {
         s->err = e.what;
+        restore_order_quietly(s);
         return NB_ERR_DEVICE;
     }
 }
