@@ -67,6 +67,7 @@ struct BHWork {
     DBuf<uint64_t> khi, klo, klo_slot;  // high keys and low keys in sorted order; low keys by body slot
     DBuf<uint32_t> idx;     // sorted position -> body slot (the identity once the bodies are in tree order)
     DBuf<uint8_t> c;        // shared digits of sorted neighbours
+    DBuf<uint8_t> cm1, cm2, cm3;  // minima of c over chunks of 64, 4096 and 262144 positions
     DBuf<uint32_t> cnt;     // nodes starting at s, then their exclusive scan
     DBuf<uint32_t> blk;     // scan block sums
     DBuf<uint32_t> total;   // node count
@@ -266,25 +267,34 @@ __device__ __forceinline__ int shared_digits(uint64_t h0, uint64_t l0, uint64_t 
 // last); the nodes starting at s are the levels (c[s-1], c[s]] plus the leaf
 // of s, so cnt[s] = 1 + max(0, c[s] - c[s-1]).  Each thread also recomputes
 // c[s-1] (the same function of the same keys) so that one launch does both.
-__global__ void k_lcp_counts(int64_t n, const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo,
-                             const uint32_t *__restrict__ idx, const double *__restrict__ x, uint8_t *__restrict__ c,
-                             uint32_t *__restrict__ cnt, int *__restrict__ err) {
+__global__ __launch_bounds__(kBlock) void k_lcp_counts(int64_t n, const uint64_t *__restrict__ khi,
+                                                       const uint64_t *__restrict__ klo, const uint32_t *__restrict__ idx,
+                                                       const double *__restrict__ x, uint8_t *__restrict__ c,
+                                                       uint8_t *__restrict__ cm1, uint32_t *__restrict__ cnt,
+                                                       int *__restrict__ err) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    int cs = 0;
-    if (s < n - 1) {
-        cs = shared_digits(khi[s], klo[s], khi[s + 1], klo[s + 1]);
-        if (cs >= kLevels) {
-            const uint32_t a = idx[s], b = idx[s + 1];
-            const bool same = x[a] == x[b] && x[n + a] == x[n + b] && x[2 * n + a] == x[2 * n + b];
-            atomicMax(err, same ? BH_ERR_INDIST : BH_ERR_DEPTH);  // indistinguishable_bodies (greg8tree.hpp:171)
-            cs = kLevels - 1;
+    int cs = 255;  // past the end: no effect on the minima below
+    if (s < n) {
+        cs = 0;
+        if (s < n - 1) {
+            cs = shared_digits(khi[s], klo[s], khi[s + 1], klo[s + 1]);
+            if (cs >= kLevels) {
+                const uint32_t a = idx[s], b = idx[s + 1];
+                const bool same = x[a] == x[b] && x[n + a] == x[n + b] && x[2 * n + a] == x[2 * n + b];
+                atomicMax(err, same ? BH_ERR_INDIST : BH_ERR_DEPTH);  // indistinguishable_bodies (greg8tree.hpp:171)
+                cs = kLevels - 1;
+            }
         }
+        int cp = 0;
+        if (s > 0) cp = min(kLevels - 1, shared_digits(khi[s - 1], klo[s - 1], khi[s], klo[s]));
+        c[s] = (uint8_t)cs;
+        cnt[s] = 1u + (uint32_t)max(0, cs - cp);
     }
-    int cp = 0;
-    if (s > 0) cp = min(kLevels - 1, shared_digits(khi[s - 1], klo[s - 1], khi[s], klo[s]));
-    c[s] = (uint8_t)cs;
-    cnt[s] = 1u + (uint32_t)max(0, cs - cp);
+    // minimum of c over each aligned chunk of 64 positions (one wave)
+    int m = cs;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0 && s < n) cm1[s >> 6] = (uint8_t)m;
 }
 
 // Exclusive scan in three launches (block scan, scan of block sums, add).
@@ -304,7 +314,12 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(int64_t n, uint32_t *__res
     if (threadIdx.x == 1023) blk[blockIdx.x] = sm[1023];
 }
 
-__global__ __launch_bounds__(1024) void k_scan_sums(int nb, uint32_t *__restrict__ blk, uint32_t *__restrict__ total) {
+// The scan of the block sums; with cm1 also the minima of c over chunks of
+// 4096 positions (cm2, from 64 entries of cm1 each) and of 262144 (cm3).
+__global__ __launch_bounds__(1024) void k_scan_sums(int nb, uint32_t *__restrict__ blk, uint32_t *__restrict__ total,
+                                                    const uint8_t *__restrict__ cm1 = nullptr, int64_t n1 = 0,
+                                                    uint8_t *__restrict__ cm2 = nullptr, int64_t n2 = 0,
+                                                    uint8_t *__restrict__ cm3 = nullptr, int64_t n3 = 0) {
     __shared__ uint32_t sm[1024];
     // sequential chunks per thread: nb may exceed 1024
     const int per = (nb + 1023) / 1024;
@@ -326,6 +341,19 @@ __global__ __launch_bounds__(1024) void k_scan_sums(int nb, uint32_t *__restrict
         run += v;
     }
     if (threadIdx.x == 1023) *total = 1u + sm[1023];  // + root
+    if (!cm1) return;
+    for (int64_t i = threadIdx.x; i < n2; i += 1024) {
+        unsigned m = 255;
+        for (int64_t j = i << 6; j < min(n1, (i + 1) << 6); ++j) m = min(m, (unsigned)cm1[j]);
+        cm2[i] = (uint8_t)m;
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < n3; i += 1024) {
+        unsigned m = 255;
+        for (int64_t j = i << 6; j < min(n2, (i + 1) << 6); ++j) m = min(m, (unsigned)cm2[j]);
+        cm3[i] = (uint8_t)m;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_scan_add(int64_t n, uint32_t *__restrict__ a, const uint32_t *__restrict__ blk) {
@@ -339,30 +367,76 @@ __device__ __forceinline__ unsigned digit_at(uint64_t hi, uint64_t lo, int level
 }
 
 // Over sorted keys the digits shared by positions a < b are the minimum of c
-// over [a, b), so the bodies sharing at least L digits with s form one run.
-// Its last position, searched in [s, ub] (ub: a position known to lie at or
-// past the end), by doubling steps then bisection:
-__device__ int64_t run_end(const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo, int64_t s, int L,
-                           int64_t ub) {
-    const uint64_t h = khi[s], l = klo[s];
-    int64_t good = s, bad = ub + 1, step = 1;
-    while (good + step < bad) {
-        const int64_t j = good + step;
-        if (shared_digits(h, l, khi[j], klo[j]) >= L) {
-            good = j;
-            step <<= 1;
-        } else {
-            bad = j;
+// over [a, b), so the run of bodies sharing at least L digits with s ends at
+// the first t >= s with c[t] < L (c[n-1] = 0 ends every run).  That search
+// goes up a hierarchy of minima -- cm1 over chunks of 64 positions, cm2 over
+// 4096, cm3 over 262144 -- and back down, reading 64 entries (four 16-byte
+// loads) per step, so a long run costs a handful of loads instead of a
+// search over the keys.
+
+// First index in [start, min(64 (chunk + 1), valid)) of chunk `chunk` of a
+// with a[i] < L (L <= 42), or -1.  Arrays are padded to whole chunks.
+__device__ __forceinline__ int64_t first_below_in(const uint8_t *__restrict__ a, int64_t chunk, int64_t start,
+                                                  int64_t valid, int L) {
+    const int64_t b = chunk << 6;
+    const uint4 *p = reinterpret_cast<const uint4 *>(a + b);
+    const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
+    const uint32_t wd[16] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
+                             w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w};
+    const int lo = (int)max<int64_t>(0, start - b), hi = (int)min<int64_t>(64, valid - b);
+    int found = -1;
+#pragma unroll
+    for (int q = 15; q >= 0; --q) {
+        uint32_t x = wd[q];
+        // bytes outside [lo, hi) read as 0xFF (never below L)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (4 * q + k < lo || 4 * q + k >= hi) x |= 0xFFu << (8 * k);
+        // bytes below L: the lowest flagged byte is exact (borrows only run upwards)
+        const uint32_t f = (x - 0x01010101u * (uint32_t)L) & ~x & 0x80808080u;
+        if (f) found = 4 * q + (__builtin_ctz(f) >> 3);
+    }
+    return found < 0 ? -1 : b + found;
+}
+
+struct RunIndex {
+    const uint8_t *c, *cm1, *cm2, *cm3;
+    int64_t n, n1, n2, n3;
+};
+
+// Last position of the run sharing at least L >= 1 digits with s: up the
+// levels through the rest of each chunk, then down inside the chunk found
+// (one inlined chunk scan, so the code stays small).
+__device__ int64_t run_end(const RunIndex &R, int64_t s, int L) {
+    int lev = 0;
+    int64_t start = s, chunk = s >> 6, found;
+    for (;;) {
+        const uint8_t *a = lev == 0 ? R.c : lev == 1 ? R.cm1 : lev == 2 ? R.cm2 : R.cm3;
+        const int64_t valid = lev == 0 ? R.n : lev == 1 ? R.n1 : lev == 2 ? R.n2 : R.n3;
+        if (found = first_below_in(a, chunk, start, valid, L); found >= 0) {
+            if (lev == 0) return found;
             break;
         }
+        if (lev < 3) {  // the entries after this chunk's at the next level up
+            start = chunk + 1;
+            chunk >>= 6;
+            ++lev;
+        } else {  // along the top level
+            ++chunk;
+            start = chunk << 6;
+            if (start >= R.n3) return R.n - 1;  // unreachable: the chunk holding n-1 has minimum 0
+        }
     }
-    while (bad - good > 1) {
-        const int64_t mid = good + (bad - good) / 2;
-        if (shared_digits(h, l, khi[mid], klo[mid]) >= L) good = mid;
-        else bad = mid;
+    while (lev > 0) {  // down: the first entry below L inside the chunk found
+        --lev;
+        const uint8_t *a = lev == 0 ? R.c : lev == 1 ? R.cm1 : lev == 2 ? R.cm2 : R.cm3;
+        const int64_t valid = lev == 0 ? R.n : lev == 1 ? R.n1 : lev == 2 ? R.n2 : R.n3;
+        found = first_below_in(a, found, found << 6, valid, L);
+        if (found < 0) return R.n - 1;  // unreachable: that chunk's minimum is below L
     }
-    return good;
+    return found;
 }
+
 // ---- nodes in DFS pre-order, and the bodies moved into tree order ----
 // Node numbering: the root is 0 and the nodes starting at sorted position s
 // are 1 + base[s] + j, internal levels (c[s-1], c[s]] first, then the leaf
@@ -381,7 +455,7 @@ __device__ int64_t run_end(const uint64_t *__restrict__ khi, const uint64_t *__r
 // stay in the previous step's tree order, so these gathers read nearly
 // sequential addresses.
 __global__ __launch_bounds__(kBlock) void k_build_nodes(
-    int64_t n, const uint8_t *__restrict__ c, const uint32_t *__restrict__ base, const uint32_t *__restrict__ total_p,
+    int64_t n, const RunIndex R, const uint32_t *__restrict__ base, const uint32_t *__restrict__ total_p,
     const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo, const double *__restrict__ box,
     uint32_t *__restrict__ idx, const double *__restrict__ x, const double *__restrict__ v,
     const double *__restrict__ m, const double *__restrict__ r, const double *__restrict__ e,
@@ -406,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
     idx[s] = (uint32_t)s;
     // nodes starting at s
     const int32_t total = (int32_t)*total_p;
-    const int cs = c[s], cp = s > 0 ? c[s - 1] : 0;
+    const int cs = R.c[s], cp = s > 0 ? R.c[s - 1] : 0;
     const uint64_t h = khi[s], l = klo[s];
     const int32_t n0 = 1 + (int32_t)base[s];
     double lo = box[0], hi = box[3];
@@ -420,7 +494,6 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
         side[0] = hi - lo;
     }
     int lev = 0;
-    int64_t ub = n - 1;
     for (int L = cp + 1; L <= cs; ++L) {
         for (; lev < L; ++lev) {
             const double mid = (lo + hi) * 0.5;
@@ -429,8 +502,7 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
             hi = up ? hi : mid;
         }
         const int32_t node = n0 + (L - cp - 1);
-        const int64_t end = run_end(khi, klo, s, L, ub);
-        ub = end;
+        const int64_t end = run_end(R, s, L);
         first[node] = (int32_t)s;
         last[node] = (int32_t)end;
         skip[node] = end >= n - 1 ? total : 1 + (int32_t)base[end + 1];
@@ -848,7 +920,6 @@ __device__ void write_walk_recs(const WalkRecs &W, int32_t x, int64_t cstride, c
 // found first; the leaves' walk records are written on the way.
 constexpr int32_t kSubtree = 1024;
 constexpr int kDepths = kLevels + 1;  // node depths 0..42
-constexpr uint32_t kTopLds = 8192;    // nodes above the subtrees sorted in LDS (else scanned per depth)
 
 __global__ void k_link(int32_t total, const int32_t *__restrict__ parent, const int32_t *__restrict__ skip,
                        const uint8_t *__restrict__ rank, const uint32_t *__restrict__ cbase,
@@ -989,6 +1060,12 @@ __global__ __launch_bounds__(kBlock) void k_com_subtrees(
     }
 }
 
+// The nodes above the subtrees, one workgroup.  Up to kTopLds of them are
+// sorted by depth into LDS together with their child lists, so each depth
+// costs one round of independent loads (the children's mass and COM) and a
+// barrier; the walk records follow in one sweep.  More such nodes: every
+// depth rescans the list with the child lists in memory.
+constexpr uint32_t kTopLds = 2048;
 __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *__restrict__ top,
                                                   const uint32_t *__restrict__ ntop, const int32_t *__restrict__ skip,
                                                   const int32_t *__restrict__ body, const uint8_t *__restrict__ depth,
@@ -998,15 +1075,15 @@ __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *
                                                   const double *__restrict__ side, const int32_t *__restrict__ first,
                                                   const int32_t *__restrict__ last) {
     __shared__ int32_t s_list[kTopLds];
+    __shared__ int32_t s_kids[kTopLds][8];
     __shared__ uint32_t s_cur[kDepths], s_off[kDepths + 1];
     const uint32_t cnt = *ntop;
-    const bool sorted = cnt <= kTopLds;
-    for (int t = threadIdx.x; t < kDepths; t += blockDim.x) s_cur[t] = 0;
+    const int t = threadIdx.x;
+    for (int i = t; i < kDepths; i += blockDim.x) s_cur[i] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x)
-        atomicAdd(&s_cur[min((int)depth[top[i]], kDepths - 1)], 1u);
+    for (uint32_t i = t; i < cnt; i += blockDim.x) atomicAdd(&s_cur[min((int)depth[top[i]], kDepths - 1)], 1u);
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         uint32_t run = 0;
         for (int L = 0; L < kDepths; ++L) {
             s_off[L] = run;
@@ -1016,31 +1093,69 @@ __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *
         s_off[kDepths] = run;
     }
     __syncthreads();
-    if (sorted)
-        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+    if (cnt <= kTopLds) {
+        for (uint32_t i = t; i < cnt; i += blockDim.x) {
             const int32_t p = top[i];
-            s_list[atomicAdd(&s_cur[min((int)depth[p], kDepths - 1)], 1u)] = p;
+            const uint32_t at = atomicAdd(&s_cur[min((int)depth[p], kDepths - 1)], 1u);
+            const int nc = min((int)nchild[p], 8);
+            const uint32_t c0 = 1u + cbase[p];
+            s_list[at] = p;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s_kids[at][k] = k < nc ? kid[c0 + k] : -1;
         }
-    __syncthreads();
-    for (int L = kDepths - 1; L >= 0; --L) {
-        const uint32_t b = s_off[L], e = s_off[L + 1];
-        if (b == e) continue;  // the same for every thread
-        if (sorted) {
-            for (uint32_t j = b + threadIdx.x; j < e; j += blockDim.x) {
+        __syncthreads();
+        for (int L = kDepths - 1; L >= 0; --L) {
+            const uint32_t b = s_off[L], e = s_off[L + 1];
+            if (b == e) continue;  // the same for every thread (LDS values after a barrier)
+            for (uint32_t j = b + t; j < e; j += blockDim.x) {
                 const int32_t p = s_list[j];
-                com_node_kids(cstride, p, nchild, cbase, kid, total, mass, com);
-                if (W.wnode) write_walk_recs(W, p, cstride, mass, com, side, body, skip, first, last);
+                // every child's loads issued together (absent children read node p's, unused)
+                int32_t cc[8];
+                bool ok[8];
+                double mv[8], xv[8], yv[8], zv[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int32_t c = s_kids[j][k];
+                    ok[k] = c > p && c < total;
+                    cc[k] = ok[k] ? c : p;
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    mv[k] = mass[cc[k]];
+                    xv[k] = com[cc[k]];
+                    yv[k] = com[cstride + cc[k]];
+                    zv[k] = com[2 * cstride + cc[k]];
+                }
+                double ms = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+                for (int k = 7; k >= 0; --k) {  // reverse octant order (calc_com :262-266)
+                    if (!ok[k]) continue;
+                    ms += mv[k];
+                    sx += mv[k] * xv[k];
+                    sy += mv[k] * yv[k];
+                    sz += mv[k] * zv[k];
+                }
+                mass[p] = ms;
+                com[p] = sx / ms;
+                com[cstride + p] = sy / ms;
+                com[2 * cstride + p] = sz / ms;
             }
-        } else {
-            for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+            __syncthreads();
+        }
+    } else {
+        for (int L = kDepths - 1; L >= 0; --L) {
+            if (s_off[L] == s_off[L + 1]) continue;
+            for (uint32_t i = t; i < cnt; i += blockDim.x) {
                 const int32_t p = top[i];
                 if (depth[p] != L) continue;
                 com_node_kids(cstride, p, nchild, cbase, kid, total, mass, com);
-                if (W.wnode) write_walk_recs(W, p, cstride, mass, com, side, body, skip, first, last);
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
+    if (W.wnode)
+        for (uint32_t i = t; i < cnt; i += blockDim.x)
+            write_walk_recs(W, top[i], cstride, mass, com, side, body, skip, first, last);
 }
 
 // Positions, velocities (and accelerations when a walk wrote them) back in
@@ -1174,7 +1289,7 @@ static void bh_build(nb_system *s) {
     ScopedTimer tm(s->prof, "bh_build", st);
     if (w.cap_n < n) {
         w.khi.ensure(n); w.klo.ensure(n); w.klo_slot.ensure(n); w.idx.ensure(n);
-        w.cnt.ensure(n); w.pos32s.ensure(n); w.c.ensure(n);
+        w.cnt.ensure(n); w.pos32s.ensure(n);
         w.blk.ensure((size_t)(n + 1023) / 1024 + 1);
         w.sorter.ensure(n);
         w.perm.ensure(n); w.perm_alt.ensure(n);
@@ -1188,6 +1303,10 @@ static void bh_build(nb_system *s) {
     if (!w.h_total) NB_HIP(hipHostMalloc((void **)&w.h_total, sizeof(uint32_t), hipHostMallocDefault));
     if (!w.h_err) NB_HIP(hipHostMalloc((void **)&w.h_err, sizeof(int), hipHostMallocDefault));
 
+    w.khi.ensure(n); w.idx.ensure(n);  // the sort may exchange these with its own buffers
+    // c and its chunk minima, padded to whole chunks of 64 for the searches' vector loads
+    const int64_t n1 = (n + 63) >> 6, n2 = (n1 + 63) >> 6, n3 = (n2 + 63) >> 6;
+    w.c.ensure((size_t)n + 64); w.cm1.ensure((size_t)n1 + 64); w.cm2.ensure((size_t)n2 + 64); w.cm3.ensure((size_t)n3 + 64);
     const int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, st, n, s->x.p, w.part.p);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, st, nparts, w.part.p, w.box.p);
@@ -1196,13 +1315,14 @@ static void bh_build(nb_system *s) {
                        w.idx.p, w.err.p);
     // the sort by the high key, then ties by the low key, and the node counts
     const int nb = (int)((n + 1023) / 1024);
-    w.sorter.sort(st, w.khi.p, w.idx.p, n, 3 * kHiLevels);
+    w.sorter.sort_swap(st, w.khi, w.idx, n, 3 * kHiLevels);
     hipLaunchKernelGGL(k_gather_lo, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.klo_slot.p, w.idx.p, w.klo.p);
     hipLaunchKernelGGL(k_fix_ties, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p);
     hipLaunchKernelGGL(k_lcp_counts, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p, s->x.p,
-                       w.c.p, w.cnt.p, w.err.p);
+                       w.c.p, w.cm1.p, w.cnt.p, w.err.p);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nb, w.blk.p, w.total.p);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nb, w.blk.p, w.total.p, w.cm1.p, n1, w.cm2.p, n2,
+                       w.cm3.p, n3);
     hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
     NB_HIP(hipMemcpyAsync(w.h_total, w.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     NB_HIP(hipMemcpyAsync(w.h_err, w.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1220,7 +1340,8 @@ static void bh_build(nb_system *s) {
     w.nodes = total;
     const int64_t cs = w.cap_nodes;  // stride of the SoA com array
     // nodes, and the bodies into tree order
-    hipLaunchKernelGGL(k_build_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.c.p, w.cnt.p, w.total.p, w.khi.p,
+    const RunIndex R{w.c.p, w.cm1.p, w.cm2.p, w.cm3.p, n, n1, n2, n3};
+    hipLaunchKernelGGL(k_build_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, R, w.cnt.p, w.total.p, w.khi.p,
                        w.klo.p, w.box.p, w.idx.p, s->x.p, s->v.p, w.permuted ? w.mt.p : s->m.p,
                        w.permuted ? w.rt.p : s->r.p, w.permuted ? w.et.p : s->e.p, w.permuted ? w.perm.p : nullptr,
                        w.xa.p, w.va.p, w.mt_alt.p, w.rt_alt.p, w.et_alt.p, w.perm_alt.p,

@@ -1,5 +1,7 @@
 // nb_sort.hpp -- device radix sort (nb_sort.hip).
 #pragma once
+#include <utility>
+
 #include "nb_internal.hpp"
 
 namespace nbe {
@@ -11,6 +13,8 @@ struct RadixSorter {
     DBuf<uint32_t> dtot;  // per-digit totals
     void ensure(int64_t n);
     void sort(hipStream_t s, uint64_t *keys, uint32_t *vals, int64_t n, int bits);
+    void sort_swap(hipStream_t s, DBuf<uint64_t> &keys, DBuf<uint32_t> &vals, int64_t n, int bits);
+    std::pair<uint64_t *, uint32_t *> passes(hipStream_t s, uint64_t *keys, uint32_t *vals, int64_t n, int bits);
 };
 
 }  // namespace nbe
