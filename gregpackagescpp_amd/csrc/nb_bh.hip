@@ -95,6 +95,7 @@ struct BHWork {
     DBuf<int32_t> roots, top;  // subtree roots and the nodes above them (bottom-up pass)
     DBuf<uint32_t> nroots;     // their counts
     DBuf<int32_t> fstack;   // frontier walk: one node stack per wave
+    DBuf<int64_t> gxr;      // frontier walk: the XCD ranges of the group order
     DBuf<int> werr;         // frontier walk: a stack would have overflowed
     DBuf<uint32_t> gcost;   // interactions of each 64-body group in the last walk
     DBuf<int32_t> gorder;   // groups of each XCD's range, heaviest first
@@ -1173,31 +1174,78 @@ __global__ void k_restore(int64_t n, const int32_t *__restrict__ perm, const dou
     }
 }
 
+// The frontier walk's schedule: eight contiguous ranges of the group order
+// (one per XCD) of about equal cost -- the previous walk's per-group counts
+// plus one -- each at most `span` groups long, and every range listed
+// heaviest first (a 64-bucket sort).  All eight workgroups compute the same
+// cuts from the same counts; workgroup x sorts range x, workgroup 0 writes
+// the bounds xr[0..8].
 __global__ __launch_bounds__(1024) void k_group_order(int64_t ngroups, int64_t span, const uint32_t *__restrict__ cost,
-                                                      int32_t *__restrict__ order) {
+                                                      int32_t *__restrict__ order, int64_t *__restrict__ xr) {
+    __shared__ uint64_t s_part[1024];
+    __shared__ int64_t s_cut[9];
     __shared__ uint32_t s_max;
     __shared__ uint32_t s_hist[64], s_base[64];
-    const int64_t lo = std::min<int64_t>(ngroups, span * blockIdx.x), hi = std::min<int64_t>(ngroups, span * (blockIdx.x + 1));
-    if (threadIdx.x == 0) s_max = 0;
-    if (threadIdx.x < 64) s_hist[threadIdx.x] = 0;
+    const int t = threadIdx.x;
+    // prefix sums of cost + 1 over per-thread chunks
+    const int64_t per = (ngroups + 1023) / 1024;
+    const int64_t c0 = std::min<int64_t>(ngroups, per * t), c1 = std::min<int64_t>(ngroups, c0 + per);
+    uint64_t sum = 0;
+    for (int64_t g = c0; g < c1; ++g) sum += (uint64_t)cost[g] + 1u;
+    s_part[t] = sum;
+    if (t < 9) s_cut[t] = t == 0 ? 0 : ngroups;
     __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint64_t u = t >= off ? s_part[t - off] : 0;
+        __syncthreads();
+        s_part[t] += u;
+        __syncthreads();
+    }
+    const uint64_t W = s_part[1023];
+    // cut x (1..7): the first group whose exclusive prefix reaches x W / 8
+    uint64_t e = t > 0 ? s_part[t - 1] : 0;
+    for (int64_t g = c0; g < c1; ++g) {
+        const uint64_t e1 = e + (uint64_t)cost[g] + 1u;
+        for (int x = 1; x < 8; ++x) {
+            const uint64_t target = (W * (uint64_t)x) / 8u;
+            if (e < target && e1 >= target) s_cut[x] = g + 1;
+        }
+        e = e1;
+    }
+    __syncthreads();
+    if (t == 0) {
+        // no range longer than span (the grid has span groups per XCD)
+        for (int x = 1; x < 8; ++x) {
+            int64_t c = std::max<int64_t>(s_cut[x], s_cut[x - 1]);
+            c = std::min<int64_t>(c, s_cut[x - 1] + span);
+            c = std::max<int64_t>(c, ngroups - (int64_t)(8 - x) * span);
+            s_cut[x] = std::min<int64_t>(c, ngroups);
+        }
+        s_cut[8] = ngroups;
+        if (blockIdx.x == 0)
+            for (int x = 0; x < 9; ++x) xr[x] = s_cut[x];
+        s_max = 0;
+    }
+    if (t < 64) s_hist[t] = 0;
+    __syncthreads();
+    const int64_t lo = s_cut[blockIdx.x], hi = s_cut[blockIdx.x + 1];
     uint32_t mx = 0;
-    for (int64_t g = lo + threadIdx.x; g < hi; g += blockDim.x) mx = max(mx, cost[g]);
+    for (int64_t g = lo + t; g < hi; g += blockDim.x) mx = max(mx, cost[g]);
     atomicMax(&s_max, mx);
     __syncthreads();
     const uint64_t M = (uint64_t)s_max + 1;
-    for (int64_t g = lo + threadIdx.x; g < hi; g += blockDim.x)
+    for (int64_t g = lo + t; g < hi; g += blockDim.x)
         atomicAdd(&s_hist[63u - (uint32_t)(((uint64_t)cost[g] * 64u) / M)], 1u);
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         uint32_t run = 0;
-        for (int b = 0; b < 64; ++b) {
-            s_base[b] = run;
-            run += s_hist[b];
+        for (int bb = 0; bb < 64; ++bb) {
+            s_base[bb] = run;
+            run += s_hist[bb];
         }
     }
     __syncthreads();
-    for (int64_t g = lo + threadIdx.x; g < hi; g += blockDim.x) {
+    for (int64_t g = lo + t; g < hi; g += blockDim.x) {
         const uint32_t at = atomicAdd(&s_base[63u - (uint32_t)(((uint64_t)cost[g] * 64u) / M)], 1u);
         order[lo + at] = (int32_t)g;
     }
@@ -1419,9 +1467,12 @@ void bh_forces(nb_system *s, int64_t *interactions) {
             } else {
                 // one wave per group; a multiple of 8 workgroups so that
                 // each XCD gets one range of `span` groups
+                // XCD ranges of equal cost hold at most 1.5x the mean count
                 const int64_t ngroups = (n + 63) / 64;
-                const int64_t G = std::max<int64_t>(8, ((ngroups + kFWaves - 1) / kFWaves + 7) / 8 * 8);
-                const int64_t span = G / 8 * kFWaves;
+                const int64_t mean = (ngroups + 7) / 8;
+                const int64_t wgs = std::max<int64_t>(1, (mean + mean / 2 + kFWaves - 1) / kFWaves);
+                const int64_t G = 8 * wgs;
+                const int64_t span = wgs * kFWaves;
                 const int32_t cap = s->tune_bh_stack > 0 ? s->tune_bh_stack : kFStackDefault;
                 w.fstack.ensure((size_t)G * kFWaves * (size_t)cap);
                 w.werr.ensure(1); w.icount2.ensure(1);
@@ -1432,11 +1483,13 @@ void bh_forces(nb_system *s, int64_t *interactions) {
                     w.gcost_n = ngroups;
                 }
                 hipLaunchKernelGGL(k_frontier_reset, dim3(1), dim3(64), 0, st, w.werr.p, w.icount2.p);
-                hipLaunchKernelGGL(k_group_order, dim3(8), dim3(1024), 0, st, ngroups, span, w.gcost.p, w.gorder.p);
+                w.gxr.ensure(9);
+                hipLaunchKernelGGL(k_group_order, dim3(8), dim3(1024), 0, st, ngroups, span, w.gcost.p, w.gorder.p,
+                                   w.gxr.p);
                 auto front = s->eps > 0 ? k_walk_frontier<true> : k_walk_frontier<false>;
                 hipLaunchKernelGGL(front, dim3((unsigned)G), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wgrp.p,
-                                   bucket, eps2, s->a.p, w.icount.p, w.fstack.p, cap, w.gorder.p, w.gcost.p,
-                                   w.werr.p);
+                                   bucket, eps2, s->a.p, w.icount.p, w.fstack.p, cap, w.gorder.p, w.gxr.p,
+                                   w.gcost.p, w.werr.p);
                 hipLaunchKernelGGL(wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wnode.p,
                                    bucket, eps2, s->a.p, w.icount2.p, w.werr.p);
                 frontier = true;

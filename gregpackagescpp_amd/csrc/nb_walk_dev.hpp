@@ -136,16 +136,25 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
 // traversal differs: each step pops up to 64 cells off the wave's stack and
 // tests one per lane, first against the group's bounding box (distance from
 // the cell's centre of mass to the box: if that passes, every body passes,
-// in FP32 as well, since rounding is monotonic) and otherwise body by body.
-// Accepted cells and bucket ranges go to per-wave LDS lists that every lane
-// then sums with VGPR operands; opened cells push their children, which the
-// grouped record layout keeps adjacent.  Each XCD takes one contiguous range
-// of the group order, which lists the groups of that range heaviest first by
-// the previous walk's counts.  A stack that would overflow sets *werr; the
-// depth-first wave walk then redoes the whole step (its launch follows,
-// gated on that flag).
-constexpr int kIL = 128;        // accepted cells buffered per wave (LDS fits 8 waves per SIMD)
+// in FP32 as well, since rounding is monotonic), then against the boxes of
+// the group's four 16-body quarters, and body by body only inside a quarter
+// whose box fails.  Accepted cells and bucket ranges go to per-wave LDS lists
+// that every lane then sums with VGPR operands; opened cells push their
+// children, which the grouped record layout keeps adjacent.  The stack's top
+// kLStack entries live in LDS; a push that would overflow them moves the LDS
+// part below to the wave's global stack (popped once the LDS part is empty),
+// so global memory sees stack traffic only for deep frontiers.  Each XCD
+// takes one contiguous range of the group order, which lists the groups of
+// that range heaviest first by the previous walk's counts.  A stack that
+// would exceed `cap` entries in all sets *werr; the depth-first wave walk
+// then redoes the whole step (its launch follows, gated on that flag).
+constexpr int kIL = 128;        // accepted cells buffered per wave (also the bucket bodies' staging)
 constexpr int kBL = 64;         // bucket ranges buffered per wave
+#ifndef NB_FRONTIER_LDS_STACK
+#define NB_FRONTIER_LDS_STACK 352  // 8 waves per SIMD fit the LDS; a test build sets a small one
+#endif
+constexpr int kLStack = NB_FRONTIER_LDS_STACK;  // stack entries per wave in LDS
+static_assert(kLStack >= 8, "the root's children start in LDS");
 constexpr int kFWaves = kBlock / 64;
 constexpr int kFStackDefault = 4096;
 
@@ -157,37 +166,72 @@ __device__ __forceinline__ float wave_max(float v) {
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
     return v;
 }
-__device__ __forceinline__ int wave_incl_scan(int v, unsigned lane) {
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(v, o);
-        v += (int)lane >= o ? t : 0;
-    }
+// minimum / maximum over the lane's 16-lane quarter
+__device__ __forceinline__ float quarter_min(float v) {
+    for (int o = 8; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float quarter_max(float v) {
+    for (int o = 8; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
     return v;
 }
 __device__ __forceinline__ int lanes_below(uint64_t m, unsigned lane) {
     return __popcll(m & ((1ull << lane) - 1ull));
+}
+// Exclusive prefix sum over the wave of v in [0, 15] and its total, from one
+// ballot per bit (no cross-lane data movement through LDS).
+__device__ __forceinline__ int wave_excl_scan_small(int v, unsigned lane, int &total) {
+    int excl = 0;
+    total = 0;
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) {
+        const uint64_t m = __ballot((v >> bit) & 1);
+        excl += lanes_below(m, lane) << bit;
+        total += __popcll(m) << bit;
+    }
+    return excl;
+}
+// squared distance from c to the box [lo, hi] (0 inside)
+__device__ __forceinline__ float box_d2(float cx, float cy, float cz, float lox, float loy, float loz, float hix,
+                                        float hiy, float hiz) {
+    const float dx = fmaxf(fmaxf(lox - cx, cx - hix), 0.f);
+    const float dy = fmaxf(fmaxf(loy - cy, cy - hiy), 0.f);
+    const float dz = fmaxf(fmaxf(loz - cz, cz - hiz), 0.f);
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
 template <bool SOFT>
 __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
     int64_t n, int32_t total, const float4 *__restrict__ pos, const WalkNode *__restrict__ wgrp, int bucket,
     float eps2, double *__restrict__ accs, unsigned long long *__restrict__ icount, int32_t *__restrict__ stacks,
-    int32_t cap, const int32_t *__restrict__ order, uint32_t *__restrict__ cost, int *__restrict__ werr) {
+    int32_t cap, const int32_t *__restrict__ order, const int64_t *__restrict__ xr, uint32_t *__restrict__ cost,
+    int *__restrict__ werr) {
 #pragma clang fp contract(fast)
     __shared__ float4 s_il[kFWaves][kIL];
     __shared__ int2 s_bl[kFWaves][kBL];
-    __shared__ float4 s_body[kFWaves][64];
     __shared__ float4 s_grp[kFWaves][64];
+    __shared__ int32_t s_stk[kFWaves][kLStack];
+    __shared__ float s_qbox[kFWaves][5][6];  // the quarters' boxes and the group's (lo xyz, hi xyz)
     const unsigned lane = __lane_id(), wv = threadIdx.x >> 6;
     // One wave per group, no loop over groups and no lane-0-only code in any
     // loop: a group handed out by one lane's atomic and broadcast by
     // readfirstlane inside a persistent loop was compiled into a loop that
-    // lanes 1-63 never left.  Workgroups are renumbered so that each XCD
-    // takes one contiguous range of the order (heaviest groups first).
-    const uint32_t g = gridDim.x, b = blockIdx.x;
-    const uint32_t lb = (g % 8u == 0u) ? (b % 8u) * (g / 8u) + b / 8u : b;
+    // lanes 1-63 never left.  Workgroups are dealt to the 8 XCDs round robin;
+    // with xr (9 range bounds) XCD b % 8 takes the groups order[xr[x] ..
+    // xr[x+1]) -- one contiguous Morton range of about equal cost, heaviest
+    // groups first -- and its workgroups past that range end at once.
+    // Without xr the grid is one range.
+    const uint32_t b = blockIdx.x;
     const int64_t ngroups = (n + 63) / 64;
-    const int64_t gi = (int64_t)lb * kFWaves + wv;
+    int64_t gi;
+    if (xr) {
+        const uint32_t x = b % 8u;
+        const int64_t lo = xr[x], hi = xr[x + 1];
+        gi = lo + (int64_t)(b / 8u) * kFWaves + wv;
+        if (gi >= hi) return;
+    } else {
+        gi = (int64_t)b * kFWaves + wv;
+    }
     if (gi >= ngroups) return;
     int32_t *const stk = stacks + ((size_t)b * kFWaves + wv) * (size_t)cap;
     const int4 r0 = wgrp[0].r;
@@ -200,87 +244,111 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
         const uint64_t active = __ballot(live);
         const int nlive = __popcll(active);
         s_grp[wv][lane] = p;
+        // the quarters' boxes (an empty quarter's box is empty: every cell passes it)
+        const float qlx = quarter_min(live ? p.x : INFINITY), qhx = quarter_max(live ? p.x : -INFINITY);
+        const float qly = quarter_min(live ? p.y : INFINITY), qhy = quarter_max(live ? p.y : -INFINITY);
+        const float qlz = quarter_min(live ? p.z : INFINITY), qhz = quarter_max(live ? p.z : -INFINITY);
+        if ((lane & 15u) == 0u) {
+            float *bx = s_qbox[wv][lane >> 4];
+            bx[0] = qlx; bx[1] = qly; bx[2] = qlz;
+            bx[3] = qhx; bx[4] = qhy; bx[5] = qhz;
+        }
+        {
+            const float lox = wave_min(qlx), hix = wave_max(qhx);
+            const float loy = wave_min(qly), hiy = wave_max(qhy);
+            const float loz = wave_min(qlz), hiz = wave_max(qhz);
+            if (lane == 0) {
+                float *bx = s_qbox[wv][4];
+                bx[0] = lox; bx[1] = loy; bx[2] = loz;
+                bx[3] = hix; bx[4] = hiy; bx[5] = hiz;
+            }
+        }
+        if (lane < (unsigned)nc0) s_stk[wv][lane] = cs0 + (int32_t)lane;
         wave_lds_sync();
-        const float lox = wave_min(live ? p.x : INFINITY), hix = wave_max(live ? p.x : -INFINITY);
-        const float loy = wave_min(live ? p.y : INFINITY), hiy = wave_max(live ? p.y : -INFINITY);
-        const float loz = wave_min(live ? p.z : INFINITY), hiz = wave_max(live ? p.z : -INFINITY);
         float ax = 0.f, ay = 0.f, az = 0.f;
         unsigned cnt = 0;
         int il = 0, bl = 0;
         auto flush_il = [&]() {
             wave_lds_sync();
             int j = 0;
-            for (; j + 4 <= il; j += 4) {
-                const float4 c0 = s_il[wv][j], c1 = s_il[wv][j + 1], c2 = s_il[wv][j + 2], c3 = s_il[wv][j + 3];
+            for (; j + 2 <= il; j += 2) {
+                const float4 c0 = s_il[wv][j], c1 = s_il[wv][j + 1];
                 monopole(c0, p, eps2, SOFT, ax, ay, az);
                 monopole(c1, p, eps2, SOFT, ax, ay, az);
-                monopole(c2, p, eps2, SOFT, ax, ay, az);
-                monopole(c3, p, eps2, SOFT, ax, ay, az);
             }
-            for (; j < il; ++j) monopole(s_il[wv][j], p, eps2, SOFT, ax, ay, az);
+            if (j < il) monopole(s_il[wv][j], p, eps2, SOFT, ax, ay, az);
             cnt += (unsigned)il;
             il = 0;
             wave_lds_sync();
         };
+        // bucket ranges, staged 64 bodies at a time through the cell list's
+        // LDS (emptied first)
         auto flush_bl = [&]() {
+            if (il) flush_il();
             wave_lds_sync();
             for (int q = 0; q < bl; ++q) {
                 const int2 rg = s_bl[wv][q];
                 const int32_t j0 = __builtin_amdgcn_readfirstlane(rg.x), nb = __builtin_amdgcn_readfirstlane(rg.y);
                 for (int32_t base = 0; base < nb; base += 64) {
                     const int32_t k = min(64, nb - base);
-                    if ((int32_t)lane < k) s_body[wv][lane] = pos[j0 + base + (int32_t)lane];
+                    if ((int32_t)lane < k) s_il[wv][lane] = pos[j0 + base + (int32_t)lane];
                     wave_lds_sync();
                     int32_t j = 0;
-                    for (; j + 4 <= k; j += 4) {
-                        const float4 b0 = s_body[wv][j], b1 = s_body[wv][j + 1], b2 = s_body[wv][j + 2],
-                                     b3 = s_body[wv][j + 3];
+                    for (; j + 2 <= k; j += 2) {
+                        const float4 b0 = s_il[wv][j], b1 = s_il[wv][j + 1];
                         monopole(b0, p, eps2, SOFT, ax, ay, az);
                         monopole(b1, p, eps2, SOFT, ax, ay, az);
-                        monopole(b2, p, eps2, SOFT, ax, ay, az);
-                        monopole(b3, p, eps2, SOFT, ax, ay, az);
                     }
-                    for (; j < k; ++j) monopole(s_body[wv][j], p, eps2, SOFT, ax, ay, az);
+                    if (j < k) monopole(s_il[wv][j], p, eps2, SOFT, ax, ay, az);
                     wave_lds_sync();
                 }
                 cnt += (unsigned)nb;
             }
             bl = 0;
         };
-        stk[lane] = cs0 + (int32_t)lane;  // entries past nc0 are never popped (cap >= 64)
-        int32_t sp = nc0;
+        int32_t sp = nc0;  // entries in the LDS part
+        int32_t gsp = 0;   // entries in the global part (below the LDS part)
         bool over = false;
         int32_t steps = 0;  // each step pops at least one cell of the tree
-        while (sp > 0 && steps++ < total) {
-            // this wave's stack writes are complete before its lanes read them
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            const int32_t take = min(64, sp);
-            sp -= take;
-            const bool valid = (int32_t)lane < take;
-            const int32_t node = valid ? stk[sp + (int32_t)lane] : 0;
-            const bool inb = valid && node > 0 && node < total;
+        while ((sp > 0 || gsp > 0) && steps++ < total) {
+            // the top `take` entries of the whole stack (global part, then
+            // LDS part), so each pop is the same whatever the split
+            const int32_t take = min(64, sp + gsp);
+            const int32_t kl = min(take, sp), kg = take - kl;
+            int32_t node = 0;
+            if (kg > 0) {
+                // this wave's global stack writes are complete before its lanes read them
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                if ((int32_t)lane < kg) node = stk[gsp - kg + (int32_t)lane];
+            }
+            if ((int32_t)lane >= kg && (int32_t)lane < take) node = s_stk[wv][sp - kl + (int32_t)lane - kg];
+            sp -= kl;
+            gsp -= kg;
+            const bool inb = (int32_t)lane < take && node > 0 && node < total;
             WalkNode rec;
             rec.q = make_float4(0.f, 0.f, 0.f, 0.f);
             rec.r = make_int4(0, 0, 0, 0);
             if (inb) rec = wgrp[node];
-            const float dx = fmaxf(fmaxf(lox - rec.q.x, rec.q.x - hix), 0.f);
-            const float dy = fmaxf(fmaxf(loy - rec.q.y, rec.q.y - hiy), 0.f);
-            const float dz = fmaxf(fmaxf(loz - rec.q.z, rec.q.z - hiz), 0.f);
-            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
             const float r2o = __int_as_float(rec.r.w);
-            bool acc = inb && d2 > r2o;
+            const float *gb = s_qbox[wv][4];
+            bool acc = inb && box_d2(rec.q.x, rec.q.y, rec.q.z, gb[0], gb[1], gb[2], gb[3], gb[4], gb[5]) > r2o;
             if (inb && !acc) {
-                // the box test is only sufficient: decide body by body with
-                // the wave walk's test
+                // the whole box is only sufficient: the quarters' boxes, then
+                // body by body with the wave walk's test inside a failing quarter
                 bool all = true;
-                for (int i = 0; i < nlive; ++i) {
-                    const float4 b = s_grp[wv][i];
-                    const float ex = rec.q.x - b.x, ey = rec.q.y - b.y, ez = rec.q.z - b.z;
-                    const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
-                    if (!(e2 > r2o)) {
-                        all = false;
-                        break;
+                for (int qq = 0; qq < 4 && all; ++qq) {
+                    const float *bx = s_qbox[wv][qq];
+                    if (box_d2(rec.q.x, rec.q.y, rec.q.z, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]) > r2o) continue;
+                    const int i1 = min(nlive, 16 * qq + 16);
+                    for (int i = 16 * qq; i < i1; ++i) {
+                        const float4 bb = s_grp[wv][i];
+                        const float ex = rec.q.x - bb.x, ey = rec.q.y - bb.y, ez = rec.q.z - bb.z;
+                        const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
+                        if (!(e2 > r2o)) {
+                            all = false;
+                            break;
+                        }
                     }
                 }
                 acc = all;
@@ -297,25 +365,47 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
             if (leafb) s_bl[wv][bl + lanes_below(m_leaf, lane)] = make_int2(rec.r.y, nbod);
             bl += n_leaf;
             const int nch = open ? min(8, rec.r.z & 15) : 0;
-            const int incl = wave_incl_scan(nch, lane);
-            const int tot = __shfl(incl, 63);
-            if (sp + tot > cap) {
+            int tot;
+            const int excl = wave_excl_scan_small(nch, lane, tot);
+            if (gsp + sp + tot > cap) {
                 over = true;
                 break;
             }
-            for (int k = 0; k < nch; ++k) stk[sp + incl - nch + k] = rec.r.x + k;
-            sp += tot;
+            const int32_t first = rec.r.x;
+            if (sp + tot > kLStack) {
+                // the LDS part moves below to the global part; the pushes go
+                // to LDS, or straight to global memory when they alone exceed it
+                for (int32_t k = (int32_t)lane; k < sp; k += 64) stk[gsp + k] = s_stk[wv][k];
+                gsp += sp;
+                sp = 0;
+                wave_lds_sync();  // the LDS part is read before the pushes overwrite it
+                if (tot > kLStack) {
+                    for (int k = 0; k < nch; ++k) stk[gsp + excl + k] = first + k;
+                    gsp += tot;
+                } else {
+                    for (int k = 0; k < nch; ++k) s_stk[wv][excl + k] = first + k;
+                    sp = tot;
+                }
+            } else {
+                for (int k = 0; k < nch; ++k) s_stk[wv][sp + excl + k] = first + k;
+                sp += tot;
+            }
+            wave_lds_sync();  // the pushes are visible to the next pop
         }
         if (il) flush_il();
         if (bl) flush_bl();
         if (over) werr[0] = 1;
-        if (live) {
-            accs[s] = ax;
-            accs[n + s] = ay;
-            accs[2 * n + s] = az;
+        {
+            // the group and slot again (not held in registers across the walk)
+            const int64_t grp2 = order ? (int64_t)order[gi] : gi;
+            const int64_t s2 = grp2 * 64 + lane;
+            if (s2 < n) {
+                accs[s2] = ax;
+                accs[n + s2] = ay;
+                accs[2 * n + s2] = az;
+            }
+            if (cost) cost[grp2] = cnt;  // every lane stores the same value
         }
-        if (cost) cost[grp] = cnt;  // every lane stores the same value
         if (lane == 0) atomicAdd(icount, (unsigned long long)cnt * (unsigned long long)nlive);
     }
 }
-

@@ -350,8 +350,23 @@ int main(int argc, char **argv) {
     }
     // frontier walk: one wave per group, workgroups in a multiple of 8
     {
+        // the product's grid (8 XCD ranges of at most 1.5x the mean count),
+        // with uneven ranges so that every XCD maps a different span
         const int64_t ngroups = (n + 63) / 64;
-        const unsigned G = (unsigned)std::max<int64_t>(8, ((ngroups + nbe::kFWaves - 1) / nbe::kFWaves + 7) / 8 * 8);
+        const int64_t mean = (ngroups + 7) / 8;
+        const int64_t wgs = std::max<int64_t>(1, (mean + mean / 2 + nbe::kFWaves - 1) / nbe::kFWaves);
+        const unsigned G = (unsigned)(8 * wgs);
+        const int64_t span = wgs * nbe::kFWaves;
+        int64_t *xr = galloc<int64_t>("xr", 9);
+        xr[0] = 0;
+        for (int x = 1; x < 8; ++x) {
+            int64_t c = (int64_t)std::llround((double)ngroups * std::pow(x / 8.0, 1.3));
+            c = std::max<int64_t>(c, xr[x - 1]);
+            c = std::min<int64_t>(c, xr[x - 1] + span);
+            c = std::max<int64_t>(c, ngroups - (int64_t)(8 - x) * span);
+            xr[x] = std::min<int64_t>(c, ngroups);
+        }
+        xr[8] = ngroups;
         double *accs = galloc<double>("accs (frontier)", 3 * (size_t)n);
         unsigned long long *icount = galloc<unsigned long long>("icount (frontier)", 1);
         int32_t *stacks = galloc<int32_t>("stacks", (size_t)G * nbe::kFWaves * (size_t)cap);
@@ -369,10 +384,10 @@ int main(int argc, char **argv) {
                 run_wave(w, [&]() {
                     if (soft)
                         nbe::k_walk_frontier<true>(n, (int32_t)total, pos, wgrp, (int)bucket, eps2, accs, icount,
-                                                   stacks, cap, order, cost, werr);
+                                                   stacks, cap, order, xr, cost, werr);
                     else
                         nbe::k_walk_frontier<false>(n, (int32_t)total, pos, wgrp, (int)bucket, eps2, accs, icount,
-                                                    stacks, cap, order, cost, werr);
+                                                    stacks, cap, order, xr, cost, werr);
                 });
             }
         printf("frontier walk: interactions %llu overflow %d\n", *icount, *werr);

@@ -78,3 +78,28 @@ def test_small_stack_reports_overflow(emu, tmp_path):
     m, pos, _ = capi.ic_two_plummer(53, 4096)
     (cw, _, _), (cf, ov, _) = run(emu, tmp_path, m, pos, 1, cap=64)
     assert ov == 1
+
+
+@pytest.fixture(scope="module")
+def emu_small_lds(tmp_path_factory):
+    """The same walks with 16 frontier stack entries in LDS, so the stack
+    moves to global memory and back many times per group."""
+    exe = str(tmp_path_factory.mktemp("emu16") / "walk_emu")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-U_FORTIFY_SOURCE", "-D_FORTIFY_SOURCE=0",
+                           "-ffp-contract=off", "-Wno-unknown-pragmas", "-DNB_FRONTIER_LDS_STACK=16", "-I", INC,
+                           "-o", exe, SRC])
+    return exe
+
+
+@pytest.mark.parametrize("bucket,theta", [(1, 0.5), (8, 0.3)])
+def test_frontier_lds_stack_split_is_invisible(emu, emu_small_lds, tmp_path, bucket, theta):
+    """Each pop takes the top of the whole stack (global part below, LDS part
+    above), so where the entries live changes no sum: a 16-entry LDS part
+    gives bit-identical accelerations and the same counts."""
+    m, pos, _ = capi.ic_two_plummer(54, 4096)
+    (cw, _, aw), (cf, ov, af) = run(emu, tmp_path, m, pos, bucket, theta=theta)
+    (cw2, _, aw2), (cf2, ov2, af2) = run(emu_small_lds, tmp_path, m, pos, bucket, theta=theta)
+    assert ov == 0 and ov2 == 0
+    assert cf == cw and cf2 == cf and cw2 == cw
+    assert np.array_equal(af2, af)
+    assert np.array_equal(aw2, aw)
