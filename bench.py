@@ -107,14 +107,14 @@ BH_BODIES = 1 << 20
 BH_THETA = 0.5
 
 
-RADIX_PASSES = 8  # 8-bit LSD passes over the 63-bit high key of the octree sort
+RADIX_PASSES = 7  # 9-bit LSD passes over the 63-bit high key of the octree sort
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 summed over the build kernels
 # of one BH step at 1,048,576 bodies (profiles/bh_walk_build_two_plummer_1M.md),
 # and the same for the frontier walk launch; None when not profiled.
-PROFILED_BH_BUILD_BYTES = None
-PROFILED_BH_WALK_BYTES = None
+PROFILED_BH_BUILD_BYTES = 1.381e9
+PROFILED_BH_WALK_BYTES = 2.197e8
 
 
 def build_bytes(n, nodes, passes=RADIX_PASSES):
