@@ -119,7 +119,7 @@ def test_frontier_repeatable_and_evolves():
 
 def test_frontier_theta_zero_is_direct_sum():
     """theta = 0 accepts no cell: every body is summed body by body (counts
-    n*n with the own body), whichever walk runs or falls back."""
+    n*n with the own body) by the frontier walk itself, not its fallback."""
     print("case frontier theta=0", flush=True)
     n = 4096
     m, pos, _ = capi.ic_plummer(47, n)
@@ -131,3 +131,7 @@ def test_frontier_theta_zero_is_direct_sum():
     ref = (m[None, :, None] * d / (r2 * np.sqrt(r2))[:, :, None]).sum(1)
     assert rel_diff(a, ref).max() < 1e-4
     assert rel_diff(b, ref).max() < 1e-4
+    # the frontier walk's own sums were used: a fallback to the depth-first
+    # walk would return b bit for bit, while the frontier walk adds the same
+    # bodies in another order (bucket ranges in pop order, not DFS order)
+    assert not np.array_equal(a, b)
