@@ -887,28 +887,36 @@ struct WalkRecs {
     double G, theta;
     int32_t total;
 };
+// The walk records of node x from its values: FP64 mass and COM, whether it
+// is a leaf, its side, skip, first and last body, child count and base, and
+// its slot in the grouped layout.
+__device__ __forceinline__ void put_walk_recs(const WalkRecs &W, int32_t x, double m, double cx, double cy,
+                                              double cz, bool leaf, double sd, int32_t sk, int32_t fi, int32_t la,
+                                              int nc, uint32_t cb, uint32_t at) {
+    WalkNode w;
+    w.q = make_float4((float)cx, (float)cy, (float)cz, (float)(W.G * m));
+    float r2o;
+    if (leaf) r2o = -1.0f;
+    else if (W.theta > 0) {
+        const double r = sd / W.theta;
+        r2o = (float)(r * r);
+    } else r2o = INFINITY;
+    w.r = make_int4(sk, fi, la - fi + 1, __float_as_int(r2o));
+    W.wnode[x] = w;
+    if (!W.wgrp) return;
+    const int32_t cnt = min(la - fi + 1, (1 << 27) - 1);
+    WalkNode g;
+    g.q = w.q;
+    g.r = make_int4(nc ? 1 + (int32_t)cb : 0, fi, (cnt << 4) | nc, __float_as_int(r2o));
+    if (at < (uint32_t)W.total) W.wgrp[at] = g;
+}
+
 __device__ void write_walk_recs(const WalkRecs &W, int32_t x, int64_t cstride, const double *__restrict__ mass,
                                 const double *__restrict__ com, const double *__restrict__ side,
                                 const int32_t *__restrict__ body, const int32_t *__restrict__ skip,
                                 const int32_t *__restrict__ first, const int32_t *__restrict__ last) {
-    WalkNode w;
-    w.q = make_float4((float)com[x], (float)com[cstride + x], (float)com[2 * cstride + x], (float)(W.G * mass[x]));
-    float r2o;
-    if (body[x] >= 0) r2o = -1.0f;
-    else if (W.theta > 0) {
-        const double r = side[x] / W.theta;
-        r2o = (float)(r * r);
-    } else r2o = INFINITY;
-    w.r = make_int4(skip[x], first[x], last[x] - first[x] + 1, __float_as_int(r2o));
-    W.wnode[x] = w;
-    if (!W.wgrp) return;
-    const int nc = W.nchild[x];
-    const int32_t cnt = min(last[x] - first[x] + 1, (1 << 27) - 1);
-    WalkNode g;
-    g.q = w.q;
-    g.r = make_int4(nc ? 1 + (int32_t)W.cbase[x] : 0, first[x], (cnt << 4) | nc, __float_as_int(r2o));
-    const uint32_t at = W.slot[x];
-    if (at < (uint32_t)W.total) W.wgrp[at] = g;
+    put_walk_recs(W, x, mass[x], com[x], com[cstride + x], com[2 * cstride + x], body[x] >= 0, side[x], skip[x],
+                  first[x], last[x], W.nchild[x], W.cbase[x], W.slot[x]);
 }
 
 // The bottom-up pass is split at subtrees of at most kSubtree nodes: one
@@ -921,6 +929,29 @@ __device__ void write_walk_recs(const WalkRecs &W, int32_t x, int64_t cstride, c
 // found first; the leaves' walk records are written on the way.
 constexpr int32_t kSubtree = 1024;
 constexpr int kDepths = kLevels + 1;  // node depths 0..42
+
+// Exclusive offsets of the per-depth counts s_cur (kDepths <= 64 entries)
+// into s_off[0..kDepths], by the first wave with one shuffle scan; s_cur is
+// reset to the offsets (the running insertion points).  The caller
+// synchronises before and after.
+__device__ __forceinline__ void depth_offsets(uint32_t *s_cur, uint32_t *s_off) {
+    static_assert(kDepths <= 64, "one depth per lane");
+    if (threadIdx.x < 64) {
+        const unsigned lane = threadIdx.x;
+        const uint32_t v = lane < (unsigned)kDepths ? s_cur[lane] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            x += lane >= (unsigned)o ? y : 0u;
+        }
+        if (lane < (unsigned)kDepths) {
+            s_off[lane] = x - v;
+            s_cur[lane] = x - v;
+        }
+        if (lane == (unsigned)kDepths - 1) s_off[kDepths] = x;
+    }
+}
 
 __global__ void k_link(int32_t total, const int32_t *__restrict__ parent, const int32_t *__restrict__ skip,
                        const uint8_t *__restrict__ rank, const uint32_t *__restrict__ cbase,
@@ -974,66 +1005,85 @@ __global__ __launch_bounds__(kBlock) void k_com_subtrees(
     const uint8_t *__restrict__ nchild, const uint32_t *__restrict__ cbase, const int32_t *__restrict__ kid,
     double *__restrict__ mass, double *__restrict__ com, int64_t cstride, WalkRecs W, const double *__restrict__ side,
     const int32_t *__restrict__ first, const int32_t *__restrict__ last) {
+    constexpr int kPer = kSubtree / kBlock;  // nodes per thread
     __shared__ double s_m[kSubtree], s_x[kSubtree], s_y[kSubtree], s_z[kSubtree];
     __shared__ int16_t s_kid[kSubtree];   // children of the subtree's nodes, local indices, grouped layout order
     __shared__ uint32_t s_list[kSubtree]; // internal nodes by depth: local index | first child << 11 | count << 22
     __shared__ uint32_t s_cur[kDepths], s_off[kDepths + 1];
     const uint32_t cnt = *nroots;
+    const int t = threadIdx.x;
     for (uint32_t q = blockIdx.x; q < cnt; q += gridDim.x) {
         const int32_t r = roots[q];
         const int32_t size = min(skip[r] - r, kSubtree);
         const uint32_t cb0 = cbase[r];
-        for (int t = threadIdx.x; t < kDepths; t += kBlock) s_cur[t] = 0;
-        // the children of the subtree's nodes fill slots 1 + cbase[r] .. in the
-        // grouped layout (cbase is a prefix sum over pre-order, and every node
-        // of the subtree but its root is a child of one of its nodes)
-        for (int32_t t = threadIdx.x; t < size - 1; t += kBlock) {
-            const int32_t ch = kid[1 + cb0 + t] - r;
-            s_kid[t] = (int16_t)(ch > 0 && ch < size ? ch : 0);
-        }
-        __syncthreads();
-        // the leaves' mass and COM into LDS; internal nodes counted per depth
-        for (int32_t i = threadIdx.x; i < size; i += kBlock) {
-            const int32_t p = r + i;
-            if (body[p] >= 0) {
-                s_m[i] = mass[p];
-                s_x[i] = com[p];
-                s_y[i] = com[cstride + p];
-                s_z[i] = com[2 * cstride + p];
-            } else {
-                atomicAdd(&s_cur[min((int)depth[p], kDepths - 1)], 1u);
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t run = 0;
-            for (int L = 0; L < kDepths; ++L) {
-                s_off[L] = run;
-                run += s_cur[L];
-                s_cur[L] = s_off[L];
-            }
-            s_off[kDepths] = run;
-        }
-        __syncthreads();
-        for (int32_t i = threadIdx.x; i < size; i += kBlock) {
-            const int32_t p = r + i;
-            if (body[p] < 0) {
-                const uint32_t c0 = cbase[p] - cb0, nc = min((uint32_t)nchild[p], 8u);
-                s_list[atomicAdd(&s_cur[min((int)depth[p], kDepths - 1)], 1u)] =
-                    (uint32_t)i | (min(c0, (uint32_t)kSubtree - 1u) << 11) | (nc << 22);
-            }
-        }
-        __syncthreads();
         const int d0 = depth[r];
+        if (t < kDepths) s_cur[t] = 0;
+        // every load of this thread's nodes (local indices t + j kBlock) at
+        // once: indices past the subtree read its last node, unused.  The
+        // children of the subtree's nodes fill slots 1 + cbase[r] .. in the
+        // grouped layout (cbase is a prefix sum over pre-order, and every
+        // node of the subtree but its root is a child of one of its nodes).
+        int32_t bd[kPer], kd[kPer];
+        uint32_t cb[kPer];
+        int dp[kPer], nc[kPer];
+        double mv[kPer], xv[kPer], yv[kPer], zv[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int32_t i = t + j * kBlock;
+            const int32_t p = r + min(i, size - 1);
+            bd[j] = body[p];
+            dp[j] = min((int)depth[p], kDepths - 1);
+            nc[j] = min((int)nchild[p], 8);
+            cb[j] = cbase[p];
+            mv[j] = mass[p];
+            xv[j] = com[p];
+            yv[j] = com[cstride + p];
+            zv[j] = com[2 * cstride + p];
+            kd[j] = kid[1 + cb0 + (uint32_t)min(i, max(size - 2, 0))];
+        }
+        __syncthreads();  // s_cur cleared (and LDS free from the previous subtree)
+        // the leaves' mass and COM and the child lists into LDS; internal
+        // nodes counted per depth
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int32_t i = t + j * kBlock;
+            if (i < size - 1) {
+                const int32_t ch = kd[j] - r;
+                s_kid[i] = (int16_t)(ch > 0 && ch < size ? ch : 0);
+            }
+            if (i < size) {
+                if (bd[j] >= 0) {
+                    s_m[i] = mv[j];
+                    s_x[i] = xv[j];
+                    s_y[i] = yv[j];
+                    s_z[i] = zv[j];
+                } else {
+                    atomicAdd(&s_cur[dp[j]], 1u);
+                }
+            }
+        }
+        __syncthreads();
+        depth_offsets(s_cur, s_off);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int32_t i = t + j * kBlock;
+            if (i < size && bd[j] < 0) {
+                const uint32_t c0 = cb[j] - cb0;
+                s_list[atomicAdd(&s_cur[dp[j]], 1u)] =
+                    (uint32_t)i | (min(c0, (uint32_t)kSubtree - 1u) << 11) | ((uint32_t)nc[j] << 22);
+            }
+        }
+        __syncthreads();
         for (int L = kDepths - 1; L >= d0; --L) {
             const uint32_t b = s_off[L], e = s_off[L + 1];
             if (b == e) continue;  // the same for every thread (LDS values after a barrier)
-            for (uint32_t j = b + threadIdx.x; j < e; j += kBlock) {
+            for (uint32_t j = b + t; j < e; j += kBlock) {
                 const uint32_t rec = s_list[j];
                 const int32_t i = (int32_t)(rec & 2047u);
-                const int32_t c0 = (int32_t)((rec >> 11) & 2047u), nc = (int32_t)(rec >> 22);
+                const int32_t c0 = (int32_t)((rec >> 11) & 2047u), ncj = (int32_t)(rec >> 22);
                 double ms = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-                for (int k = nc - 1; k >= 0; --k) {  // reverse octant order (calc_com :262-266)
+                for (int k = ncj - 1; k >= 0; --k) {  // reverse octant order (calc_com :262-266)
                     const int32_t ch = s_kid[min(c0 + k, kSubtree - 1)];
                     if (ch <= i) continue;
                     const double mc = s_m[ch];
@@ -1049,13 +1099,34 @@ __global__ __launch_bounds__(kBlock) void k_com_subtrees(
             }
             __syncthreads();  // this depth's sums before the next one up reads them
         }
-        for (uint32_t j = threadIdx.x; j < s_off[kDepths]; j += kBlock) {
-            const int32_t i = (int32_t)(s_list[j] & 2047u), p = r + i;
-            mass[p] = s_m[i];
-            com[p] = s_x[i];
-            com[cstride + p] = s_y[i];
-            com[2 * cstride + p] = s_z[i];
-            if (W.wnode) write_walk_recs(W, p, cstride, mass, com, side, body, skip, first, last);
+        // this thread's internal nodes out, with their walk records (the
+        // record inputs of all of them loaded at once; leaves read the root's)
+        double sd[kPer];
+        int32_t sk[kPer], fi[kPer], la[kPer];
+        uint32_t at[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int32_t i = t + j * kBlock;
+            const int32_t p = i < size && bd[j] < 0 ? r + i : r;
+            sd[j] = side[p];
+            sk[j] = skip[p];
+            fi[j] = first[p];
+            la[j] = last[p];
+            at[j] = W.slot[p];
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int32_t i = t + j * kBlock;
+            if (i < size && bd[j] < 0) {
+                const int32_t p = r + i;
+                mass[p] = s_m[i];
+                com[p] = s_x[i];
+                com[cstride + p] = s_y[i];
+                com[2 * cstride + p] = s_z[i];
+                if (W.wnode)
+                    put_walk_recs(W, p, s_m[i], s_x[i], s_y[i], s_z[i], false, sd[j], sk[j], fi[j], la[j], nc[j],
+                                  cb[j], at[j]);
+            }
         }
         __syncthreads();  // LDS reused by the next subtree
     }
@@ -1182,7 +1253,7 @@ __global__ void k_restore(int64_t n, const int32_t *__restrict__ perm, const dou
 // the bounds xr[0..8].
 __global__ __launch_bounds__(1024) void k_group_order(int64_t ngroups, int64_t span, const uint32_t *__restrict__ cost,
                                                       int32_t *__restrict__ order, int64_t *__restrict__ xr) {
-    __shared__ uint64_t s_part[1024];
+    __shared__ uint64_t s_part[16];
     __shared__ int64_t s_cut[9];
     __shared__ uint32_t s_max;
     __shared__ uint32_t s_hist[64], s_base[64];
@@ -1190,27 +1261,48 @@ __global__ __launch_bounds__(1024) void k_group_order(int64_t ngroups, int64_t s
     // prefix sums of cost + 1 over per-thread chunks
     const int64_t per = (ngroups + 1023) / 1024;
     const int64_t c0 = std::min<int64_t>(ngroups, per * t), c1 = std::min<int64_t>(ngroups, c0 + per);
+    // the chunk's counts, eight independent loads at a time
     uint64_t sum = 0;
-    for (int64_t g = c0; g < c1; ++g) sum += (uint64_t)cost[g] + 1u;
-    s_part[t] = sum;
-    if (t < 9) s_cut[t] = t == 0 ? 0 : ngroups;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint64_t u = t >= off ? s_part[t - off] : 0;
-        __syncthreads();
-        s_part[t] += u;
-        __syncthreads();
+    for (int64_t g0 = c0; g0 < c1; g0 += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = cost[std::min<int64_t>(g0 + k, ngroups - 1)];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sum += g0 + k < c1 ? (uint64_t)v[k] + 1u : 0u;
     }
-    const uint64_t W = s_part[1023];
+    if (t < 9) s_cut[t] = t == 0 ? 0 : ngroups;
+    // inclusive scan of the chunk sums: within each wave by shuffles, then
+    // the 16 wave totals
+    const unsigned lane = __lane_id(), wv = (unsigned)t >> 6;
+    uint64_t x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(x, o);
+        x += lane >= (unsigned)o ? y : 0ull;
+    }
+    if (lane == 63) s_part[wv] = x;
+    __syncthreads();
+    uint64_t pre = 0, W = 0;
+    for (unsigned i = 0; i < blockDim.x / 64; ++i) {
+        pre += i < wv ? s_part[i] : 0ull;
+        W += s_part[i];
+    }
     // cut x (1..7): the first group whose exclusive prefix reaches x W / 8
-    uint64_t e = t > 0 ? s_part[t - 1] : 0;
-    for (int64_t g = c0; g < c1; ++g) {
-        const uint64_t e1 = e + (uint64_t)cost[g] + 1u;
-        for (int x = 1; x < 8; ++x) {
-            const uint64_t target = (W * (uint64_t)x) / 8u;
-            if (e < target && e1 >= target) s_cut[x] = g + 1;
+    uint64_t e = pre + x - sum;
+    for (int64_t g0 = c0; g0 < c1; g0 += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = cost[std::min<int64_t>(g0 + k, ngroups - 1)];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (g0 + k >= c1) break;
+            const uint64_t e1 = e + (uint64_t)v[k] + 1u;
+            for (int x = 1; x < 8; ++x) {
+                const uint64_t target = (W * (uint64_t)x) / 8u;
+                if (e < target && e1 >= target) s_cut[x] = g0 + k + 1;
+            }
+            e = e1;
         }
-        e = e1;
     }
     __syncthreads();
     if (t == 0) {

@@ -44,62 +44,73 @@ __global__ __launch_bounds__(kThreads) void k_hist(const uint64_t *__restrict__ 
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Block-wide inclusive scan of one value per thread.
-__device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t *sm) {
-    const int t = threadIdx.x;
-    sm[t] = v;
-    __syncthreads();
-    for (int off = 1; off < kThreads; off <<= 1) {
-        const uint32_t u = t >= off ? sm[t - off] : 0;
-        __syncthreads();
-        sm[t] += u;
-        __syncthreads();
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const unsigned lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Block-wide exclusive scan of one value per thread: a scan within each
+// wave (shuffles), the wave totals through LDS, two barriers.  s_w holds
+// kWaves entries; total receives the block's sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_w, uint32_t &total) {
+    const unsigned lane = __lane_id(), w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        x += lane >= (unsigned)o ? y : 0u;
     }
-    const uint32_t r = sm[t];
+    if (lane == 63) s_w[w] = x;
     __syncthreads();
-    return r;
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) {
+        const uint32_t u = s_w[i];
+        pre += i < (int)w ? u : 0u;
+        tot += u;
+    }
+    __syncthreads();  // s_w is reused by the next scan
+    total = tot;
+    return pre + x - v;
 }
 
 // One workgroup per digit: exclusive scan of that digit's per-tile counts
 // (in place) and the digit's total.
 __global__ __launch_bounds__(kThreads) void k_tile_scan(uint32_t *__restrict__ hist, int ntiles,
                                                         uint32_t *__restrict__ dtot) {
-    __shared__ uint32_t sm[kThreads];
+    __shared__ uint32_t s_w[kWaves];
     uint32_t *row = hist + (size_t)blockIdx.x * ntiles;
     uint32_t carry = 0;
     for (int b = 0; b < ntiles; b += kThreads) {
         const int i = b + threadIdx.x;
         const uint32_t v = i < ntiles ? row[i] : 0;
-        const uint32_t inc = block_incl_scan(v, sm);
-        const uint32_t chunk = sm[kThreads - 1];
-        if (i < ntiles) row[i] = carry + inc - v;
+        uint32_t chunk;
+        const uint32_t ex = block_excl_scan(v, s_w, chunk);
+        if (i < ntiles) row[i] = carry + ex;
         carry += chunk;
-        __syncthreads();  // sm is rewritten by the next chunk
     }
     if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
 }
 
-__device__ __forceinline__ uint64_t lanemask_lt() {
-    const unsigned lane = __lane_id();
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
-
+// The stable scatter of one tile.  Wave w holds the tile's items [w * 256,
+// w * 256 + 256) in registers, 64 per round, so the (wave, round, lane)
+// order is the index order.  Ranks: per wave, a running count per digit in
+// LDS (this wave's row only, so the wave's own program order orders its LDS
+// reads and writes) gives each item its rank among the wave's items of its
+// digit; one barrier later the rows become, per digit, the tile's digit
+// start plus the counts of the waves before.
 __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                       uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, int n,
                                                       int shift, const uint32_t *__restrict__ hist, int ntiles,
                                                       const uint32_t *__restrict__ dtot) {
-    // the per-wave digit counts of the ranking rounds, then the sorted tile
-    constexpr int kRankBytes = kWaves * kBins * 4;
-    constexpr int kTileBytes = kTileItems * 12;
-    __shared__ __align__(16) unsigned char s_raw[kRankBytes > kTileBytes ? kRankBytes : kTileBytes];
-    __shared__ uint32_t s_scan[kThreads];
-    __shared__ uint32_t s_run[kBins];   // next local position of each digit
+    __shared__ uint32_t wcnt[kWaves][kBins];
+    __shared__ __align__(16) uint64_t s_k[kTileItems];
+    __shared__ uint32_t s_v[kTileItems];
     __shared__ uint32_t s_gofs[kBins];  // global position - local position, per digit
-    uint32_t(*wcnt)[kBins] = reinterpret_cast<uint32_t(*)[kBins]>(s_raw);
-    uint64_t *s_k = reinterpret_cast<uint64_t *>(s_raw);
-    uint32_t *s_v = reinterpret_cast<uint32_t *>(s_raw + kTileItems * 8);
-
+    __shared__ uint32_t s_w[kWaves];
+    constexpr int kPerWave = kTileItems / kWaves;  // 256
     const int t = threadIdx.x;
+    const unsigned lane = __lane_id();
     const int wave = t >> 6;
     const int base = blockIdx.x * kTileItems;
     const int live_n = min(kTileItems, n - base);
@@ -107,32 +118,20 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t *__restrict
     uint32_t v[kRounds];
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
-        const int i = base + r * kThreads + t;
-        k[r] = i < n ? kin[i] : 0;
-        v[r] = i < n ? vin[i] : 0;
+        const int li = wave * kPerWave + r * 64 + (int)lane;
+        const int i = base + li;
+        k[r] = li < live_n ? kin[i] : 0;
+        v[r] = li < live_n ? vin[i] : 0;
     }
-    // the tile's own digit counts (s_run) and the digits' global starts
-    s_run[t] = 0;
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kRounds; ++r)
-        if (r * kThreads + t < live_n) atomicAdd(&s_run[(unsigned)(k[r] >> shift) & (kBins - 1)], 1u);
-    __syncthreads();
-    const uint32_t own = s_run[t];
-    const uint32_t lstart = block_incl_scan(own, s_scan) - own;
-    const uint32_t dt = dtot[t];
-    const uint32_t dstart = block_incl_scan(dt, s_scan) - dt;
-    s_gofs[t] = dstart + hist[t * ntiles + blockIdx.x] - lstart;
-    s_run[t] = lstart;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) wcnt[w][t] = 0;
     __syncthreads();
-    // ranks: local position of each item in the digit-sorted tile
+    // ranks within the wave
     const uint64_t lt = lanemask_lt();
-    uint32_t pos[kRounds];
+    uint32_t rk[kRounds];
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
-        const bool live = r * kThreads + t < live_n;
+        const bool live = wave * kPerWave + r * 64 + (int)lane < live_n;
         const unsigned d = (unsigned)(k[r] >> shift) & (kBins - 1);
         uint64_t peers = __ballot(live);
 #pragma unroll
@@ -140,29 +139,46 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const uint64_t *__restrict
             const uint64_t m = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? m : ~m;
         }
-        const unsigned rank = __popcll(peers & lt);
-        if (live && rank == 0) wcnt[wave][d] = __popcll(peers);
-        __syncthreads();
-        uint32_t off = s_run[d] + rank;
-        for (int w = 0; w < wave; ++w) off += wcnt[w][d];
-        pos[r] = off;
-        __syncthreads();
-        uint32_t add = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            add += wcnt[w][t];
-            wcnt[w][t] = 0;
-        }
-        s_run[t] += add;
-        __syncthreads();
+        const uint32_t before = wcnt[wave][d];
+        rk[r] = before + (uint32_t)__popcll(peers & lt);
+        // every lane has read the count before the leader of its digit adds
+        // the round's peers (the wave's LDS accesses complete in order)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (live && (peers & lt) == 0) wcnt[wave][d] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    // the tile sorted by digit in LDS (overlays the per-wave counts, which the
-    // last round's barrier has retired)
+    __syncthreads();
+    // per digit t: the tile's count, its start in the tile, the waves' bases
+    uint32_t cnt[kWaves], tile_cnt = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        cnt[w] = wcnt[w][t];
+        tile_cnt += cnt[w];
+    }
+    uint32_t unused;
+    const uint32_t lstart = block_excl_scan(tile_cnt, s_w, unused);
+    const uint32_t dt = dtot[t];
+    const uint32_t dstart = block_excl_scan(dt, s_w, unused);
+    s_gofs[t] = dstart + hist[t * ntiles + blockIdx.x] - lstart;
+    uint32_t run = lstart;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        wcnt[w][t] = run;
+        run += cnt[w];
+    }
+    __syncthreads();
+    // the tile sorted by digit in LDS, then out in that order
 #pragma unroll
     for (int r = 0; r < kRounds; ++r)
-        if (r * kThreads + t < live_n) {
-            s_k[pos[r]] = k[r];
-            s_v[pos[r]] = v[r];
+        if (wave * kPerWave + r * 64 + (int)lane < live_n) {
+            const unsigned d = (unsigned)(k[r] >> shift) & (kBins - 1);
+            const uint32_t pos = wcnt[wave][d] + rk[r];
+            s_k[pos] = k[r];
+            s_v[pos] = v[r];
         }
     __syncthreads();
     for (int j = t; j < live_n; j += kThreads) {

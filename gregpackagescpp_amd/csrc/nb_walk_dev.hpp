@@ -151,10 +151,16 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
 constexpr int kIL = 128;        // accepted cells buffered per wave (also the bucket bodies' staging)
 constexpr int kBL = 64;         // bucket ranges buffered per wave
 #ifndef NB_FRONTIER_LDS_STACK
-#define NB_FRONTIER_LDS_STACK 352  // 8 waves per SIMD fit the LDS; a test build sets a small one
+#define NB_FRONTIER_LDS_STACK 320  // 8 waves per SIMD fit the LDS; a test build sets a small one
 #endif
 constexpr int kLStack = NB_FRONTIER_LDS_STACK;  // stack entries per wave in LDS
 static_assert(kLStack >= 8, "the root's children start in LDS");
+#ifndef NB_FRONTIER_PARTS
+#define NB_FRONTIER_PARTS 8
+#endif
+constexpr int kSubBoxes = NB_FRONTIER_PARTS;  // parts of a group with their own box
+constexpr int kSubN = 64 / kSubBoxes;         // bodies per part
+static_assert(kSubBoxes * kSubN == 64 && kSubN >= 2, "parts of a wave64 group");
 constexpr int kFWaves = kBlock / 64;
 constexpr int kFStackDefault = 4096;
 
@@ -166,13 +172,13 @@ __device__ __forceinline__ float wave_max(float v) {
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
     return v;
 }
-// minimum / maximum over the lane's 16-lane quarter
-__device__ __forceinline__ float quarter_min(float v) {
-    for (int o = 8; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+// minimum / maximum over the lane's part of kSubN consecutive lanes
+__device__ __forceinline__ float part_min(float v) {
+    for (int o = kSubN / 2; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
     return v;
 }
-__device__ __forceinline__ float quarter_max(float v) {
-    for (int o = 8; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+__device__ __forceinline__ float part_max(float v) {
+    for (int o = kSubN / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
     return v;
 }
 __device__ __forceinline__ int lanes_below(uint64_t m, unsigned lane) {
@@ -211,7 +217,7 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
     __shared__ int2 s_bl[kFWaves][kBL];
     __shared__ float4 s_grp[kFWaves][64];
     __shared__ int32_t s_stk[kFWaves][kLStack];
-    __shared__ float s_qbox[kFWaves][5][6];  // the quarters' boxes and the group's (lo xyz, hi xyz)
+    __shared__ float s_qbox[kFWaves][kSubBoxes + 1][6];  // the parts' boxes and the group's (lo xyz, hi xyz)
     const unsigned lane = __lane_id(), wv = threadIdx.x >> 6;
     // One wave per group, no loop over groups and no lane-0-only code in any
     // loop: a group handed out by one lane's atomic and broadcast by
@@ -244,12 +250,12 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
         const uint64_t active = __ballot(live);
         const int nlive = __popcll(active);
         s_grp[wv][lane] = p;
-        // the quarters' boxes (an empty quarter's box is empty: every cell passes it)
-        const float qlx = quarter_min(live ? p.x : INFINITY), qhx = quarter_max(live ? p.x : -INFINITY);
-        const float qly = quarter_min(live ? p.y : INFINITY), qhy = quarter_max(live ? p.y : -INFINITY);
-        const float qlz = quarter_min(live ? p.z : INFINITY), qhz = quarter_max(live ? p.z : -INFINITY);
-        if ((lane & 15u) == 0u) {
-            float *bx = s_qbox[wv][lane >> 4];
+        // the parts' boxes (an empty part's box is empty: every cell passes it)
+        const float qlx = part_min(live ? p.x : INFINITY), qhx = part_max(live ? p.x : -INFINITY);
+        const float qly = part_min(live ? p.y : INFINITY), qhy = part_max(live ? p.y : -INFINITY);
+        const float qlz = part_min(live ? p.z : INFINITY), qhz = part_max(live ? p.z : -INFINITY);
+        if ((lane & (unsigned)(kSubN - 1)) == 0u) {
+            float *bx = s_qbox[wv][lane / kSubN];
             bx[0] = qlx; bx[1] = qly; bx[2] = qlz;
             bx[3] = qhx; bx[4] = qhy; bx[5] = qhz;
         }
@@ -258,7 +264,7 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
             const float loy = wave_min(qly), hiy = wave_max(qhy);
             const float loz = wave_min(qlz), hiz = wave_max(qhz);
             if (lane == 0) {
-                float *bx = s_qbox[wv][4];
+                float *bx = s_qbox[wv][kSubBoxes];
                 bx[0] = lox; bx[1] = loy; bx[2] = loz;
                 bx[3] = hix; bx[4] = hiy; bx[5] = hiz;
             }
@@ -331,17 +337,17 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
             rec.r = make_int4(0, 0, 0, 0);
             if (inb) rec = wgrp[node];
             const float r2o = __int_as_float(rec.r.w);
-            const float *gb = s_qbox[wv][4];
+            const float *gb = s_qbox[wv][kSubBoxes];
             bool acc = inb && box_d2(rec.q.x, rec.q.y, rec.q.z, gb[0], gb[1], gb[2], gb[3], gb[4], gb[5]) > r2o;
             if (inb && !acc) {
-                // the whole box is only sufficient: the quarters' boxes, then
-                // body by body with the wave walk's test inside a failing quarter
+                // the whole box is only sufficient: the parts' boxes, then
+                // body by body with the wave walk's test inside a failing part
                 bool all = true;
-                for (int qq = 0; qq < 4 && all; ++qq) {
+                for (int qq = 0; qq < kSubBoxes && all; ++qq) {
                     const float *bx = s_qbox[wv][qq];
                     if (box_d2(rec.q.x, rec.q.y, rec.q.z, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]) > r2o) continue;
-                    const int i1 = min(nlive, 16 * qq + 16);
-                    for (int i = 16 * qq; i < i1; ++i) {
+                    const int i1 = min(nlive, kSubN * qq + kSubN);
+                    for (int i = kSubN * qq; i < i1; ++i) {
                         const float4 bb = s_grp[wv][i];
                         const float ex = rec.q.x - bb.x, ey = rec.q.y - bb.y, ez = rec.q.z - bb.z;
                         const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
