@@ -137,19 +137,22 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
 // tests one per lane, first against the group's bounding box (distance from
 // the cell's centre of mass to the box: if that passes, every body passes,
 // in FP32 as well, since rounding is monotonic), then against the boxes of
-// the group's four 16-body quarters, and body by body only inside a quarter
-// whose box fails.  Accepted cells and bucket ranges go to per-wave LDS lists
-// that every lane then sums with VGPR operands; opened cells push their
-// children, which the grouped record layout keeps adjacent.  The stack's top
-// kLStack entries live in LDS; a push that would overflow them moves the LDS
-// part below to the wave's global stack (popped once the LDS part is empty),
-// so global memory sees stack traffic only for deep frontiers.  Each XCD
-// takes one contiguous range of the group order, which lists the groups of
-// that range heaviest first by the previous walk's counts.  A stack that
+// the group's kSubBoxes parts (consecutive bodies), and body by body only
+// inside a part whose box fails.  Accepted cells and bucket ranges go to
+// per-wave LDS lists that every lane then sums with VGPR operands; opened
+// cells push their children, which the grouped record layout keeps adjacent.
+// The stack's top kLStack entries live in LDS; a push that would overflow
+// them moves the bottom of the LDS part onto the wave's global stack (the
+// entries nearest the top stay in LDS, and every pop takes the top of the
+// whole stack), so global memory sees stack traffic only for deep frontiers.
+// Each XCD takes one contiguous range of the group order (ranges of about
+// equal cost), which lists its groups heaviest first by the previous walk's
+// counts.  A stack that
 // would exceed `cap` entries in all sets *werr; the depth-first wave walk
 // then redoes the whole step (its launch follows, gated on that flag).
 constexpr int kIL = 128;        // accepted cells buffered per wave (also the bucket bodies' staging)
 constexpr int kBL = 64;         // bucket ranges buffered per wave
+static_assert(kIL >= 64 && kBL >= 64, "one step adds up to 64 entries to either list after a flush");
 #ifndef NB_FRONTIER_LDS_STACK
 #define NB_FRONTIER_LDS_STACK 320  // 8 waves per SIMD fit the LDS; a test build sets a small one
 #endif
@@ -378,20 +381,36 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
                 break;
             }
             const int32_t first = rec.r.x;
-            if (sp + tot > kLStack) {
-                // the LDS part moves below to the global part; the pushes go
-                // to LDS, or straight to global memory when they alone exceed it
+            if (sp + tot > kLStack && tot > kLStack) {
+                // pushes that alone exceed the LDS part: all of it moves below
+                // to the global part, and the pushes go straight after it
                 for (int32_t k = (int32_t)lane; k < sp; k += 64) stk[gsp + k] = s_stk[wv][k];
                 gsp += sp;
                 sp = 0;
-                wave_lds_sync();  // the LDS part is read before the pushes overwrite it
-                if (tot > kLStack) {
-                    for (int k = 0; k < nch; ++k) stk[gsp + excl + k] = first + k;
-                    gsp += tot;
-                } else {
-                    for (int k = 0; k < nch; ++k) s_stk[wv][excl + k] = first + k;
-                    sp = tot;
+                for (int k = 0; k < nch; ++k) stk[gsp + excl + k] = first + k;
+                gsp += tot;
+            } else if (sp + tot > kLStack) {
+                // the bottom of the LDS part moves below to the global part;
+                // its top `keep` entries (popped soonest) slide down in LDS
+                const int32_t keep = min(sp, (kLStack - tot) / 2), mv = sp - keep;
+                for (int32_t k = (int32_t)lane; k < mv; k += 64) stk[gsp + k] = s_stk[wv][k];
+                constexpr int kKeepPer = (kLStack / 2 + 63) / 64;
+                int32_t kv[kKeepPer];
+#pragma unroll
+                for (int q = 0; q < kKeepPer; ++q) {
+                    const int32_t k = (int32_t)lane + 64 * q;
+                    kv[q] = k < keep ? s_stk[wv][mv + k] : 0;
                 }
+                wave_lds_sync();  // every kept entry is read before any is rewritten
+#pragma unroll
+                for (int q = 0; q < kKeepPer; ++q) {
+                    const int32_t k = (int32_t)lane + 64 * q;
+                    if (k < keep) s_stk[wv][k] = kv[q];
+                }
+                gsp += mv;
+                sp = keep;
+                for (int k = 0; k < nch; ++k) s_stk[wv][sp + excl + k] = first + k;
+                sp += tot;
             } else {
                 for (int k = 0; k < nch; ++k) s_stk[wv][sp + excl + k] = first + k;
                 sp += tot;
