@@ -137,7 +137,7 @@ struct nb_batch {
     DBuf<double> x, v, a, m, r, e;
     DBuf<int64_t> lab, off, next, log, loglen;
     DBuf<int> cnt, ord, errf;
-    DBuf<unsigned char> flags;
+    int max_n = 0;  // largest system at creation (merges only shrink systems)
     BatchParams P{6.6743e-11, 1.0, 0.5, 0.0, 3, 1, 1e10, 1};
     int64_t step = 0;
     // HIP events around the stepping launches of the last nb_batch_evolve
@@ -153,6 +153,29 @@ struct nb_batch {
 
 namespace {
 thread_local std::string g_batch_err;
+
+// k_batch_steps instance for the batch: precision, and 64-body blocks per
+// wave for its largest system (3 covers 768 bodies, 4 the 1024 maximum)
+void launch_batch_steps(nb_batch *b, const BodyView &all, const BatchParams &P, int64_t step0, int64_t steps) {
+    const dim3 grid((unsigned)b->nsys), block(kThreads);
+    const bool small = b->max_n <= batch_capacity<3>();
+    if (P.mixed) {
+        if (small)
+            hipLaunchKernelGGL((k_batch_steps<true, 3>), grid, block, 0, b->stream, all, b->ord.p, b->off.p, b->cnt.p,
+                               b->next.p, b->log.p, b->loglen.p, b->errf.p, P, step0, steps);
+        else
+            hipLaunchKernelGGL((k_batch_steps<true, 4>), grid, block, 0, b->stream, all, b->ord.p, b->off.p, b->cnt.p,
+                               b->next.p, b->log.p, b->loglen.p, b->errf.p, P, step0, steps);
+    } else {
+        if (small)
+            hipLaunchKernelGGL((k_batch_steps<false, 3>), grid, block, 0, b->stream, all, b->ord.p, b->off.p,
+                               b->cnt.p, b->next.p, b->log.p, b->loglen.p, b->errf.p, P, step0, steps);
+        else
+            hipLaunchKernelGGL((k_batch_steps<false, 4>), grid, block, 0, b->stream, all, b->ord.p, b->off.p,
+                               b->cnt.p, b->next.p, b->log.p, b->loglen.p, b->errf.p, P, step0, steps);
+    }
+    NB_HIP(hipGetLastError());
+}
 
 int bfail(nb_batch *b, int code, const std::string &msg) {
     if (b) b->err = msg;
@@ -212,7 +235,6 @@ int nb_batch_create(int device, int64_t nsys, const int64_t *offsets, const doub
         const size_t T = (size_t)std::max<int64_t>(total, 1);
         b->x.ensure(3 * T); b->v.ensure(3 * T); b->a.ensure(3 * T);
         b->m.ensure(T); b->r.ensure(T); b->e.ensure(T); b->lab.ensure(T); b->log.ensure(4 * T);
-        b->flags.ensure(T);
         b->off.ensure((size_t)nsys + 1); b->next.ensure((size_t)nsys); b->loglen.ensure((size_t)nsys);
         b->cnt.ensure((size_t)nsys); b->ord.ensure((size_t)nsys); b->errf.ensure(1);
         std::vector<double> tmp(3 * T, 0.0);
@@ -236,6 +258,7 @@ int nb_batch_create(int device, int64_t nsys, const int64_t *offsets, const doub
             next[(size_t)s] = k;
             for (int64_t q = 0; q < k; ++q) lab[(size_t)(offsets[s] + q)] = q;
         }
+        for (int c : cnt) b->max_n = std::max(b->max_n, c);
         // longest systems first (n^2 work): the dispatcher hands out workgroups in order
         b->order.resize((size_t)nsys);
         std::iota(b->order.begin(), b->order.end(), 0);
@@ -294,9 +317,7 @@ int nb_batch_evolve(nb_batch *b, uint64_t steps, int64_t *elapsed_ns) {
             Pf.dt = 0.0;
             Pf.half_dt = 0.0;
             Pf.coll = 0;
-            hipLaunchKernelGGL(k_batch_steps, dim3((unsigned)b->nsys), dim3(kThreads), 0, b->stream, all, b->ord.p,
-                               b->off.p, b->cnt.p, b->next.p, b->flags.p, b->log.p, b->loglen.p, b->errf.p, Pf,
-                               b->step, (int64_t)1);
+            launch_batch_steps(b, all, Pf, b->step, 1);
         }
         const int64_t chunk = 1000;  // bound the duration of one launch
         if (!b->ev0) NB_HIP(hipEventCreate(&b->ev0));
@@ -305,9 +326,7 @@ int nb_batch_evolve(nb_batch *b, uint64_t steps, int64_t *elapsed_ns) {
         int64_t launches = 0;
         for (uint64_t done = 0; done < steps;) {
             const int64_t k = (int64_t)std::min<uint64_t>(chunk, steps - done);
-            hipLaunchKernelGGL(k_batch_steps, dim3((unsigned)b->nsys), dim3(kThreads), 0, b->stream, all, b->ord.p,
-                               b->off.p, b->cnt.p, b->next.p, b->flags.p, b->log.p, b->loglen.p, b->errf.p, P0,
-                               b->step, k);
+            launch_batch_steps(b, all, P0, b->step, k);
             NB_HIP(hipGetLastError());
             b->step += k;
             done += (uint64_t)k;

@@ -71,6 +71,7 @@ static inline float emu_rsqf(float x) {
 #define __shared__ static thread_local
 #define __align__(n) __attribute__((aligned(n)))
 #define __builtin_amdgcn_rsqf emu_rsqf
+#define NB_OPAQUE(x) __asm__ volatile("" : "+r"(x))
 // a barrier is named by its source line, which code duplication by the host
 // compiler cannot change
 void emu_sync(int line);
@@ -81,6 +82,12 @@ template <class T>
 static inline T atomicAdd(T *p, T v) {
     T o = *p;
     *p = o + v;
+    return o;
+}
+template <class T>
+static inline T atomicMin(T *p, T v) {
+    T o = *p;
+    if (v < o) *p = v;
     return o;
 }
 template <class T>
@@ -274,7 +281,6 @@ int main(int argc, char **argv) {
     double *x = galloc<double>("x", 3 * T), *v = galloc<double>("v", 3 * T), *a = galloc<double>("a", 3 * T);
     double *dm = galloc<double>("m", T), *dr = galloc<double>("r", T), *de = galloc<double>("e", T);
     int64_t *lab = galloc<int64_t>("labels", T), *log = galloc<int64_t>("log", 4 * T);
-    unsigned char *flags = galloc<unsigned char>("flags", T);
     int64_t *off = galloc<int64_t>("offsets", nsys + 1), *next = galloc<int64_t>("next_labels", nsys);
     int64_t *loglen = galloc<int64_t>("log_lens", nsys);
     int *cnt = galloc<int>("counts", nsys), *ord = galloc<int>("order", nsys), *err = galloc<int>("err", 1);
@@ -307,6 +313,20 @@ int main(int argc, char **argv) {
     std::vector<int64_t> by_size((size_t)nsys);
     for (int64_t s = 0; s < nsys; ++s) by_size[(size_t)s] = s;
     std::stable_sort(by_size.begin(), by_size.end(), [&](int64_t p, int64_t q) { return cnt[p] > cnt[q]; });
+    // the instance nb_batch_evolve picks: precision, and 64-body blocks per
+    // wave for the largest system
+    int max_n = 0;
+    for (int64_t s = 0; s < nsys; ++s) max_n = std::max(max_n, cnt[s]);
+    const bool small = max_n <= nbe::batch_capacity<3>();
+    auto launch = [&](const nbe::BatchParams &P, int64_t step0, int64_t k) {
+        if (mixed) {
+            if (small) nbe::k_batch_steps<true, 3>(all, ord, off, cnt, next, log, loglen, err, P, step0, k);
+            else nbe::k_batch_steps<true, 4>(all, ord, off, cnt, next, log, loglen, err, P, step0, k);
+        } else {
+            if (small) nbe::k_batch_steps<false, 3>(all, ord, off, cnt, next, log, loglen, err, P, step0, k);
+            else nbe::k_batch_steps<false, 4>(all, ord, off, cnt, next, log, loglen, err, P, step0, k);
+        }
+    };
     std::atomic<int64_t> next_sys{0}, finished{0};
     auto worker = [&]() {
         Block b;
@@ -317,13 +337,11 @@ int main(int argc, char **argv) {
             g_sys = s;
             blockIdx.x = (unsigned)s;
             g_launch = 0;
-            run_block(b, [&]() { nbe::k_batch_steps(all, ord, off, cnt, next, flags, log, loglen, err, Pf, 0, 1); });
+            run_block(b, [&]() { launch(Pf, 0, 1); });
             for (int64_t done = 0; done < steps;) {
                 const int64_t k = std::min(chunk, steps - done);
                 ++g_launch;
-                run_block(b, [&]() {
-                    nbe::k_batch_steps(all, ord, off, cnt, next, flags, log, loglen, err, P0, done, k);
-                });
+                run_block(b, [&]() { launch(P0, done, k); });
                 done += k;
             }
             const int64_t f = finished.fetch_add(1) + 1;

@@ -33,14 +33,23 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 G_SI = float(np.longdouble(66743) / np.longdouble(10 ** 15))
 
 
-@pytest.fixture(scope="module")
-def emu(tmp_path_factory):
-    out = str(tmp_path_factory.mktemp("emu") / "batch_emu")
+def build_emu(out, *defines):
     cmd = ["g++", "-std=c++17", "-O3", "-U_FORTIFY_SOURCE", "-D_FORTIFY_SOURCE=0", "-ffp-contract=off",
-           "-Wall", "-Wno-unknown-pragmas", "-I", os.path.join(ROOT, "gregpackagescpp_amd", "csrc"),
+           "-Wall", "-Wno-unknown-pragmas", *defines, "-I", os.path.join(ROOT, "gregpackagescpp_amd", "csrc"),
            os.path.join(ROOT, "tests", "emu", "batch_emu.cpp"), "-o", out, "-lpthread"]
     subprocess.run(cmd, check=True)
     return out
+
+
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    return build_emu(str(tmp_path_factory.mktemp("emu") / "batch_emu"))
+
+
+@pytest.fixture(scope="module")
+def emu_every_step(tmp_path_factory):
+    """The same kernel with the exact collision replay after every mixed step."""
+    return build_emu(str(tmp_path_factory.mktemp("emu") / "batch_emu_every"), "-DNB_BATCH_PREFILTER=0")
 
 
 def run_emu(exe, tmp, offsets, m, r, e, pos, vel, steps, *, mixed, merge=True, chunk=1000, guard="end",
@@ -120,3 +129,55 @@ def test_emu_mixed_comet_systems_clean(emu, tmp_path, guard):
     for c in st["counts"]:
         assert len(np.unique(st["ids"][k:k + c])) == c
         k += c
+
+
+def _concat(systems):
+    offsets = np.concatenate([[0], np.cumsum([len(s[0]) for s in systems])])
+    m, r, e, pos, vel = (np.concatenate([s[k] for s in systems]) for k in range(5))
+    return offsets, m, r, e, pos, vel
+
+
+@pytest.mark.parametrize("sizes", [[300, 520, 9], [800, 64]])
+def test_emu_fp64_large_systems_bit_exact(emu, tmp_path, sizes):
+    """Systems that give a wave two, three (up to 768 bodies) or four (above
+    768, the larger instance) 64-body blocks, with merges, bit for bit."""
+    systems = [rubble(n, 300 + k) for k, n in enumerate(sizes)]
+    offsets, m, r, e, pos, vel = _concat(systems)
+    st = run_emu(emu, str(tmp_path), offsets, m, r, e, pos, vel, 12, mixed=False, merge=True, chunk=5)
+    assert st["err"] == 0
+    k = 0
+    for s, (ms, rs, es, ps, vs) in enumerate(systems):
+        ref = O.kdk(ms, ps, vs, G_SI, 5.0, 12, coll=2, r=rs, e=es, dtype=np.float64)
+        c = int(st["counts"][s])
+        assert c == ref["n"]
+        assert np.array_equal(st["pos"][k:k + c], ref["pos"])
+        assert np.array_equal(st["vel"][k:k + c], ref["vel"])
+        assert np.array_equal(st["ids"][k:k + c], ref["labels"])
+        assert np.array_equal(st["log"][st["log"][:, 0] == s][:, 1:], ref["log"])
+        k += c
+    assert len(st["log"]) > 0
+
+
+def test_emu_mixed_prefilter_skips_only_clean_steps(emu, emu_every_step, tmp_path):
+    """The FP32 prefilter lets a step skip the exact replay only when no body
+    overlaps another: with the replay after every step as well, every state,
+    id and merge record is bit-identical.  Dense rubble (merges and bounces
+    from the first steps), comet systems (the largest, 513 bodies, above the
+    512 of two blocks per wave) and a system of 900 bodies (the larger
+    instance)."""
+    offsets, m, r, pos, vel, e = capi.ic_comet_ensemble(3, 4096, 8, 512)
+    sizes = np.diff(offsets)
+    order = np.argsort(-sizes, kind="stable")
+    systems = []
+    for s in np.concatenate([order[:3], order[1000:1003]]):
+        sl = slice(offsets[s], offsets[s + 1])
+        systems.append((m[sl], r[sl], e[sl], pos[sl], vel[sl]))
+    systems += [rubble(n, 400 + k) for k, n in enumerate([40, 200, 900])]
+    offs, m2, r2, e2, p2, v2 = _concat(systems)
+    for steps in (60, 400):
+        a = run_emu(emu, str(tmp_path), offs, m2, r2, e2, p2, v2, steps, mixed=True, chunk=45)
+        b = run_emu(emu_every_step, str(tmp_path), offs, m2, r2, e2, p2, v2, steps, mixed=True, chunk=45)
+        assert a["err"] == 0 and b["err"] == 0
+        for key in ("counts", "m", "r", "pos", "vel", "ids", "log"):
+            assert np.array_equal(a[key], b[key]), key
+        assert len(a["log"]) > 0
