@@ -436,6 +436,10 @@ constexpr int batch_capacity() {
 
 static_assert(batch_capacity<4>() == kBatchMaxN, "the largest instance holds the largest system");
 
+// j bodies per group in the FP32 tile loop (even)
+#ifndef NB_BATCH_UNROLL
+#define NB_BATCH_UNROLL 4
+#endif
 // 0 runs the exact replay after every mixed step as well (a CPU test build
 // uses it to show that the prefilter skips only steps without overlaps).
 #ifndef NB_BATCH_PREFILTER
@@ -474,11 +478,14 @@ constexpr float kPrefilterMargin = 0x1.0p-20f;
 // FP64: the reference's order (cumulative_acc :352-357, or bh_acc's softened
 // form), j ascending, positions and masses staged in LDS as FP64; the replay
 // runs after every step.
+//
+// Registers: the 3-block instances are held to 4 waves per SIMD (128 VGPRs;
+// the compiler spills one 8-byte value, outside the step loop); the 4-block
+// instance would spill inside the step loop at that bound and runs at 3.
 template <bool MIXED, int MAXB>
-__global__ __launch_bounds__(kThreads) void k_batch_steps(BodyView all, const int *__restrict__ order,
-                                                          const int64_t *__restrict__ offsets, int *counts,
-                                                          int64_t *next_labels, int64_t *log, int64_t *log_lens,
-                                                          int *err, BatchParams P, int64_t step0, int64_t nsteps) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MAXB <= 3 ? 4 : 1))) void k_batch_steps(
+    BodyView all, const int *__restrict__ order, const int64_t *__restrict__ offsets, int *counts,
+    int64_t *next_labels, int64_t *log, int64_t *log_lens, int *err, BatchParams P, int64_t step0, int64_t nsteps) {
     constexpr int NMAX = batch_capacity<MAXB>();
     constexpr int kStage = MIXED ? 20 * NMAX : 32 * NMAX;
     __shared__ __align__(16) unsigned char s_stage[kStage];
@@ -601,7 +608,35 @@ __global__ __launch_bounds__(kThreads) void k_batch_steps(BodyView all, const in
                         const int je = min(n, jt + 64);
                         float px = 0.f, py = 0.f, pz = 0.f;
                         if (jt != blk * 64) {
-                            for (int j = jt; j < je; ++j) {
+                            // j in groups of NB_BATCH_UNROLL (independent chains for the
+                            // latency of rsq and LDS; the maxima take three-operand maxes)
+                            int j = jt;
+                            for (; j + NB_BATCH_UNROLL - 1 < je; j += NB_BATCH_UNROLL) {
+                                float inv[NB_BATCH_UNROLL], wq[NB_BATCH_UNROLL], dq[NB_BATCH_UNROLL][3];
+#pragma unroll
+                                for (int u = 0; u < NB_BATCH_UNROLL; ++u) {
+                                    const float4 pj = s_pos[j + u];
+                                    dq[u][0] = pj.x - pi.x;
+                                    dq[u][1] = pj.y - pi.y;
+                                    dq[u][2] = pj.z - pi.z;
+                                    const float r2 = __builtin_fmaf(
+                                        dq[u][2], dq[u][2], __builtin_fmaf(dq[u][1], dq[u][1], dq[u][0] * dq[u][0]));
+                                    inv[u] = __builtin_amdgcn_rsqf(r2);
+                                    wq[u] = pj.w * inv[u] * (inv[u] * inv[u]);
+                                }
+#pragma unroll
+                                for (int u = 0; u < NB_BATCH_UNROLL; ++u) {
+                                    px = __builtin_fmaf(wq[u], dq[u][0], px);
+                                    py = __builtin_fmaf(wq[u], dq[u][1], py);
+                                    pz = __builtin_fmaf(wq[u], dq[u][2], pz);
+                                }
+#pragma unroll
+                                for (int u = 0; u < NB_BATCH_UNROLL; u += 2) {
+                                    mi = fmaxf(mi, fmaxf(inv[u], inv[u + 1]));
+                                    mr = fmaxf(mr, fmaxf(s_R[j + u] * inv[u], s_R[j + u + 1] * inv[u + 1]));
+                                }
+                            }
+                            for (; j < je; ++j) {
                                 const float4 pj = s_pos[j];
                                 const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
                                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));

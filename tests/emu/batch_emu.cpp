@@ -68,6 +68,7 @@ static inline float emu_rsqf(float x) {
 #define __device__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define amdgpu_waves_per_eu(...)
 #define __shared__ static thread_local
 #define __align__(n) __attribute__((aligned(n)))
 #define __builtin_amdgcn_rsqf emu_rsqf
