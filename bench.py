@@ -115,6 +115,9 @@ PEAK_FP64_TFLOPS = 78.6
 # and the same for the frontier walk launch; None when not profiled.
 PROFILED_BH_BUILD_BYTES = 1.381e9
 PROFILED_BH_WALK_BYTES = 2.197e8
+# the same for one 1000-step launch of the mixed batch kernel on the 4096-system
+# comet ensemble (profiles/batch_comet_ensemble4096.md)
+PROFILED_BATCH_LAUNCH_BYTES = 2.535e8
 
 
 def build_bytes(n, nodes, passes=RADIX_PASSES):
@@ -362,7 +365,8 @@ def bench_batch(capi, steps, precision=None):
             "merges": int(len(log)), "kernel_ms": k_ms, "launches": k_launches,
             "roofline": {"bound": "valu", "kernel": "k_batch_steps", "achieved": achieved, "peak": peak,
                          "unit": "TFLOP/s", "frac": achieved / peak, "flop_per_interaction": FLOP_PER_INTERACTION,
-                         "avg_launch_ms": k_ms / max(k_launches, 1), "traffic": None},
+                         "avg_launch_ms": k_ms / max(k_launches, 1),
+                         "traffic": PROFILED_BATCH_LAUNCH_BYTES if mixed else None},
             "workload": "4096 comet-string systems (seed 3): 1.9e27 kg central body + U{8..512} fragments, "
                         "dt=5 s, inelastic merging on, "
                         + ("FP32 pair forces from LDS" if mixed else "FP64 pair forces in the reference's order")
