@@ -108,6 +108,14 @@ struct BHWork {
     int64_t nodes = 0;
     uint32_t *h_total = nullptr;
     int *h_err = nullptr;
+    // the octree sort's passes over the high key (7 = all of it), chosen from
+    // whether a previous build had runs of more than 32 bodies sharing 12, 15
+    // or 18 levels (longrun, read back without a wait of its own)
+    int sort_passes = 7;
+    int builds_since_reset = 0;
+    DBuf<uint32_t> longrun;
+    uint32_t *h_longrun = nullptr;
+    hipEvent_t ev_count = nullptr;  // the node count and error flag have reached h_total / h_err
     // collision pass
     DBuf<double> nbox;      // node corners btm[3], top[3]
     DBuf<int32_t> leaf_of;  // sorted position -> leaf node
@@ -124,6 +132,8 @@ struct BHWork {
     DBuf<double> g_geo;
     uint32_t *h_bc_total = nullptr;
     ~BHWork() {
+        if (ev_count) (void)hipEventDestroy(ev_count);
+        if (h_longrun) (void)hipHostFree(h_longrun);
         if (h_total) (void)hipHostFree(h_total);
         if (h_err) (void)hipHostFree(h_err);
         if (h_bc_total) (void)hipHostFree(h_bc_total);
@@ -235,27 +245,41 @@ __global__ void k_gather_lo(int64_t n, const uint64_t *__restrict__ klo, const u
     if (s < n) out[s] = klo[idx[s]];
 }
 
-// Runs of equal high keys are ordered by the low key (insertion sort, runs are short).
-__global__ void k_fix_ties(int64_t n, const uint64_t *__restrict__ khi, uint64_t *__restrict__ klo,
+// The sort orders the high key's bits from lo_bit up; runs of bodies equal
+// in those bits are put in (high key, low key, index) order here, by one
+// insertion sort per run (the runs are short -- the pass count is chosen for
+// that -- and nearly sorted, since the input is the previous tree's order).
+// With the full high key sorted (lo_bit 0) only bodies sharing 21 levels
+// form runs, and they are ordered by the low key.
+__global__ void k_fix_ties(int64_t n, int lo_bit, uint64_t *__restrict__ khi, uint64_t *__restrict__ klo,
                            uint32_t *__restrict__ idx) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= n - 1) return;
-    if (khi[s] != khi[s + 1] || (s > 0 && khi[s - 1] == khi[s])) return;
+    const uint64_t top = khi[s] >> lo_bit;
+    if ((khi[s + 1] >> lo_bit) != top || (s > 0 && (khi[s - 1] >> lo_bit) == top)) return;
     int64_t e = s + 1;
-    while (e + 1 < n && khi[e + 1] == khi[s]) ++e;
+    while (e + 1 < n && (khi[e + 1] >> lo_bit) == top) ++e;
     for (int64_t a = s + 1; a <= e; ++a) {
-        uint64_t kl = klo[a];
-        uint32_t id = idx[a];
+        const uint64_t kh = khi[a], kl = klo[a];
+        const uint32_t id = idx[a];
         int64_t b = a - 1;
-        while (b >= s && (klo[b] > kl || (klo[b] == kl && idx[b] > id))) {
+        while (b >= s && (khi[b] > kh || (khi[b] == kh && (klo[b] > kl || (klo[b] == kl && idx[b] > id))))) {
+            khi[b + 1] = khi[b];
             klo[b + 1] = klo[b];
             idx[b + 1] = idx[b];
             --b;
         }
+        khi[b + 1] = kh;
         klo[b + 1] = kl;
         idx[b + 1] = id;
     }
 }
+
+// Octree sort passes (see BHWork::sort_passes): the candidates below the full
+// 7 are 4, 5 and 6 (12, 15 and 18 shared levels); a candidate is used when
+// the last measured runs at its level held at most 32 bodies (k_lcp_counts).
+constexpr int kMinSortPasses = 4;
+constexpr int kSortStats = 3;
 
 __device__ __forceinline__ int shared_digits(uint64_t h0, uint64_t l0, uint64_t h1, uint64_t l1) {
     if (h0 != h1) return (__clzll(h0 ^ h1) - 1) / 3;  // keys use the low 63 bits
@@ -268,11 +292,17 @@ __device__ __forceinline__ int shared_digits(uint64_t h0, uint64_t l0, uint64_t 
 // last); the nodes starting at s are the levels (c[s-1], c[s]] plus the leaf
 // of s, so cnt[s] = 1 + max(0, c[s] - c[s-1]).  Each thread also recomputes
 // c[s-1] (the same function of the same keys) so that one launch does both.
+//
+// For the next builds' sort it also flags, per level L = 3P (P = 4..6
+// passes), whether some aligned 64-position chunk holds 16 consecutive
+// positions with c >= L: a run of bodies sharing L levels longer than 32 has
+// 32 consecutive such positions, 16 of them inside one chunk, so no flag
+// means every run at that level holds at most 32 bodies.
 __global__ __launch_bounds__(kBlock) void k_lcp_counts(int64_t n, const uint64_t *__restrict__ khi,
                                                        const uint64_t *__restrict__ klo, const uint32_t *__restrict__ idx,
                                                        const double *__restrict__ x, uint8_t *__restrict__ c,
                                                        uint8_t *__restrict__ cm1, uint32_t *__restrict__ cnt,
-                                                       int *__restrict__ err) {
+                                                       int *__restrict__ err, uint32_t *__restrict__ longrun) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     int cs = 255;  // past the end: no effect on the minima below
     if (s < n) {
@@ -290,6 +320,17 @@ __global__ __launch_bounds__(kBlock) void k_lcp_counts(int64_t n, const uint64_t
         if (s > 0) cp = min(kLevels - 1, shared_digits(khi[s - 1], klo[s - 1], khi[s], klo[s]));
         c[s] = (uint8_t)cs;
         cnt[s] = 1u + (uint32_t)max(0, cs - cp);
+    }
+    // long runs at the candidate sort levels (cs is 255 past the end)
+#pragma unroll
+    for (int q = 0; q < kSortStats; ++q) {
+        const int L = 3 * (kMinSortPasses + q);
+        uint64_t r = __ballot(s < n && cs >= L);
+        r &= r >> 1;
+        r &= r >> 2;
+        r &= r >> 4;
+        r &= r >> 8;  // bit i: positions i .. i+15 of the chunk all share L levels
+        if (r != 0 && (threadIdx.x & 63) == 0) longrun[q] = 1u;
     }
     // minimum of c over each aligned chunk of 64 positions (one wave)
     int m = cs;
@@ -438,6 +479,7 @@ __device__ int64_t run_end(const RunIndex &R, int64_t s, int L) {
     return found;
 }
 
+
 // ---- nodes in DFS pre-order, and the bodies moved into tree order ----
 // Node numbering: the root is 0 and the nodes starting at sorted position s
 // are 1 + base[s] + j, internal levels (c[s-1], c[s]] first, then the leaf
@@ -450,6 +492,8 @@ __device__ int64_t run_end(const RunIndex &R, int64_t s, int L) {
 //  * depth, octant, side (top.x - btm.x replayed by set_corners' halving on
 //    x along the key's digits);
 //  * for the leaf: body = s and the leaf's mass and COM.
+// It runs before the host knows the node count: with more nodes than the
+// arrays hold (cap) it writes nothing, and the host launches it again.
 // The bodies themselves move into tree order in the same pass (every
 // per-body array, gathered through idx), with the FP32 (x, y, z, G m)
 // stream of the walk; idx becomes the identity.  Between steps the bodies
@@ -467,7 +511,8 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
     int8_t *__restrict__ octant, double *__restrict__ side, double *__restrict__ mass, double *__restrict__ com,
     int64_t cstride) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
+    const int32_t total = (int32_t)*total_p;
+    if (s >= n || (int64_t)total > cstride) return;  // cstride is the node capacity
     // bodies into tree order
     const uint32_t b = idx[s];
     const double px = x[b], py = x[n + b], pz = x[2 * n + b], pm = m[b];
@@ -480,7 +525,6 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
     pos32[s] = make_float4((float)px, (float)py, (float)pz, (float)(G * pm));
     idx[s] = (uint32_t)s;
     // nodes starting at s
-    const int32_t total = (int32_t)*total_p;
     const int cs = R.c[s], cp = s > 0 ? R.c[s - 1] : 0;
     const uint64_t h = khi[s], l = klo[s];
     const int32_t n0 = 1 + (int32_t)base[s];
@@ -1417,8 +1461,8 @@ void swap_buf(DBuf<T> &a, DBuf<T> &b) {
 }  // namespace
 
 // Builds the octree of the current positions into s->bh and leaves the bodies
-// in the tree's order (synchronises once, after the node count scan, to size
-// the node arrays and to surface build errors).
+// in the tree's order (waits once for the node count, to size the later
+// launches and to surface build errors, while k_build_nodes runs).
 static void bh_build(nb_system *s) {
     const int64_t n = s->n;
     if (!s->bh) s->bh = new BHWork();
@@ -1442,6 +1486,12 @@ static void bh_build(nb_system *s) {
     w.nroots.ensure(2);
     if (!w.h_total) NB_HIP(hipHostMalloc((void **)&w.h_total, sizeof(uint32_t), hipHostMallocDefault));
     if (!w.h_err) NB_HIP(hipHostMalloc((void **)&w.h_err, sizeof(int), hipHostMallocDefault));
+    if (!w.ev_count) NB_HIP(hipEventCreateWithFlags(&w.ev_count, hipEventDisableTiming));
+    if (!w.h_longrun) {
+        NB_HIP(hipHostMalloc((void **)&w.h_longrun, kSortStats * sizeof(uint32_t), hipHostMallocDefault));
+        for (int q = 0; q < kSortStats; ++q) w.h_longrun[q] = 1u;
+    }
+    w.longrun.ensure(kSortStats);
 
     w.khi.ensure(n); w.idx.ensure(n);  // the sort may exchange these with its own buffers
     // c and its chunk minima, padded to whole chunks of 64 for the searches' vector loads
@@ -1455,38 +1505,68 @@ static void bh_build(nb_system *s) {
                        w.idx.p, w.err.p);
     // the sort by the high key, then ties by the low key, and the node counts
     const int nb = (int)((n + 1023) / 1024);
-    w.sorter.sort_swap(st, w.khi, w.idx, n, 3 * kHiLevels);
+    const int passes = s->tune_bh_sort > 0 ? s->tune_bh_sort : w.sort_passes;
+    const int lo_bit = 3 * kHiLevels - RadixSorter::kDigitBits * passes;
+    w.sorter.sort_swap(st, w.khi, w.idx, n, 3 * kHiLevels, lo_bit);
     hipLaunchKernelGGL(k_gather_lo, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.klo_slot.p, w.idx.p, w.klo.p);
-    hipLaunchKernelGGL(k_fix_ties, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p);
+    hipLaunchKernelGGL(k_fix_ties, dim3(blocks(n)), dim3(kBlock), 0, st, n, lo_bit, w.khi.p, w.klo.p, w.idx.p);
+    NB_HIP(hipMemsetAsync(w.longrun.p, 0, kSortStats * sizeof(uint32_t), st));
     hipLaunchKernelGGL(k_lcp_counts, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p, s->x.p,
-                       w.c.p, w.cm1.p, w.cnt.p, w.err.p);
+                       w.c.p, w.cm1.p, w.cnt.p, w.err.p, w.longrun.p);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nb, w.blk.p, w.total.p, w.cm1.p, n1, w.cm2.p, n2,
                        w.cm3.p, n3);
     hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
-    NB_HIP(hipMemcpyAsync(w.h_total, w.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    NB_HIP(hipMemcpyAsync(w.h_err, w.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    NB_HIP(hipStreamSynchronize(st));
-    raise_bh_error(*w.h_err);
-    const int32_t total = (int32_t)*w.h_total;
-    if (w.cap_nodes < total) {
-        size_t t = (size_t)total + total / 4;
+    // Node arrays for at least 2n nodes (or the last build's count and a
+    // quarter), so the nodes are written while the host waits for the count;
+    // with more nodes, k_build_nodes writes nothing and runs again below.
+    auto grow_nodes = [&](int64_t want) {
+        if (w.cap_nodes >= want) return;
+        const size_t t = (size_t)want;
         w.first.ensure(t); w.last.ensure(t); w.skip.ensure(t); w.parent.ensure(t); w.body.ensure(t);
         w.depth.ensure(t); w.nchild.ensure(t); w.octant.ensure(t); w.mass.ensure(t); w.com.ensure(3 * t);
         w.side.ensure(t); w.wnode.ensure(t); w.slot.ensure(t); w.kid.ensure(t); w.roots.ensure(t); w.top.ensure(t);
         w.wgrp.ensure(t); w.rank.ensure(t); w.cbase.ensure(t); w.cblk.ensure(t / 1024 + 2); w.ctot.ensure(1);
-        w.cap_nodes = (int64_t)t;
+        w.cap_nodes = want;
+    };
+    grow_nodes(2 * n + 64);
+    const RunIndex R{w.c.p, w.cm1.p, w.cm2.p, w.cm3.p, n, n1, n2, n3};
+    // nodes, and the bodies into tree order
+    auto build_nodes = [&]() {
+        hipLaunchKernelGGL(k_build_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, R, w.cnt.p, w.total.p, w.khi.p,
+                           w.klo.p, w.box.p, w.idx.p, s->x.p, s->v.p, w.permuted ? w.mt.p : s->m.p,
+                           w.permuted ? w.rt.p : s->r.p, w.permuted ? w.et.p : s->e.p,
+                           w.permuted ? w.perm.p : nullptr, w.xa.p, w.va.p, w.mt_alt.p, w.rt_alt.p, w.et_alt.p,
+                           w.perm_alt.p, w.pos32s.p, s->G, w.first.p, w.last.p, w.skip.p, w.body.p, w.depth.p,
+                           w.octant.p, w.side.p, w.mass.p, w.com.p, w.cap_nodes);
+    };
+    // the host waits for the count only; k_build_nodes runs meanwhile
+    NB_HIP(hipMemcpyAsync(w.h_total, w.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    NB_HIP(hipMemcpyAsync(w.h_err, w.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    NB_HIP(hipEventRecord(w.ev_count, st));
+    build_nodes();
+    NB_HIP(hipEventSynchronize(w.ev_count));
+    raise_bh_error(*w.h_err);
+    // the next sort: the fewest passes whose runs were all at most 32 bodies
+    // (the flags of the last build whose copy has landed: one of this
+    // system's builds since its bodies were set)
+    if (w.builds_since_reset >= 1) {
+        int next = 3 * kHiLevels / RadixSorter::kDigitBits;
+        for (int q = kSortStats - 1; q >= 0; --q)
+            if (w.h_longrun[q] == 0) next = kMinSortPasses + q;
+            else break;
+        w.sort_passes = next;
     }
+    ++w.builds_since_reset;
+    const int32_t total = (int32_t)*w.h_total;
+    if (total > w.cap_nodes) {
+        NB_HIP(hipStreamSynchronize(st));  // the launch that wrote nothing has ended before its arrays go
+        grow_nodes((int64_t)total + total / 4);
+        build_nodes();
+    }
+    NB_HIP(hipMemcpyAsync(w.h_longrun, w.longrun.p, kSortStats * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     w.nodes = total;
     const int64_t cs = w.cap_nodes;  // stride of the SoA com array
-    // nodes, and the bodies into tree order
-    const RunIndex R{w.c.p, w.cm1.p, w.cm2.p, w.cm3.p, n, n1, n2, n3};
-    hipLaunchKernelGGL(k_build_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, R, w.cnt.p, w.total.p, w.khi.p,
-                       w.klo.p, w.box.p, w.idx.p, s->x.p, s->v.p, w.permuted ? w.mt.p : s->m.p,
-                       w.permuted ? w.rt.p : s->r.p, w.permuted ? w.et.p : s->e.p, w.permuted ? w.perm.p : nullptr,
-                       w.xa.p, w.va.p, w.mt_alt.p, w.rt_alt.p, w.et_alt.p, w.perm_alt.p,
-                       w.pos32s.p, s->G, w.first.p, w.last.p, w.skip.p, w.body.p, w.depth.p, w.octant.p,
-                       w.side.p, w.mass.p, w.com.p, cs);
     swap_buf(s->x, w.xa); swap_buf(s->v, w.va);
     swap_buf(w.mt, w.mt_alt); swap_buf(w.rt, w.rt_alt); swap_buf(w.et, w.et_alt);
     swap_buf(w.perm, w.perm_alt);
@@ -1535,7 +1615,10 @@ void bh_restore_order(nb_system *s) {
 
 // Forgets the tree order (the bodies were replaced).
 void bh_forget_order(nb_system *s) {
-    if (s->bh) s->bh->permuted = false;
+    if (!s->bh) return;
+    s->bh->permuted = false;
+    s->bh->sort_passes = 3 * kHiLevels / RadixSorter::kDigitBits;  // new bodies: the full sort until measured
+    s->bh->builds_since_reset = 0;
 }
 
 void bh_forces(nb_system *s, int64_t *interactions) {

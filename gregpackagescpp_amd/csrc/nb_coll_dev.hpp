@@ -67,48 +67,6 @@ __device__ __forceinline__ double mw(double m1, double a, double m2, double b) {
     return (m1 * a + m2 * b) / (m1 + m2);
 }
 
-// merge ctor body(one, two) (gregbod.hpp:933-940)
-__device__ Body merge_new(const Body &o, const Body &t, int64_t label) {
-    Body b;
-    b.m = o.m + t.m;
-    b.r = newton_cbrt(o.r * o.r * o.r + t.r * t.r * t.r);
-    b.x = mw(o.m, o.x, t.m, t.x); b.y = mw(o.m, o.y, t.m, t.y); b.z = mw(o.m, o.z, t.m, t.z);
-    b.vx = mw(o.m, o.vx, t.m, t.vx); b.vy = mw(o.m, o.vy, t.m, t.vy); b.vz = mw(o.m, o.vz, t.m, t.vz);
-    b.ax = mw(o.m, o.ax, t.m, t.ax); b.ay = mw(o.m, o.ay, t.m, t.ay); b.az = mw(o.m, o.az, t.m, t.az);
-    b.e = (o.e + t.e) / 2;
-    b.lab = label;
-    return b;
-}
-
-// body::operator+= (gregbod.hpp:1065-1082)
-__device__ void merge_into(Body &s, const Body &t) {
-    double x = mw(s.m, s.x, t.m, t.x), y = mw(s.m, s.y, t.m, t.y), z = mw(s.m, s.z, t.m, t.z);
-    double vx = mw(s.m, s.vx, t.m, t.vx), vy = mw(s.m, s.vy, t.m, t.vy), vz = mw(s.m, s.vz, t.m, t.vz);
-    double ax = mw(s.m, s.ax, t.m, t.ax), ay = mw(s.m, s.ay, t.m, t.ay), az = mw(s.m, s.az, t.m, t.az);
-    s.x = x; s.y = y; s.z = z; s.vx = vx; s.vy = vy; s.vz = vz; s.ax = ax; s.ay = ay; s.az = az;
-    s.e = (s.e + t.e) / 2;
-    s.m += t.m;
-    s.r = newton_cbrt(s.r * s.r * s.r + t.r * t.r * t.r);
-}
-
-// Geometry of one (outer o, inner j) pair exactly as s_coll computes it
-// (gregsys.hpp:731-743): r12 = p_j - p_o, dist = |r12|, n = r12/dist.
-struct PairGeo {
-    double dist, nx, ny, nz;
-    bool overlap;
-};
-__device__ __forceinline__ PairGeo pair_geo(const Body &o, double xj, double yj, double zj, double rj) {
-    PairGeo g;
-    double rad = o.r + rj;
-    double dx = xj - o.x, dy = yj - o.y, dz = zj - o.z;
-    g.dist = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
-    g.nx = dx / g.dist;
-    g.ny = dy / g.dist;
-    g.nz = dz / g.dist;
-    g.overlap = rad > g.dist;
-    return g;
-}
-
 struct ReplayOut {
     int64_t *log;      // 4 int64 per record
     int64_t log_cap;
@@ -224,7 +182,9 @@ __device__ void coll_replay(const BodyView &B, int &n_ref, bool merge, double mi
         int cur = outer;  // the outer body's index: moves down when a body before it is erased
         bool rescan = false;
         while (true) {
-            // ---- scan j in parallel: overlaps and the merge candidate ----
+            // ---- scan j in parallel: overlaps and the merge candidate; the
+            // pair geometry as s_coll computes it (gregsys.hpp:731-743):
+            // r12 = p_j - p_o, dist = |r12|, n = r12/dist ----
             if (t == 0) s_nent = 0;
             __syncthreads();
             const int cur_n = s_n;
@@ -277,8 +237,10 @@ __device__ void coll_replay(const BodyView &B, int &n_ref, bool merge, double mi
             const double cand_mom = s_rv[0];
             if (!(merge && cand >= 0 && cand_mom >= s_maxmom)) break;  // uniform
             // ---- merge outer with the candidate (gregsys.hpp:766-808): the
-            // merge constructor for the first, operator+= after it; the same
-            // arithmetic field by field, one field per thread ----
+            // merge constructor body(one, two) for the first (gregbod.hpp:
+            // 933-940), operator+= after it (gregbod.hpp:1065-1082) -- the
+            // same arithmetic field by field, one field per thread; only the
+            // label differs (a new one for the first merge) ----
             double nv = 0.0;
             if (t < 12) {
                 const double cm = el(B.m, cand);

@@ -24,6 +24,7 @@ constexpr int kTileItems = kThreads * kRounds;  // 2048
 constexpr int kBits = 9;
 constexpr int kBins = 1 << kBits;  // 512, one per thread
 static_assert(kBins == kThreads, "one digit bin per thread");
+static_assert(kBits == RadixSorter::kDigitBits, "the digit width the callers plan with");
 
 __global__ __launch_bounds__(kThreads) void k_hist(const uint64_t *__restrict__ keys, int n, int shift,
                                                    uint32_t *__restrict__ hist, int ntiles) {
@@ -201,16 +202,17 @@ void RadixSorter::ensure(int64_t n) {
 
 // Runs the digit passes from (keys, vals) and returns the buffers that hold
 // the result: the caller's after an even number of passes, else the
-// sorter's own.  Digits above `bits` are assumed zero.
+// sorter's own.  Digits above `bits` are assumed zero; the passes start at
+// bit lo_bit, so bits below it do not take part in the order.
 std::pair<uint64_t *, uint32_t *> RadixSorter::passes(hipStream_t s, uint64_t *keys, uint32_t *vals, int64_t n,
-                                                      int bits) {
+                                                      int bits, int lo_bit) {
     if (n <= 1) return {keys, vals};
     if (n >= ((int64_t)1 << 31) - kTileItems) throw ApiError(NB_ERR_CAPACITY, "radix sort: too many items");
     ensure(n);
     const int nt = (int)((n + kTileItems - 1) / kTileItems);
     uint64_t *ka = keys, *kb = k2.p;
     uint32_t *va = vals, *vb = v2.p;
-    for (int shift = 0; shift < bits; shift += kBits) {
+    for (int shift = lo_bit; shift < bits; shift += kBits) {
         hipLaunchKernelGGL(k_hist, dim3(nt), dim3(kThreads), 0, s, ka, (int)n, shift, hist.p, nt);
         hipLaunchKernelGGL(k_tile_scan, dim3(kBins), dim3(kThreads), 0, s, hist.p, nt, dtot.p);
         hipLaunchKernelGGL(k_scatter, dim3(nt), dim3(kThreads), 0, s, ka, va, kb, vb, (int)n, shift, hist.p, nt,
@@ -235,8 +237,9 @@ void RadixSorter::sort(hipStream_t s, uint64_t *keys, uint32_t *vals, int64_t n,
 // with the caller's instead of copying back (both hold at least n items; the
 // caller must size its buffers with ensure() before each use, as the sizes
 // travel with the exchange).
-void RadixSorter::sort_swap(hipStream_t s, DBuf<uint64_t> &keys, DBuf<uint32_t> &vals, int64_t n, int bits) {
-    const auto r = passes(s, keys.p, vals.p, n, bits);
+void RadixSorter::sort_swap(hipStream_t s, DBuf<uint64_t> &keys, DBuf<uint32_t> &vals, int64_t n, int bits,
+                            int lo_bit) {
+    const auto r = passes(s, keys.p, vals.p, n, bits, lo_bit);
     if (r.first != keys.p) {
         std::swap(keys.p, k2.p);
         std::swap(keys.n, k2.n);

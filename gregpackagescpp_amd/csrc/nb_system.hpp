@@ -55,6 +55,7 @@ struct nb_system {
     int tune_step_graphs = 0;    // 0: graphs for n <= kGraphMaxBodies, 1: always, 2: never
     int tune_bh_walk = 0;        // 0/1: frontier walk (FP32 path), 2: per-wave DFS walk
     int tune_bh_stack = 0;       // frontier walk stack entries per wave (0: engine default)
+    int tune_bh_sort = 0;        // octree sort: radix passes over the high key (4..7; 0: adaptive)
 
     // captured direct step loop (kGraphSteps steps) and what it was captured for
     nbe::StepGraph step_graph;

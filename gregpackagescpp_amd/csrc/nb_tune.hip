@@ -24,6 +24,7 @@ int *knob(nb_system *s, const char *key) {
     if (!std::strcmp(key, "bh_bucket")) return &s->tune_bh_bucket;
     if (!std::strcmp(key, "bh_walk")) return &s->tune_bh_walk;
     if (!std::strcmp(key, "bh_stack")) return &s->tune_bh_stack;
+    if (!std::strcmp(key, "bh_sort_passes")) return &s->tune_bh_sort;
     return nullptr;
 }
 
@@ -105,6 +106,10 @@ int nb_set_tuning(nb_system *s, const char *key, int64_t value) {
     }
     if (!std::strcmp(key, "bh_walk") && value != 0 && value != 1 && value != 2) {
         s->err = "bh_walk must be 1 (frontier walk) or 2 (per-wave DFS walk)";
+        return NB_ERR_INVALID_ARGUMENT;
+    }
+    if (!std::strcmp(key, "bh_sort_passes") && value != 0 && (value < 4 || value > 7)) {
+        s->err = "bh_sort_passes must be between 4 and 7";
         return NB_ERR_INVALID_ARGUMENT;
     }
     if (!std::strcmp(key, "bh_stack") && value != 0 && (value < 64 || value > 4096)) {

@@ -80,6 +80,60 @@ def test_topology_vs_reference_fixture(path):
         assert (np.abs(t["mass"] - f["mass"]) / f["mass"]).max() < 1e-13
 
 
+def clustered(seed, n_cluster, n_field, side):
+    """n_cluster bodies in a cube of the given side at (0.3, 0.3, 0.3) and
+    n_field spread over [-1, 1]^3: the cluster's bodies share about
+    log2(2.2 / side) octant levels, so a sort over fewer levels leaves one
+    long run of ties."""
+    rng = np.random.default_rng(seed)
+    pos = np.concatenate([0.3 + side * rng.random((n_cluster, 3)), rng.uniform(-1, 1, (n_field, 3))])
+    m = np.full(len(pos), 1.0 / 4096)
+    return m, pos
+
+
+@pytest.mark.parametrize("passes", [4, 5, 6])
+def test_topology_with_fewer_sort_passes(passes):
+    """The octree sort over fewer 9-bit passes of the high key (the rest is
+    ordered run by run): the same tree, bit for bit, on a well-spread system
+    and on one with a tight cluster (one run of 1500 bodies sharing about 14
+    levels in input order, so the run is fixed from scratch)."""
+    print(f"case sort passes={passes}", flush=True)
+    for m, pos in (capi.ic_two_plummer(2, 32768)[:2], clustered(3, 1500, 2500, 1e-4)):
+        s = system(m, pos)
+        s.set_tuning("bh_sort_passes", passes)
+        t = s.tree()
+        o = O.bh_tree(m, pos)
+        assert len(t["depth"]) == len(o["depth"])
+        for k in ("depth", "octant", "body", "parent"):
+            assert np.array_equal(t[k], o[k]), k
+        assert np.array_equal(t["mass"], o["mass"])
+        s.close()
+
+
+def test_topology_after_adaptive_sort_steps():
+    """After BH steps the sort has dropped to fewer passes (its runs at 12
+    levels are short here); the tree of the evolved positions still equals
+    the oracle's, and a tight cluster set afterwards gets the full sort
+    again."""
+    m, pos, vel = capi.ic_two_plummer(6, 65536)
+    s = system(m, pos, vel, prec=capi.PREC_MIXED)
+    s.set_iterations(4)
+    s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)
+    st = s.state()
+    t = s.tree()
+    o = O.bh_tree(m, st["pos"])
+    assert len(t["depth"]) == len(o["depth"])
+    for k in ("depth", "octant", "body", "parent"):
+        assert np.array_equal(t[k], o[k]), k
+    mc, pc = clustered(4, 3000, 1000, 1e-5)
+    s.set_bodies(mc, pc, np.zeros_like(pc))
+    t = s.tree()
+    o = O.bh_tree(mc, pc)
+    for k in ("depth", "octant", "body", "parent"):
+        assert np.array_equal(t[k], o[k]), k
+    s.close()
+
+
 def test_full_size_topology():
     """Two 524,288-body Plummer spheres (the 1M workload): node count and leaf
     membership bit-exact against the oracle's insertion-built tree."""
