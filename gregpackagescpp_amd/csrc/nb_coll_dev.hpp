@@ -402,8 +402,8 @@ static_assert(batch_capacity<4>() == kBatchMaxN, "the largest instance holds the
 #ifndef NB_BATCH_UNROLL
 #define NB_BATCH_UNROLL 4
 #endif
-// 0 runs the exact replay after every mixed step as well (a CPU test build
-// uses it to show that the prefilter skips only steps without overlaps).
+// 0 runs the exact replay after every step as well (a CPU test build uses it
+// to show that the prefilter skips only steps without overlaps).
 #ifndef NB_BATCH_PREFILTER
 #define NB_BATCH_PREFILTER 1
 #endif
@@ -438,8 +438,9 @@ constexpr float kPrefilterMargin = 0x1.0p-20f;
 // softening the prefilter is the squared-distance count d2 < (R_i + R_j)^2.
 //
 // FP64: the reference's order (cumulative_acc :352-357, or bh_acc's softened
-// form), j ascending, positions and masses staged in LDS as FP64; the replay
-// runs after every step.
+// form), j ascending, positions, masses and radii staged in LDS as FP64; the
+// pair loop also takes the replay's own squared overlap test, so the replay
+// runs on steps where some pair passes it.
 //
 // Registers: the 3-block instances are held to 4 waves per SIMD (128 VGPRs;
 // the compiler spills one 8-byte value, outside the step loop); the 4-block
@@ -449,7 +450,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MAXB <
     BodyView all, const int *__restrict__ order, const int64_t *__restrict__ offsets, int *counts,
     int64_t *next_labels, int64_t *log, int64_t *log_lens, int *err, BatchParams P, int64_t step0, int64_t nsteps) {
     constexpr int NMAX = batch_capacity<MAXB>();
-    constexpr int kStage = MIXED ? 20 * NMAX : 32 * NMAX;
+    constexpr int kStage = MIXED ? 20 * NMAX : 40 * NMAX;
     __shared__ __align__(16) unsigned char s_stage[kStage];
     __shared__ unsigned char s_pref[NMAX], s_exact[NMAX];
     __shared__ double s_ref[3];
@@ -457,8 +458,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MAXB <
     __shared__ int64_t s_next;
     float4 *const s_pos = (float4 *)s_stage;                 // MIXED: x', y', z', G m
     float *const s_R = (float *)(s_stage + 16 * NMAX);        // MIXED: prefilter radius
-    double *const s_x = (double *)s_stage;                    // FP64: x, y, z, m blocks
-    double *const s_y = s_x + NMAX, *const s_z = s_y + NMAX, *const s_m = s_z + NMAX;
+    double *const s_x = (double *)s_stage;                    // FP64: x, y, z, m, r blocks
+    double *const s_y = s_x + NMAX, *const s_z = s_y + NMAX, *const s_m = s_z + NMAX, *const s_r = s_m + NMAX;
 
     const int sys = order[blockIdx.x];
     const int64_t off = offsets[sys];
@@ -506,6 +507,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MAXB <
                     }
                 } else {
                     s_m[i] = at(B.m, o);
+                    s_r[i] = at(B.r, o);
                 }
             }
         }
@@ -686,15 +688,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MAXB <
                     flagged |= flag;
                 }
             } else {
-                const double xi = X[k][0], yi = X[k][1], zi = X[k][2];
+                // The squared distance of each pair is the collision test's own
+                // (the same operands and order), so the replay's squared test
+                // d2 < (r_i + r_j)^2 (1 + 1e-9), which every overlap passes,
+                // flags the bodies the exact test must look at.
+                const double xi = X[k][0], yi = X[k][1], zi = X[k][2], ri = s_r[i];
                 ax = 0.0;
                 ay = 0.0;
                 az = 0.0;
+                bool hit = false;
                 if (P.eps2 == 0.0) {
                     for (int j = 0; j < n; ++j) {
                         if (j == i) continue;
                         const double dx = s_x[j] - xi, dy = s_y[j] - yi, dz = s_z[j] - zi;
                         const double s = dx * dx + dy * dy + dz * dz;
+                        const double rs = ri + s_r[j];
+                        hit = hit || s < rs * rs * (1.0 + 1e-9);
                         const double wx = s * dx, wy = s * dy, wz = s * dz;
                         const double r3 = __builtin_sqrt(wx * wx + wy * wy + wz * wz);
                         const double kf = (P.G * s_m[j]) / r3;
@@ -704,11 +713,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MAXB <
                     for (int j = 0; j < n; ++j) {
                         if (j == i) continue;
                         const double dx = s_x[j] - xi, dy = s_y[j] - yi, dz = s_z[j] - zi;
-                        const double s = dx * dx + dy * dy + dz * dz + P.eps2;
+                        const double d2 = dx * dx + dy * dy + dz * dz;
+                        const double rs = ri + s_r[j];
+                        hit = hit || d2 < rs * rs * (1.0 + 1e-9);
+                        const double s = d2 + P.eps2;
                         const double sq = __builtin_sqrt(s);
                         const double kf = P.G * (s_m[j] / s);
                         ax += kf * (dx / sq); ay += kf * (dy / sq); az += kf * (dz / sq);
                     }
+                }
+                if (valid) {
+                    s_pref[i] = hit ? 1 : 0;
+                    flagged |= hit ? 1 : 0;
                 }
             }
             if (valid) {
@@ -727,7 +743,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MAXB <
         }
         // every read of this step's staged positions is done before the next step writes them
         const int any = __syncthreads_or(flagged);
-        constexpr bool use_pref = MIXED && NB_BATCH_PREFILTER;
+        constexpr bool use_pref = NB_BATCH_PREFILTER;
         if (P.coll && (any || !use_pref)) {
             store_state();
             __syncthreads();
