@@ -107,20 +107,19 @@ BH_BODIES = 1 << 20
 BH_THETA = 0.5
 
 
-RADIX_PASSES = 7  # 9-bit LSD passes over the 63-bit high key of the octree sort
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 summed over the build kernels
 # of one BH step at 1,048,576 bodies (profiles/bh_walk_build_two_plummer_1M.md),
 # and the same for the frontier walk launch; None when not profiled.
-PROFILED_BH_BUILD_BYTES = 1.381e9
+PROFILED_BH_BUILD_BYTES = 1.181e9
 PROFILED_BH_WALK_BYTES = 2.197e8
 # the same for one 1000-step launch of the mixed batch kernel on the 4096-system
 # comet ensemble (profiles/batch_comet_ensemble4096.md)
 PROFILED_BATCH_LAUNCH_BYTES = 2.535e8
 
 
-def build_bytes(n, nodes, passes=RADIX_PASSES):
+def build_bytes(n, nodes, passes):
     """Algorithmic HBM bytes of one octree build: per body 32 B read (x, y,
     z, m) and 12 B written (key, index); per radix pass 24 B per body (key
     and index read and written); 48 B per node written; 32 B per body for
@@ -158,10 +157,11 @@ def bench_bh(capi, steps):
     s.profile(False)
     _, walk_cnt = s.bh_accelerations()
     nodes = s.tree_count()
+    passes = s.tuning("bh_sort_passes")  # 9-bit radix passes the engine's sort takes in these builds
     bucket = s.tuning("bh_bucket")
     s.close()
     build_ms, walk_ms = b_ms / max(nb, 1), w_ms / max(nw, 1)
-    bb = build_bytes(BH_BODIES, nodes)
+    bb = build_bytes(BH_BODIES, nodes, passes)
     build_gbs = bb / (build_ms * 1e-3) / 1e9
     walk_tf = FLOP_PER_INTERACTION * i_alg / (walk_ms * 1e-3) / 1e12
     return {"value": i_alg / t, "unit": "interactions/s", "bodies": BH_BODIES, "theta": BH_THETA,
@@ -170,7 +170,7 @@ def bench_bh(capi, steps):
             "build_ms": build_ms, "walk_ms": walk_ms, "steps": steps,
             "build_roofline": {"bound": "hbm", "achieved": build_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                "frac": build_gbs / PEAK_HBM_GBS, "algorithmic_bytes": bb,
-                               "radix_passes": RADIX_PASSES, "traffic": PROFILED_BH_BUILD_BYTES},
+                               "radix_passes": passes, "traffic": PROFILED_BH_BUILD_BYTES},
             "walk_roofline": {"bound": "valu", "achieved": walk_tf, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                               "frac": walk_tf / PEAK_FP32_TFLOPS, "flop_per_interaction": FLOP_PER_INTERACTION,
                               "evaluated_tflops": FLOP_PER_INTERACTION * walk_cnt / (walk_ms * 1e-3) / 1e12,

@@ -63,14 +63,16 @@ struct BHWork {
     int64_t cap_n = 0, cap_nodes = 0;
     DBuf<double> part;      // bbox partials
     DBuf<double> box;       // btm[3], top[3]
-    DBuf<int> err;          // BH_ERR_*
+    // the build's device scalars, in one array so that one kernel clears them
+    // and one copy brings the ones the host needs back: the node count, the
+    // error (BH_ERR_*), the long-run flags of the sort levels and the root counts
+    DBuf<uint32_t> scal;
     DBuf<uint64_t> khi, klo, klo_slot;  // high keys and low keys in sorted order; low keys by body slot
     DBuf<uint32_t> idx;     // sorted position -> body slot (the identity once the bodies are in tree order)
     DBuf<uint8_t> c;        // shared digits of sorted neighbours
     DBuf<uint8_t> cm1, cm2, cm3;  // minima of c over chunks of 64, 4096 and 262144 positions
     DBuf<uint32_t> cnt;     // nodes starting at s, then their exclusive scan
     DBuf<uint32_t> blk;     // scan block sums
-    DBuf<uint32_t> total;   // node count
     // the bodies in tree order: slot -> input index, and the other copy of
     // every per-body array
     bool permuted = false;
@@ -93,7 +95,6 @@ struct BHWork {
     DBuf<int32_t> kid;      // node at each position of the grouped layout (children of x: 1 + cbase[x] ...)
     DBuf<uint32_t> cblk, ctot;
     DBuf<int32_t> roots, top;  // subtree roots and the nodes above them (bottom-up pass)
-    DBuf<uint32_t> nroots;     // their counts
     DBuf<int32_t> fstack;   // frontier walk: one node stack per wave
     DBuf<int64_t> gxr;      // frontier walk: the XCD ranges of the group order
     DBuf<int> werr;         // frontier walk: a stack would have overflowed
@@ -106,16 +107,12 @@ struct BHWork {
     DBuf<unsigned long long> icount;
     RadixSorter sorter;
     int64_t nodes = 0;
-    uint32_t *h_total = nullptr;
-    int *h_err = nullptr;
+    uint32_t *h_scal = nullptr;  // kScHost leading entries of scal
     // the octree sort's passes over the high key (7 = all of it), chosen from
-    // whether a previous build had runs of more than 32 bodies sharing 12, 15
-    // or 18 levels (longrun, read back without a wait of its own)
+    // whether the last build had runs of more than 32 bodies sharing 12, 15
+    // or 18 levels
     int sort_passes = 7;
-    int builds_since_reset = 0;
-    DBuf<uint32_t> longrun;
-    uint32_t *h_longrun = nullptr;
-    hipEvent_t ev_count = nullptr;  // the node count and error flag have reached h_total / h_err
+    hipEvent_t ev_count = nullptr;  // the node count, error and flags have reached h_scal
     // collision pass
     DBuf<double> nbox;      // node corners btm[3], top[3]
     DBuf<int32_t> leaf_of;  // sorted position -> leaf node
@@ -133,9 +130,7 @@ struct BHWork {
     uint32_t *h_bc_total = nullptr;
     ~BHWork() {
         if (ev_count) (void)hipEventDestroy(ev_count);
-        if (h_longrun) (void)hipHostFree(h_longrun);
-        if (h_total) (void)hipHostFree(h_total);
-        if (h_err) (void)hipHostFree(h_err);
+        if (h_scal) (void)hipHostFree(h_scal);
         if (h_bc_total) (void)hipHostFree(h_bc_total);
     }
 };
@@ -143,6 +138,15 @@ struct BHWork {
 void bh_work_free(BHWork *w) { delete w; }
 
 namespace {
+
+// Octree sort passes (see BHWork::sort_passes): the candidates below the full
+// 7 are 4, 5 and 6 (12, 15 and 18 shared levels); a candidate is used when
+// the last measured runs at its level held at most 32 bodies (k_lcp_counts).
+constexpr int kMinSortPasses = 4;
+constexpr int kSortStats = 3;
+// slots of BHWork::scal
+constexpr int kScTotal = 0, kScErr = 1, kScLong = 2, kScRoots = kScLong + kSortStats, kScalars = kScRoots + 2;
+constexpr int kScHost = kScRoots;  // total, error and long-run flags come back to the host
 
 // ---- 1. bounding cube ----
 __global__ __launch_bounds__(kBlock) void k_bbox_partial(int64_t n, const double *__restrict__ x,
@@ -174,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(int64_t n, const double
 // find_min_max (greg8tree.hpp:1012-1047): longest edge with the reference's
 // tie rule, pad = L/20, btm = min - pad, L' = L + 2*pad, top = btm + L'.
 __global__ __launch_bounds__(kBlock) void k_bbox_final(int nparts, const double *__restrict__ part,
-                                                       double *__restrict__ box) {
+                                                       double *__restrict__ box, uint32_t *__restrict__ scal) {
     __shared__ double sm[6][kBlock];
     double lmn[3] = {INFINITY, INFINITY, INFINITY}, lmx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int b = threadIdx.x; b < nparts; b += kBlock)
@@ -203,6 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(int nparts, const double 
     for (int c = 0; c < 3; ++c) box[c] = mn[c] - pad;
     le += 2 * pad;
     for (int c = 0; c < 3; ++c) box[3 + c] = box[c] + le;
+    for (int q = kScErr; q < kScalars; ++q) scal[q] = 0u;  // the build's counters and flags start at zero
 }
 
 // ---- 2. octant digits by the reference's midpoint recursion ----
@@ -275,11 +280,6 @@ __global__ void k_fix_ties(int64_t n, int lo_bit, uint64_t *__restrict__ khi, ui
     }
 }
 
-// Octree sort passes (see BHWork::sort_passes): the candidates below the full
-// 7 are 4, 5 and 6 (12, 15 and 18 shared levels); a candidate is used when
-// the last measured runs at its level held at most 32 bodies (k_lcp_counts).
-constexpr int kMinSortPasses = 4;
-constexpr int kSortStats = 3;
 
 __device__ __forceinline__ int shared_digits(uint64_t h0, uint64_t l0, uint64_t h1, uint64_t l1) {
     if (h0 != h1) return (__clzll(h0 ^ h1) - 1) / 3;  // keys use the low 63 bits
@@ -1482,16 +1482,11 @@ static void bh_build(nb_system *s) {
     // the other copy of every per-body array, for the move into tree order
     w.xa.ensure(3 * (size_t)n); w.va.ensure(3 * (size_t)n); w.aa.ensure(3 * (size_t)n);
     for (DBuf<double> *b : {&w.mt, &w.rt, &w.et, &w.mt_alt, &w.rt_alt, &w.et_alt}) b->ensure((size_t)n);
-    w.part.ensure(6 * 1024); w.box.ensure(6); w.err.ensure(1); w.total.ensure(1); w.icount.ensure(1);
-    w.nroots.ensure(2);
-    if (!w.h_total) NB_HIP(hipHostMalloc((void **)&w.h_total, sizeof(uint32_t), hipHostMallocDefault));
-    if (!w.h_err) NB_HIP(hipHostMalloc((void **)&w.h_err, sizeof(int), hipHostMallocDefault));
+    w.part.ensure(6 * 1024); w.box.ensure(6); w.scal.ensure(kScalars); w.icount.ensure(1);
+    if (!w.h_scal) NB_HIP(hipHostMalloc((void **)&w.h_scal, kScHost * sizeof(uint32_t), hipHostMallocDefault));
     if (!w.ev_count) NB_HIP(hipEventCreateWithFlags(&w.ev_count, hipEventDisableTiming));
-    if (!w.h_longrun) {
-        NB_HIP(hipHostMalloc((void **)&w.h_longrun, kSortStats * sizeof(uint32_t), hipHostMallocDefault));
-        for (int q = 0; q < kSortStats; ++q) w.h_longrun[q] = 1u;
-    }
-    w.longrun.ensure(kSortStats);
+    uint32_t *const total_d = w.scal.p + kScTotal, *const nroots_d = w.scal.p + kScRoots;
+    int *const err_d = (int *)(w.scal.p + kScErr);
 
     w.khi.ensure(n); w.idx.ensure(n);  // the sort may exchange these with its own buffers
     // c and its chunk minima, padded to whole chunks of 64 for the searches' vector loads
@@ -1499,10 +1494,9 @@ static void bh_build(nb_system *s) {
     w.c.ensure((size_t)n + 64); w.cm1.ensure((size_t)n1 + 64); w.cm2.ensure((size_t)n2 + 64); w.cm3.ensure((size_t)n3 + 64);
     const int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, st, n, s->x.p, w.part.p);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, st, nparts, w.part.p, w.box.p);
-    NB_HIP(hipMemsetAsync(w.err.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, st, nparts, w.part.p, w.box.p, w.scal.p);
     hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kBlock), 0, st, n, s->x.p, w.box.p, w.khi.p, w.klo_slot.p,
-                       w.idx.p, w.err.p);
+                       w.idx.p, err_d);
     // the sort by the high key, then ties by the low key, and the node counts
     const int nb = (int)((n + 1023) / 1024);
     const int passes = s->tune_bh_sort > 0 ? s->tune_bh_sort : w.sort_passes;
@@ -1510,11 +1504,10 @@ static void bh_build(nb_system *s) {
     w.sorter.sort_swap(st, w.khi, w.idx, n, 3 * kHiLevels, lo_bit);
     hipLaunchKernelGGL(k_gather_lo, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.klo_slot.p, w.idx.p, w.klo.p);
     hipLaunchKernelGGL(k_fix_ties, dim3(blocks(n)), dim3(kBlock), 0, st, n, lo_bit, w.khi.p, w.klo.p, w.idx.p);
-    NB_HIP(hipMemsetAsync(w.longrun.p, 0, kSortStats * sizeof(uint32_t), st));
     hipLaunchKernelGGL(k_lcp_counts, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p, s->x.p,
-                       w.c.p, w.cm1.p, w.cnt.p, w.err.p, w.longrun.p);
+                       w.c.p, w.cm1.p, w.cnt.p, err_d, w.scal.p + kScLong);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nb, w.blk.p, w.total.p, w.cm1.p, n1, w.cm2.p, n2,
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nb, w.blk.p, total_d, w.cm1.p, n1, w.cm2.p, n2,
                        w.cm3.p, n3);
     hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
     // Node arrays for at least 2n nodes (or the last build's count and a
@@ -1533,7 +1526,7 @@ static void bh_build(nb_system *s) {
     const RunIndex R{w.c.p, w.cm1.p, w.cm2.p, w.cm3.p, n, n1, n2, n3};
     // nodes, and the bodies into tree order
     auto build_nodes = [&]() {
-        hipLaunchKernelGGL(k_build_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, R, w.cnt.p, w.total.p, w.khi.p,
+        hipLaunchKernelGGL(k_build_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, R, w.cnt.p, total_d, w.khi.p,
                            w.klo.p, w.box.p, w.idx.p, s->x.p, s->v.p, w.permuted ? w.mt.p : s->m.p,
                            w.permuted ? w.rt.p : s->r.p, w.permuted ? w.et.p : s->e.p,
                            w.permuted ? w.perm.p : nullptr, w.xa.p, w.va.p, w.mt_alt.p, w.rt_alt.p, w.et_alt.p,
@@ -1541,30 +1534,25 @@ static void bh_build(nb_system *s) {
                            w.octant.p, w.side.p, w.mass.p, w.com.p, w.cap_nodes);
     };
     // the host waits for the count only; k_build_nodes runs meanwhile
-    NB_HIP(hipMemcpyAsync(w.h_total, w.total.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    NB_HIP(hipMemcpyAsync(w.h_err, w.err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    NB_HIP(hipMemcpyAsync(w.h_scal, w.scal.p, kScHost * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     NB_HIP(hipEventRecord(w.ev_count, st));
     build_nodes();
     NB_HIP(hipEventSynchronize(w.ev_count));
-    raise_bh_error(*w.h_err);
-    // the next sort: the fewest passes whose runs were all at most 32 bodies
-    // (the flags of the last build whose copy has landed: one of this
-    // system's builds since its bodies were set)
-    if (w.builds_since_reset >= 1) {
+    raise_bh_error((int)w.h_scal[kScErr]);
+    // the next sort: the fewest passes whose runs in this build were all at most 32 bodies
+    {
         int next = 3 * kHiLevels / RadixSorter::kDigitBits;
         for (int q = kSortStats - 1; q >= 0; --q)
-            if (w.h_longrun[q] == 0) next = kMinSortPasses + q;
+            if (w.h_scal[kScLong + q] == 0) next = kMinSortPasses + q;
             else break;
         w.sort_passes = next;
     }
-    ++w.builds_since_reset;
-    const int32_t total = (int32_t)*w.h_total;
+    const int32_t total = (int32_t)w.h_scal[kScTotal];
     if (total > w.cap_nodes) {
         NB_HIP(hipStreamSynchronize(st));  // the launch that wrote nothing has ended before its arrays go
         grow_nodes((int64_t)total + total / 4);
         build_nodes();
     }
-    NB_HIP(hipMemcpyAsync(w.h_longrun, w.longrun.p, kSortStats * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     w.nodes = total;
     const int64_t cs = w.cap_nodes;  // stride of the SoA com array
     swap_buf(s->x, w.xa); swap_buf(s->v, w.va);
@@ -1579,19 +1567,18 @@ static void bh_build(nb_system *s) {
     hipLaunchKernelGGL(k_scan_blocks, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nbn, w.cblk.p, w.ctot.p);
     hipLaunchKernelGGL(k_scan_add, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
-    NB_HIP(hipMemsetAsync(w.nroots.p, 0, 2 * sizeof(uint32_t), st));
     const bool records = s->precision == NB_PREC_MIXED;
     const WalkRecs W{records ? w.wnode.p : nullptr, records && s->tune_bh_walk != 2 ? w.wgrp.p : nullptr, w.slot.p,
                      w.cbase.p, w.nchild.p, s->G, s->theta, total};
     hipLaunchKernelGGL(k_link, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.parent.p, w.skip.p, w.rank.p,
-                       w.cbase.p, w.slot.p, w.kid.p, w.roots.p, w.nroots.p, w.top.p, w.nroots.p + 1, W, cs, w.mass.p,
+                       w.cbase.p, w.slot.p, w.kid.p, w.roots.p, nroots_d, w.top.p, nroots_d + 1, W, cs, w.mass.p,
                        w.com.p, w.side.p, w.body.p, w.first.p, w.last.p);
     // mass, COM and the internal nodes' walk records, bottom up
     const unsigned sub_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, total / 128 + 1));
-    hipLaunchKernelGGL(k_com_subtrees, dim3(sub_grid), dim3(kBlock), 0, st, total, w.roots.p, w.nroots.p, w.skip.p,
+    hipLaunchKernelGGL(k_com_subtrees, dim3(sub_grid), dim3(kBlock), 0, st, total, w.roots.p, nroots_d, w.skip.p,
                        w.body.p, w.depth.p, w.nchild.p, w.cbase.p, w.kid.p, w.mass.p, w.com.p, cs, W, w.side.p,
                        w.first.p, w.last.p);
-    hipLaunchKernelGGL(k_com_top, dim3(1), dim3(1024), 0, st, total, w.top.p, w.nroots.p + 1, w.skip.p, w.body.p,
+    hipLaunchKernelGGL(k_com_top, dim3(1), dim3(1024), 0, st, total, w.top.p, nroots_d + 1, w.skip.p, w.body.p,
                        w.depth.p, w.nchild.p, w.cbase.p, w.kid.p, w.mass.p, w.com.p, cs, W, w.side.p, w.first.p,
                        w.last.p);
     NB_HIP(hipGetLastError());
@@ -1614,11 +1601,14 @@ void bh_restore_order(nb_system *s) {
 }
 
 // Forgets the tree order (the bodies were replaced).
+int bh_next_sort_passes(const nb_system *s) {
+    return s->bh ? s->bh->sort_passes : 3 * kHiLevels / RadixSorter::kDigitBits;
+}
+
 void bh_forget_order(nb_system *s) {
     if (!s->bh) return;
     s->bh->permuted = false;
     s->bh->sort_passes = 3 * kHiLevels / RadixSorter::kDigitBits;  // new bodies: the full sort until measured
-    s->bh->builds_since_reset = 0;
 }
 
 void bh_forces(nb_system *s, int64_t *interactions) {

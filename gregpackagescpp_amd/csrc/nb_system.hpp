@@ -104,6 +104,7 @@ void bh_steps(nb_system *s, uint64_t steps);
 // in input order (every entry point does so before it returns)
 void bh_restore_order(nb_system *s);
 void bh_forget_order(nb_system *s);  // the bodies were replaced
+int bh_next_sort_passes(const nb_system *s);  // radix passes the next octree sort takes when adaptive
 inline void restore_order_quietly(nb_system *s) {
     try {
         bh_restore_order(s);

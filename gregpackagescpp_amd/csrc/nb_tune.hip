@@ -132,6 +132,8 @@ int nb_get_tuning(const nb_system *cs, const char *key, int64_t *value) {
     int *k = knob(s, key);
     if (!k) return NB_ERR_INVALID_ARGUMENT;
     *value = *k;
+    // the adaptive sort reports the passes its next build takes
+    if (!std::strcmp(key, "bh_sort_passes") && *k == 0) *value = bh_next_sort_passes(s);
     return NB_OK;
 }
 

@@ -299,7 +299,8 @@ int nb_time_elapsed(const nb_system *sys, double *t);
  *                  depth-first (same cells accepted)
  *   "bh_sort_passes" 9-bit radix passes of the octree sort over the high key
  *                  (4..7; the default adapts them to how many bodies shared
- *                  the coarser cells in the last builds; the tree is the same)
+ *                  the coarser cells in the last build, and nb_get_tuning then
+ *                  reports the passes the next build takes; the tree is the same)
  *   "step_graphs"  1 capture step loops of small systems in HIP graphs, 2 off */
 int nb_set_tuning(nb_system *sys, const char *key, int64_t value);
 int nb_get_tuning(const nb_system *sys, const char *key, int64_t *value);
