@@ -63,6 +63,8 @@ void collision_pass(nb_system *s, uint64_t step) {
     CollWork &w = *s->coll;
     const int64_t n = s->n;
     hipStream_t st = s->stream;
+    // the pass addresses bodies by 32-bit byte offsets (el() in nb_coll_dev.hpp)
+    if (n >= ((int64_t)1 << 28)) throw ApiError(NB_ERR_CAPACITY, "collision pass: at most 2^28 bodies");
     // labels and the next label travel with every pass (the host copies are authoritative)
     w.lab.ensure((size_t)std::max<int64_t>(n, 1));
     w.loglen.ensure(1);

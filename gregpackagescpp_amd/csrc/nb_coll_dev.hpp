@@ -24,7 +24,8 @@ struct BodyView {
 // Element i of a body array, addressed as the array base plus a 32-bit byte
 // offset: the loads and stores then take the uniform base in SGPRs and one
 // VGPR of offset for every array, instead of a 64-bit address per array.
-// Bodies per system stay below 2^28 (the callers check).
+// Bodies per system stay below 2^28: collision_pass checks it, and a batched
+// system holds at most kBatchMaxN.
 template <class T>
 __device__ __forceinline__ T &el(T *base, int i) {
     return *(T *)((char *)base + (unsigned)i * (unsigned)sizeof(T));
