@@ -1003,15 +1003,25 @@ __global__ void k_link(int32_t total, const int32_t *__restrict__ parent, const 
                        uint32_t *__restrict__ nroots, int32_t *__restrict__ top, uint32_t *__restrict__ ntop,
                        WalkRecs W, int64_t cstride, const double *__restrict__ mass, const double *__restrict__ com,
                        const double *__restrict__ side, const int32_t *__restrict__ body,
-                       const int32_t *__restrict__ first, const int32_t *__restrict__ last) {
+                       const int32_t *__restrict__ first, const int32_t *__restrict__ last,
+                       const float4 *__restrict__ pos32) {
     const int32_t x = blockIdx.x * kBlock + threadIdx.x;
     if (x >= total) return;
     const int32_t p = parent[x];
     const uint32_t at = x == 0 ? 0u : (p >= 0 && p < total ? 1u + cbase[p] + rank[x] : 0u);
     slot[x] = at;
     if (at < (uint32_t)total && (x == 0 || at > 0)) kid[at] = x;
-    if (body[x] >= 0) {
-        if (W.wnode) write_walk_recs(W, x, cstride, mass, com, side, body, skip, first, last);
+    const int32_t b = body[x];
+    if (b >= 0) {
+        // A leaf's records are put_walk_recs' for body b: its COM and G m in
+        // FP32 are the body's FP32 position stream entry, skip x + 1, one
+        // body, no children, and an opening radius that accepts it always.
+        if (W.wnode) {
+            const float4 q = pos32[b];
+            const int r2o = __float_as_int(-1.0f);
+            W.wnode[x] = WalkNode{q, make_int4(x + 1, b, 1, r2o)};
+            if (W.wgrp && at < (uint32_t)total) W.wgrp[at] = WalkNode{q, make_int4(0, b, 1 << 4, r2o)};
+        }
         return;
     }
     const int32_t size = skip[x] - x;
@@ -1572,7 +1582,7 @@ static void bh_build(nb_system *s) {
                      w.cbase.p, w.nchild.p, s->G, s->theta, total};
     hipLaunchKernelGGL(k_link, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.parent.p, w.skip.p, w.rank.p,
                        w.cbase.p, w.slot.p, w.kid.p, w.roots.p, nroots_d, w.top.p, nroots_d + 1, W, cs, w.mass.p,
-                       w.com.p, w.side.p, w.body.p, w.first.p, w.last.p);
+                       w.com.p, w.side.p, w.body.p, w.first.p, w.last.p, w.pos32s.p);
     // mass, COM and the internal nodes' walk records, bottom up
     const unsigned sub_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, total / 128 + 1));
     hipLaunchKernelGGL(k_com_subtrees, dim3(sub_grid), dim3(kBlock), 0, st, total, w.roots.p, nroots_d, w.skip.p,
