@@ -111,6 +111,31 @@ def test_batch_fp64_bit_exact_vs_oracle():
     assert len(log) > 0
 
 
+@pytest.mark.parametrize("sizes", [[300, 520, 9], [800, 64]])
+def test_batch_fp64_large_systems_bit_exact(sizes):
+    """Systems whose lanes own two or three 64-body blocks (up to 768 bodies)
+    and, with an 800-body system in the batch, the four-block kernel: states,
+    ids and merge records bit-identical to the double restatement."""
+    print(f"case batch fp64 sizes={sizes}", flush=True)
+    systems = [rubble(n, 300 + k) for k, n in enumerate(sizes)]
+    b, _ = make_batch(systems, capi.PREC_FP64)
+    b.evolve(12)
+    st = b.state()
+    log = b.merge_log()
+    k = 0
+    for s, (m, r, e, pos, vel) in enumerate(systems):
+        ref = O.kdk(m, pos, vel, G_SI, 5.0, 12, coll=2, r=r, e=e, dtype=np.float64)
+        c = int(st["counts"][s])
+        assert c == ref["n"]
+        assert np.array_equal(st["pos"][k:k + c], ref["pos"])
+        assert np.array_equal(st["vel"][k:k + c], ref["vel"])
+        assert np.array_equal(st["m"][k:k + c], ref["m"]) and np.array_equal(st["r"][k:k + c], ref["r"])
+        assert np.array_equal(st["ids"][k:k + c], ref["labels"])
+        assert np.array_equal(log[log[:, 0] == s][:, 1:], ref["log"])
+        k += c
+    assert len(log) > 0
+
+
 def test_batch_matches_single_system_api():
     systems = [rubble(n, 200 + k) for k, n in enumerate([12, 30])]
     b, _ = make_batch(systems, capi.PREC_FP64)
