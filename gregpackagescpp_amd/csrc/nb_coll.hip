@@ -10,16 +10,21 @@
 // equal keys dropped, axis momenta captured at scan time, one max_mom per call,
 // a full rescan after every merge), so it is replayed exactly, by one
 // workgroup per system:
-//  * every pair is first tested in parallel (the reference's own FP64 test
-//    r_i + r_j > |p_j - p_i|); outer bodies with no overlapping j > i cannot
-//    change anything and are skipped (positions do not move during the pass
-//    until a merge, after which every later outer body is scanned);
+//  * every pair is first tested in parallel, with no barrier between outer
+//    bodies (the reference's own FP64 test r_i + r_j > |p_j - p_i|); outer
+//    bodies with no overlapping j > i cannot change anything and are skipped
+//    (positions do not move during the pass until a merge, after which every
+//    later outer body is scanned); the workgroup finds the next flagged
+//    outer body together, 8 flags per thread;
 //  * a scanned outer body's j range is split over the threads; the merge
 //    candidate is the LAST j reaching the largest curr_mom (the sequential
-//    "curr >= max_mom" record rule), found by a block reduction;
+//    "curr >= max_mom" record rule), found by a block reduction; a merge is
+//    computed one body field per thread;
 //  * bounce entries are sorted by (distance, j) -- equal distances keep the
 //    smallest j, as the map's emplace does -- and applied in order by one
 //    thread, since each entry updates the outer body's velocity.
+// The batched kernel runs the pass only on steps where its force loop's
+// overlap prefilter flags a body (nb_coll_dev.hpp).
 // Arithmetic is FP64 with contraction off; the merged radius uses a fixed
 // Newton cube root (bit-reproducible, within 1 ulp of cbrt).
 #include <algorithm>
