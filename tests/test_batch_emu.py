@@ -137,10 +137,11 @@ def _concat(systems):
     return offsets, m, r, e, pos, vel
 
 
-@pytest.mark.parametrize("sizes", [[300, 520, 9], [800, 64]])
+@pytest.mark.parametrize("sizes", [[300, 520, 9], [800, 64], [1, 2, 65, 128, 129, 257, 3]])
 def test_emu_fp64_large_systems_bit_exact(emu, tmp_path, sizes):
     """Systems that give a wave two, three (up to 768 bodies) or four (above
-    768, the larger instance) 64-body blocks, with merges, bit for bit."""
+    768, the larger instance) 64-body blocks, and sizes at the edges (1 and
+    2 bodies, one past a block), with merges, bit for bit."""
     systems = [rubble(n, 300 + k) for k, n in enumerate(sizes)]
     offsets, m, r, e, pos, vel = _concat(systems)
     st = run_emu(emu, str(tmp_path), offsets, m, r, e, pos, vel, 12, mixed=False, merge=True, chunk=5)

@@ -111,11 +111,12 @@ def test_batch_fp64_bit_exact_vs_oracle():
     assert len(log) > 0
 
 
-@pytest.mark.parametrize("sizes", [[300, 520, 9], [800, 64]])
+@pytest.mark.parametrize("sizes", [[300, 520, 9], [800, 64], [1, 2, 65, 128, 129, 257, 3]])
 def test_batch_fp64_large_systems_bit_exact(sizes):
     """Systems whose lanes own two or three 64-body blocks (up to 768 bodies)
-    and, with an 800-body system in the batch, the four-block kernel: states,
-    ids and merge records bit-identical to the double restatement."""
+    and, with an 800-body system in the batch, the four-block kernel; sizes at
+    the edges (1 and 2 bodies, one past a block): states, ids and merge
+    records bit-identical to the double restatement."""
     print(f"case batch fp64 sizes={sizes}", flush=True)
     systems = [rubble(n, 300 + k) for k, n in enumerate(sizes)]
     b, _ = make_batch(systems, capi.PREC_FP64)
