@@ -444,8 +444,9 @@ constexpr float kPrefilterMargin = 0x1.0p-20f;
 // runs on steps where some pair passes it.
 //
 // Registers: the 3-block instances are held to 4 waves per SIMD (128 VGPRs;
-// the compiler spills one 8-byte value, outside the step loop); the 4-block
-// instance would spill inside the step loop at that bound and runs at 3.
+// the compiler spills a few values, reloaded once per owned block per step,
+// outside the tile loops); the 4-block instance would spill inside the force
+// code at that bound and runs at 3.
 template <bool MIXED, int MAXB>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MAXB <= 3 ? 4 : 1))) void k_batch_steps(
     BodyView all, const int *__restrict__ order, const int64_t *__restrict__ offsets, int *counts,
