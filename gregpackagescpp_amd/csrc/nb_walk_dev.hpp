@@ -290,29 +290,48 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
             il = 0;
             wave_lds_sync();
         };
-        // bucket ranges, staged 64 bodies at a time through the cell list's
-        // LDS (emptied first)
+        // bucket ranges, their bodies concatenated in list order and staged
+        // 64 at a time through the cell list's LDS (emptied first): each lane
+        // finds the range holding its body by a search over the ranges'
+        // starts, so a round of loads serves several short ranges.  The
+        // bodies are summed in the same order as range by range.
         auto flush_bl = [&]() {
             if (il) flush_il();
             wave_lds_sync();
-            for (int q = 0; q < bl; ++q) {
-                const int2 rg = s_bl[wv][q];
-                const int32_t j0 = __builtin_amdgcn_readfirstlane(rg.x), nb = __builtin_amdgcn_readfirstlane(rg.y);
-                for (int32_t base = 0; base < nb; base += 64) {
-                    const int32_t k = min(64, nb - base);
-                    if ((int32_t)lane < k) s_il[wv][lane] = pos[j0 + base + (int32_t)lane];
-                    wave_lds_sync();
-                    int32_t j = 0;
-                    for (; j + 2 <= k; j += 2) {
-                        const float4 b0 = s_il[wv][j], b1 = s_il[wv][j + 1];
-                        monopole(b0, p, eps2, SOFT, ax, ay, az);
-                        monopole(b1, p, eps2, SOFT, ax, ay, az);
-                    }
-                    if (j < k) monopole(s_il[wv][j], p, eps2, SOFT, ax, ay, az);
-                    wave_lds_sync();
-                }
-                cnt += (unsigned)nb;
+            const int2 rg = (int)lane < bl ? s_bl[wv][lane] : make_int2(0, 0);
+            int incl = rg.y;  // inclusive prefix sum of the range lengths
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(incl, o);
+                if ((int)lane >= o) incl += y;
             }
+            const int total_b = __shfl(incl, 63);
+            wave_lds_sync();  // every lane has read its range before the starts replace the lengths
+            if ((int)lane < bl) s_bl[wv][lane] = make_int2(rg.x, incl - rg.y);
+            wave_lds_sync();
+            for (int base = 0; base < total_b; base += 64) {
+                const int g = base + (int)lane;
+                if (g < total_b) {
+                    int lo = 0, hi = bl - 1;  // the last range whose start is at most g
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_bl[wv][mid].y <= g) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const int2 r = s_bl[wv][lo];
+                    s_il[wv][lane] = pos[r.x + (g - r.y)];
+                }
+                wave_lds_sync();
+                const int k = min(64, total_b - base);
+                int j = 0;
+                for (; j + 2 <= k; j += 2) {
+                    const float4 b0 = s_il[wv][j], b1 = s_il[wv][j + 1];
+                    monopole(b0, p, eps2, SOFT, ax, ay, az);
+                    monopole(b1, p, eps2, SOFT, ax, ay, az);
+                }
+                if (j < k) monopole(s_il[wv][j], p, eps2, SOFT, ax, ay, az);
+                wave_lds_sync();
+            }
+            cnt += (unsigned)total_b;
             bl = 0;
         };
         int32_t sp = nc0;  // entries in the LDS part
