@@ -82,6 +82,7 @@ struct BHWork {
     // masses, radii and restitution in tree order (the context's own arrays
     // stay in input order: Barnes-Hut steps do not change them)
     DBuf<double> mt, rt, et, mt_alt, rt_alt, et_alt;
+    bool re_tree = false;  // rt and et hold the current tree order (only the collision pass reads them)
     // nodes (DFS pre-order)
     DBuf<int32_t> first, last, skip, parent, body;
     DBuf<uint8_t> depth, nchild;
@@ -480,6 +481,9 @@ __device__ int64_t run_end(const RunIndex &R, int64_t s, int L) {
 }
 
 
+// k_build_nodes' handling of radii and restitutions
+constexpr int kReSkip = 0, kReFromPrevious = 1, kReFromInput = 2;
+
 // ---- nodes in DFS pre-order, and the bodies moved into tree order ----
 // Node numbering: the root is 0 and the nodes starting at sorted position s
 // are 1 + base[s] + j, internal levels (c[s-1], c[s]] first, then the leaf
@@ -503,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
     int64_t n, const RunIndex R, const uint32_t *__restrict__ base, const uint32_t *__restrict__ total_p,
     const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo, const double *__restrict__ box,
     uint32_t *__restrict__ idx, const double *__restrict__ x, const double *__restrict__ v,
-    const double *__restrict__ m, const double *__restrict__ r, const double *__restrict__ e,
+    const double *__restrict__ m, const double *__restrict__ r, const double *__restrict__ e, int re_mode,
     const int32_t *__restrict__ perm, double *__restrict__ xo, double *__restrict__ vo, double *__restrict__ mo,
     double *__restrict__ ro, double *__restrict__ eo, int32_t *__restrict__ permo, float4 *__restrict__ pos32,
     double G, int32_t *__restrict__ first, int32_t *__restrict__ last, int32_t *__restrict__ skip,
@@ -519,9 +523,15 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
     xo[s] = px; xo[n + s] = py; xo[2 * n + s] = pz;
     vo[s] = v[b]; vo[n + s] = v[n + b]; vo[2 * n + s] = v[2 * n + b];
     mo[s] = pm;
-    ro[s] = r[b];
-    eo[s] = e[b];
-    permo[s] = perm ? perm[b] : (int32_t)b;
+    const int32_t in = perm ? perm[b] : (int32_t)b;
+    permo[s] = in;
+    // radii and restitutions (read only by the collision pass): not moved,
+    // moved along from the previous tree order, or taken from input order
+    if (re_mode != kReSkip) {
+        const int32_t q = re_mode == kReFromInput ? in : (int32_t)b;
+        ro[s] = r[q];
+        eo[s] = e[q];
+    }
     pos32[s] = make_float4((float)px, (float)py, (float)pz, (float)(G * pm));
     idx[s] = (uint32_t)s;
     // nodes starting at s
@@ -1534,11 +1544,15 @@ static void bh_build(nb_system *s) {
     };
     grow_nodes(2 * n + 64);
     const RunIndex R{w.c.p, w.cm1.p, w.cm2.p, w.cm3.p, n, n1, n2, n3};
+    // radii and restitutions follow the bodies only while the collision pass runs
+    const bool need_re = s->coll_mode == NB_OVERLAP_COLL_CHECK;
+    const bool re_from_tree = w.permuted && w.re_tree;
+    const int re_mode = !need_re ? kReSkip : re_from_tree ? kReFromPrevious : kReFromInput;
     // nodes, and the bodies into tree order
     auto build_nodes = [&]() {
         hipLaunchKernelGGL(k_build_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, R, w.cnt.p, total_d, w.khi.p,
                            w.klo.p, w.box.p, w.idx.p, s->x.p, s->v.p, w.permuted ? w.mt.p : s->m.p,
-                           w.permuted ? w.rt.p : s->r.p, w.permuted ? w.et.p : s->e.p,
+                           re_from_tree ? w.rt.p : s->r.p, re_from_tree ? w.et.p : s->e.p, re_mode,
                            w.permuted ? w.perm.p : nullptr, w.xa.p, w.va.p, w.mt_alt.p, w.rt_alt.p, w.et_alt.p,
                            w.perm_alt.p, w.pos32s.p, s->G, w.first.p, w.last.p, w.skip.p, w.body.p, w.depth.p,
                            w.octant.p, w.side.p, w.mass.p, w.com.p, w.cap_nodes);
@@ -1566,7 +1580,12 @@ static void bh_build(nb_system *s) {
     w.nodes = total;
     const int64_t cs = w.cap_nodes;  // stride of the SoA com array
     swap_buf(s->x, w.xa); swap_buf(s->v, w.va);
-    swap_buf(w.mt, w.mt_alt); swap_buf(w.rt, w.rt_alt); swap_buf(w.et, w.et_alt);
+    swap_buf(w.mt, w.mt_alt);
+    if (need_re) {
+        swap_buf(w.rt, w.rt_alt);
+        swap_buf(w.et, w.et_alt);
+    }
+    w.re_tree = need_re;
     swap_buf(w.perm, w.perm_alt);
     w.permuted = true;
     w.acc_permuted = false;  // until a walk writes accelerations in tree order

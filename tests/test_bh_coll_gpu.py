@@ -56,6 +56,31 @@ def test_bh_coll_fp64_bit_identical(n, radius):
     assert np.array_equal(st["vel"], rv)
 
 
+def test_bh_coll_after_steps_without_collisions():
+    """BH steps without collisions (radii and restitutions are then not moved
+    into tree order), then with them on the same context: bit-identical to
+    the double oracle run in the same two parts."""
+    print("case bh coll after plain steps", flush=True)
+    m, r, e, pos, vel = cluster(24, 4096, 0.03)
+    s = capi.System(0)
+    s.set_G(1.0)
+    s.set_softening(0.01)
+    s.set_bh_opening_angle(0.5)
+    s.set_timestep(1e-3)
+    s.set_precision(capi.PREC_FP64)
+    s.set_bodies(m, pos, vel, r=r, rest_c=e)
+    s.set_iterations(3)
+    s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)
+    s.set_collisions(capi.OVERLAP_COLL_CHECK, mergers=False, min_tot_com_mom=1e10)
+    s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)
+    st = s.state()
+    s.close()
+    p1, v1, _ = O.bh_kdk(m, pos, vel, 1.0, 1e-3, 0.01, 0.5, 3)
+    rp, rv, _ = O.bh_kdk_coll(m, r, e, p1, v1, 1.0, 1e-3, 0.01, 0.5, 3)
+    assert np.array_equal(st["pos"], rp)
+    assert np.array_equal(st["vel"], rv)
+
+
 def test_bh_coll_mergers_flag_only_bounces():
     print("case bh coll mergers", flush=True)
     m, r, e, pos, vel = cluster(22, 4096, 0.03)
