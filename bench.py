@@ -119,12 +119,15 @@ PROFILED_BH_WALK_BYTES = 2.197e8
 PROFILED_BATCH_LAUNCH_BYTES = 2.535e8
 
 
-def build_bytes(n, nodes, passes):
-    """Algorithmic HBM bytes of one octree build: per body 32 B read (x, y,
+def build_bytes(n, nodes, passes, kicks=0.0):
+    """Algorithmic HBM bytes of one octree build of a KDK step: the step's
+    drift, which the build's first pass applies (x and v read, x written:
+    72 B per body), and the previous step's kick in a fraction `kicks` of
+    the builds (a read, v written: 48 B per body); per body 32 B read (x, y,
     z, m) and 12 B written (key, index); per radix pass 24 B per body (key
     and index read and written); 48 B per node written; 32 B per body for
     the reorder of the bodies into tree order."""
-    return n * (32 + 12) + passes * n * 24 + nodes * 48 + n * 32
+    return n * (72 + 48 * kicks) + n * (32 + 12) + passes * n * 24 + nodes * 48 + n * 32
 
 
 def bench_bh(capi, steps):
@@ -151,7 +154,7 @@ def bench_bh(capi, steps):
     s.advance(steps)
     t = (time.perf_counter() - t0) / steps
     s.profile(True)
-    s.advance(2)
+    s.advance(2)  # two builds: the second one also applies the first step's kick
     b_ms, nb = s.profile_read("bh_build")
     w_ms, nw = s.profile_read("bh_walk")
     s.profile(False)
@@ -161,7 +164,7 @@ def bench_bh(capi, steps):
     bucket = s.tuning("bh_bucket")
     s.close()
     build_ms, walk_ms = b_ms / max(nb, 1), w_ms / max(nw, 1)
-    bb = build_bytes(BH_BODIES, nodes, passes)
+    bb = build_bytes(BH_BODIES, nodes, passes, kicks=0.5)
     build_gbs = bb / (build_ms * 1e-3) / 1e9
     walk_tf = FLOP_PER_INTERACTION * i_alg / (walk_ms * 1e-3) / 1e12
     return {"value": i_alg / t, "unit": "interactions/s", "bodies": BH_BODIES, "theta": BH_THETA,

@@ -114,6 +114,12 @@ struct BHWork {
     // or 18 levels
     int sort_passes = 7;
     hipEvent_t ev_count = nullptr;  // the node count, error and flags have reached h_scal
+    // The frontier walk's schedule (k_frontier_reset, k_group_order) runs on
+    // a second stream while the tree is built: it reads only the previous
+    // walk's group costs.  ev_sched_in follows that walk on the context's
+    // stream, ev_sched_out the schedule, which the walk waits for.
+    hipStream_t sched = nullptr;
+    hipEvent_t ev_sched_in = nullptr, ev_sched_out = nullptr;
     // collision pass
     DBuf<double> nbox;      // node corners btm[3], top[3]
     DBuf<int32_t> leaf_of;  // sorted position -> leaf node
@@ -130,6 +136,12 @@ struct BHWork {
     DBuf<double> g_geo;
     uint32_t *h_bc_total = nullptr;
     ~BHWork() {
+        if (sched) {
+            (void)hipStreamSynchronize(sched);
+            (void)hipStreamDestroy(sched);
+        }
+        if (ev_sched_in) (void)hipEventDestroy(ev_sched_in);
+        if (ev_sched_out) (void)hipEventDestroy(ev_sched_out);
         if (ev_count) (void)hipEventDestroy(ev_count);
         if (h_scal) (void)hipHostFree(h_scal);
         if (h_bc_total) (void)hipHostFree(h_bc_total);
@@ -150,15 +162,30 @@ constexpr int kScTotal = 0, kScErr = 1, kScLong = 2, kScRoots = kScLong + kSortS
 constexpr int kScHost = kScRoots;  // total, error and long-run flags come back to the host
 
 // ---- 1. bounding cube ----
-__global__ __launch_bounds__(kBlock) void k_bbox_partial(int64_t n, const double *__restrict__ x,
+constexpr int kBoxParts = 16384;  // most partial boxes of k_bbox_partial
+// With DRIFT the step's drift x += dt v is applied first (and with KICK the
+// previous step's closing kick v += kdt a before it), in the arithmetic of
+// the separate kick and drift kernels, so the box is the drifted bodies'.
+template <bool KICK, bool DRIFT>
+__global__ __launch_bounds__(kBlock) void k_bbox_partial(int64_t n, double *__restrict__ x, double *__restrict__ v,
+                                                         const double *__restrict__ a, double kdt, double dt,
                                                          double *__restrict__ part) {
     __shared__ double sm[6][kBlock];
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         for (int c = 0; c < 3; ++c) {
-            double v = x[c * n + i];
-            mn[c] = fmin(mn[c], v);
-            mx[c] = fmax(mx[c], v);
+            double xc = x[c * n + i];
+            if (DRIFT) {
+                double vc = v[c * n + i];
+                if (KICK) {
+                    vc += kdt * a[c * n + i];
+                    v[c * n + i] = vc;
+                }
+                xc += dt * vc;
+                x[c * n + i] = xc;
+            }
+            mn[c] = fmin(mn[c], xc);
+            mx[c] = fmax(mx[c], xc);
         }
     for (int c = 0; c < 3; ++c) {
         sm[c][threadIdx.x] = mn[c];
@@ -245,26 +272,29 @@ __global__ __launch_bounds__(kBlock) void k_keys(int64_t n, const double *__rest
     idx[i] = (uint32_t)i;
 }
 
-__global__ void k_gather_lo(int64_t n, const uint64_t *__restrict__ klo, const uint32_t *__restrict__ idx,
-                            uint64_t *__restrict__ out) {
+// The sort orders the high key's bits from lo_bit up; this pass brings in
+// each sorted position's low key (from its body's slot) and puts runs of
+// bodies equal in the sorted bits into (high key, low key, index) order, by
+// one insertion sort per run, done by the run's first position (the runs are
+// short -- the pass count is chosen for that -- and nearly sorted, since the
+// input is the previous tree's order).  With the full high key sorted
+// (lo_bit 0) only bodies sharing 21 levels form runs, and they are ordered
+// by the low key.
+__global__ void k_fix_ties(int64_t n, int lo_bit, uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo_slot,
+                           uint64_t *__restrict__ klo, uint32_t *__restrict__ idx) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s < n) out[s] = klo[idx[s]];
-}
-
-// The sort orders the high key's bits from lo_bit up; runs of bodies equal
-// in those bits are put in (high key, low key, index) order here, by one
-// insertion sort per run (the runs are short -- the pass count is chosen for
-// that -- and nearly sorted, since the input is the previous tree's order).
-// With the full high key sorted (lo_bit 0) only bodies sharing 21 levels
-// form runs, and they are ordered by the low key.
-__global__ void k_fix_ties(int64_t n, int lo_bit, uint64_t *__restrict__ khi, uint64_t *__restrict__ klo,
-                           uint32_t *__restrict__ idx) {
-    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n - 1) return;
+    if (s >= n) return;
     const uint64_t top = khi[s] >> lo_bit;
-    if ((khi[s + 1] >> lo_bit) != top || (s > 0 && (khi[s - 1] >> lo_bit) == top)) return;
+    const bool after = s > 0 && (khi[s - 1] >> lo_bit) == top;
+    const bool before = s + 1 < n && (khi[s + 1] >> lo_bit) == top;
+    if (!before) {
+        if (!after) klo[s] = klo_slot[idx[s]];  // a run of one
+        return;
+    }
+    if (after) return;  // inside a run: its first position orders it
     int64_t e = s + 1;
     while (e + 1 < n && (khi[e + 1] >> lo_bit) == top) ++e;
+    for (int64_t a = s; a <= e; ++a) klo[a] = klo_slot[idx[a]];
     for (int64_t a = s + 1; a <= e; ++a) {
         const uint64_t kh = khi[a], kl = klo[a];
         const uint32_t id = idx[a];
@@ -280,7 +310,6 @@ __global__ void k_fix_ties(int64_t n, int lo_bit, uint64_t *__restrict__ khi, ui
         idx[b + 1] = id;
     }
 }
-
 
 __device__ __forceinline__ int shared_digits(uint64_t h0, uint64_t l0, uint64_t h1, uint64_t l1) {
     if (h0 != h1) return (__clzll(h0 ^ h1) - 1) / 3;  // keys use the low 63 bits
@@ -1480,10 +1509,18 @@ void swap_buf(DBuf<T> &a, DBuf<T> &b) {
 
 }  // namespace
 
-// Builds the octree of the current positions into s->bh and leaves the bodies
-// in the tree's order (waits once for the node count, to size the later
-// launches and to surface build errors, while k_build_nodes runs).
-static void bh_build(nb_system *s) {
+// The step's drift (and the previous step's closing kick) applied by the
+// build's first pass.
+struct BHDrift {
+    bool drift = false;  // x += dt v
+    bool kick = false;   // v += dt a first (the previous step's closing kick)
+};
+
+// Builds the octree of the current positions (after the drift, if any) into
+// s->bh and leaves the bodies in the tree's order (waits once for the node
+// count, to size the later launches and to surface build errors, while
+// k_build_nodes runs).
+static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
     const int64_t n = s->n;
     if (!s->bh) s->bh = new BHWork();
     BHWork &w = *s->bh;
@@ -1502,7 +1539,7 @@ static void bh_build(nb_system *s) {
     // the other copy of every per-body array, for the move into tree order
     w.xa.ensure(3 * (size_t)n); w.va.ensure(3 * (size_t)n); w.aa.ensure(3 * (size_t)n);
     for (DBuf<double> *b : {&w.mt, &w.rt, &w.et, &w.mt_alt, &w.rt_alt, &w.et_alt}) b->ensure((size_t)n);
-    w.part.ensure(6 * 1024); w.box.ensure(6); w.scal.ensure(kScalars); w.icount.ensure(1);
+    w.part.ensure(6 * (size_t)kBoxParts); w.box.ensure(6); w.scal.ensure(kScalars); w.icount.ensure(1);
     if (!w.h_scal) NB_HIP(hipHostMalloc((void **)&w.h_scal, kScHost * sizeof(uint32_t), hipHostMallocDefault));
     if (!w.ev_count) NB_HIP(hipEventCreateWithFlags(&w.ev_count, hipEventDisableTiming));
     uint32_t *const total_d = w.scal.p + kScTotal, *const nroots_d = w.scal.p + kScRoots;
@@ -1512,8 +1549,11 @@ static void bh_build(nb_system *s) {
     // c and its chunk minima, padded to whole chunks of 64 for the searches' vector loads
     const int64_t n1 = (n + 63) >> 6, n2 = (n1 + 63) >> 6, n3 = (n2 + 63) >> 6;
     w.c.ensure((size_t)n + 64); w.cm1.ensure((size_t)n1 + 64); w.cm2.ensure((size_t)n2 + 64); w.cm3.ensure((size_t)n3 + 64);
-    const int nparts = (int)std::min<int64_t>(1024, (n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(kBlock), 0, st, n, s->x.p, w.part.p);
+    // one body per thread when the pass also moves the bodies (up to 4M of them), else a grid-stride 1024 blocks
+    const int nparts = (int)std::min<int64_t>(dr.drift ? kBoxParts : 1024, (n + kBlock - 1) / kBlock);
+    auto bbox = !dr.drift ? k_bbox_partial<false, false>
+                          : dr.kick ? k_bbox_partial<true, true> : k_bbox_partial<false, true>;
+    hipLaunchKernelGGL(bbox, dim3(nparts), dim3(kBlock), 0, st, n, s->x.p, s->v.p, s->a.p, s->dt, s->dt, w.part.p);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, st, nparts, w.part.p, w.box.p, w.scal.p);
     hipLaunchKernelGGL(k_keys, dim3(blocks(n)), dim3(kBlock), 0, st, n, s->x.p, w.box.p, w.khi.p, w.klo_slot.p,
                        w.idx.p, err_d);
@@ -1522,8 +1562,8 @@ static void bh_build(nb_system *s) {
     const int passes = s->tune_bh_sort > 0 ? s->tune_bh_sort : w.sort_passes;
     const int lo_bit = 3 * kHiLevels - RadixSorter::kDigitBits * passes;
     w.sorter.sort_swap(st, w.khi, w.idx, n, 3 * kHiLevels, lo_bit);
-    hipLaunchKernelGGL(k_gather_lo, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.klo_slot.p, w.idx.p, w.klo.p);
-    hipLaunchKernelGGL(k_fix_ties, dim3(blocks(n)), dim3(kBlock), 0, st, n, lo_bit, w.khi.p, w.klo.p, w.idx.p);
+    hipLaunchKernelGGL(k_fix_ties, dim3(blocks(n)), dim3(kBlock), 0, st, n, lo_bit, w.khi.p, w.klo_slot.p, w.klo.p,
+                       w.idx.p);
     hipLaunchKernelGGL(k_lcp_counts, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p, s->x.p,
                        w.c.p, w.cm1.p, w.cnt.p, err_d, w.scal.p + kScLong);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
@@ -1640,11 +1680,45 @@ void bh_forget_order(nb_system *s) {
     s->bh->sort_passes = 3 * kHiLevels / RadixSorter::kDigitBits;  // new bodies: the full sort until measured
 }
 
-void bh_forces(nb_system *s, int64_t *interactions) {
+static void bh_forces_step(nb_system *s, int64_t *interactions, BHDrift dr) {
     const int64_t n = s->n;
-    bh_build(s);
+    if (!s->bh) s->bh = new BHWork();
     BHWork &w = *s->bh;
     hipStream_t st = s->stream;
+    // The frontier walk's launch: one wave per group, a multiple of 8
+    // workgroups so that each XCD gets one range of `span` groups (XCD
+    // ranges of equal cost hold at most 1.5x the mean count).  Its schedule
+    // is computed on the second stream while the tree is built.
+    const bool frontier_walk = s->precision == NB_PREC_MIXED && s->tune_bh_walk != 2 && n >= 2;
+    const int64_t ngroups = (n + 63) / 64;
+    const int64_t mean = (ngroups + 7) / 8;
+    const int64_t wgs = std::max<int64_t>(1, (mean + mean / 2 + kFWaves - 1) / kFWaves);
+    const int64_t G = 8 * wgs;
+    const int64_t span = wgs * kFWaves;
+    const int32_t cap = s->tune_bh_stack > 0 ? s->tune_bh_stack : kFStackDefault;
+    if (frontier_walk) {
+        w.fstack.ensure((size_t)G * kFWaves * (size_t)cap);
+        w.werr.ensure(1); w.icount2.ensure(1); w.gxr.ensure(9);
+        if (!w.sched) {
+            NB_HIP(hipStreamCreateWithFlags(&w.sched, hipStreamNonBlocking));
+            NB_HIP(hipEventCreateWithFlags(&w.ev_sched_in, hipEventDisableTiming));
+            NB_HIP(hipEventCreateWithFlags(&w.ev_sched_out, hipEventDisableTiming));
+        }
+        NB_HIP(hipEventRecord(w.ev_sched_in, st));
+        NB_HIP(hipStreamWaitEvent(w.sched, w.ev_sched_in, 0));
+        if (w.gcost_n != ngroups) {
+            w.gcost.ensure((size_t)ngroups);
+            w.gorder.ensure((size_t)ngroups);
+            NB_HIP(hipMemsetAsync(w.gcost.p, 0, (size_t)ngroups * sizeof(uint32_t), w.sched));
+            w.gcost_n = ngroups;
+        }
+        hipLaunchKernelGGL(k_frontier_reset, dim3(1), dim3(64), 0, w.sched, w.werr.p, w.icount2.p);
+        hipLaunchKernelGGL(k_group_order, dim3(8), dim3(1024), 0, w.sched, ngroups, span, w.gcost.p, w.gorder.p,
+                           w.gxr.p);
+        NB_HIP(hipGetLastError());
+        NB_HIP(hipEventRecord(w.ev_sched_out, w.sched));
+    }
+    bh_build(s, dr);
     const int32_t total = (int32_t)w.nodes;
     const int64_t cs = w.cap_nodes;
     NB_HIP(hipMemsetAsync(w.icount.p, 0, sizeof(unsigned long long), st));
@@ -1655,31 +1729,11 @@ void bh_forces(nb_system *s, int64_t *interactions) {
             const int bucket = s->tune_bh_bucket > 0 ? s->tune_bh_bucket : w.bucket;
             const float eps2 = s->eps > 0 ? (float)(s->eps * s->eps) : 0.0f;
             auto wave = s->eps > 0 ? k_walk_wave<true> : k_walk_wave<false>;
-            if (s->tune_bh_walk == 2) {
+            if (!frontier_walk) {
                 hipLaunchKernelGGL(wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wnode.p,
                                    bucket, eps2, s->a.p, w.icount.p, nullptr);
             } else {
-                // one wave per group; a multiple of 8 workgroups so that
-                // each XCD gets one range of `span` groups
-                // XCD ranges of equal cost hold at most 1.5x the mean count
-                const int64_t ngroups = (n + 63) / 64;
-                const int64_t mean = (ngroups + 7) / 8;
-                const int64_t wgs = std::max<int64_t>(1, (mean + mean / 2 + kFWaves - 1) / kFWaves);
-                const int64_t G = 8 * wgs;
-                const int64_t span = wgs * kFWaves;
-                const int32_t cap = s->tune_bh_stack > 0 ? s->tune_bh_stack : kFStackDefault;
-                w.fstack.ensure((size_t)G * kFWaves * (size_t)cap);
-                w.werr.ensure(1); w.icount2.ensure(1);
-                if (w.gcost_n != ngroups) {
-                    w.gcost.ensure((size_t)ngroups);
-                    w.gorder.ensure((size_t)ngroups);
-                    NB_HIP(hipMemsetAsync(w.gcost.p, 0, (size_t)ngroups * sizeof(uint32_t), st));
-                    w.gcost_n = ngroups;
-                }
-                hipLaunchKernelGGL(k_frontier_reset, dim3(1), dim3(64), 0, st, w.werr.p, w.icount2.p);
-                w.gxr.ensure(9);
-                hipLaunchKernelGGL(k_group_order, dim3(8), dim3(1024), 0, st, ngroups, span, w.gcost.p, w.gorder.p,
-                                   w.gxr.p);
+                NB_HIP(hipStreamWaitEvent(st, w.ev_sched_out, 0));
                 auto front = s->eps > 0 ? k_walk_frontier<true> : k_walk_frontier<false>;
                 hipLaunchKernelGGL(front, dim3((unsigned)G), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wgrp.p,
                                    bucket, eps2, s->a.p, w.icount.p, w.fstack.p, cap, w.gorder.p, w.gxr.p,
@@ -1771,18 +1825,29 @@ static void bh_collision_pass(nb_system *s) {
     NB_HIP(hipGetLastError());
 }
 
+void bh_forces(nb_system *s, int64_t *interactions) { bh_forces_step(s, interactions, BHDrift()); }
+
 // evolve's KDK + Barnes-Hut loop body (gregsys.hpp:2914-2939): drift a full
 // step, rebuild, walk, kick a full step (velocities stay half a step ahead),
-// then bh_s_coll when collisions are on.  The bodies stay in tree order from
-// step to step; a record is taken in input order.
+// then bh_s_coll when collisions are on.  The drift is applied by the
+// build's first pass, and so is the previous step's kick when nothing
+// between the two reads the velocities (no collision pass, no record, not
+// the call's last step).  The bodies stay in tree order from step to step; a
+// record is taken in input order.
 void bh_steps(nb_system *s, uint64_t steps) {
+    bool kick_pending = false;
     for (uint64_t k = 0; k < steps; ++k) {
-        launch_drift(s->stream, s->n, s->v.p, s->x.p, s->dt);
-        bh_forces(s, nullptr);
-        launch_kick(s->stream, s->n, s->a.p, s->v.p, s->dt);
-        if (s->coll_mode == NB_OVERLAP_COLL_CHECK) bh_collision_pass(s);
+        BHDrift dr;
+        dr.drift = true;
+        dr.kick = kick_pending;
+        bh_forces_step(s, nullptr, dr);
+        const bool coll = s->coll_mode == NB_OVERLAP_COLL_CHECK;
+        const bool rec = record_due(s, s->step_count + 1);
+        kick_pending = !coll && !rec && k + 1 < steps;
+        if (!kick_pending) launch_kick(s->stream, s->n, s->a.p, s->v.p, s->dt);
+        if (coll) bh_collision_pass(s);
         ++s->step_count;
-        if (record_due(s, s->step_count)) {
+        if (rec) {
             bh_restore_order(s);
             record_state(s, s->step_count);
         }
