@@ -59,6 +59,13 @@ enum { BH_ERR_NONE = 0, BH_ERR_PLACEMENT = 1, BH_ERR_INDIST = 2, BH_ERR_DEPTH = 
 #define NB_WAVES_PER_EU(w) __attribute__((amdgpu_waves_per_eu(w)))
 #include "nb_walk_dev.hpp"
 
+// A node's FP64 mass and centre of mass in one 32-byte record, written for
+// the subtree roots and the nodes above them, so that the top of the tree's
+// pass reads each child from one cache line (the SoA arrays take four).
+struct __align__(32) MassCom {
+    double m, x, y, z;
+};
+
 struct BHWork {
     int64_t cap_n = 0, cap_nodes = 0;
     DBuf<double> part;      // bbox partials
@@ -96,6 +103,7 @@ struct BHWork {
     DBuf<int32_t> kid;      // node at each position of the grouped layout (children of x: 1 + cbase[x] ...)
     DBuf<uint32_t> cblk, ctot;
     DBuf<int32_t> roots, top;  // subtree roots and the nodes above them (bottom-up pass)
+    DBuf<MassCom> mc;          // mass and COM of the subtree roots and the nodes above them
     DBuf<int32_t> fstack;   // frontier walk: one node stack per wave
     DBuf<int64_t> gxr;      // frontier walk: the XCD ranges of the group order
     DBuf<int> werr;         // frontier walk: a stack would have overflowed
@@ -272,29 +280,26 @@ __global__ __launch_bounds__(kBlock) void k_keys(int64_t n, const double *__rest
     idx[i] = (uint32_t)i;
 }
 
-// The sort orders the high key's bits from lo_bit up; this pass brings in
-// each sorted position's low key (from its body's slot) and puts runs of
-// bodies equal in the sorted bits into (high key, low key, index) order, by
-// one insertion sort per run, done by the run's first position (the runs are
-// short -- the pass count is chosen for that -- and nearly sorted, since the
-// input is the previous tree's order).  With the full high key sorted
-// (lo_bit 0) only bodies sharing 21 levels form runs, and they are ordered
-// by the low key.
-__global__ void k_fix_ties(int64_t n, int lo_bit, uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo_slot,
-                           uint64_t *__restrict__ klo, uint32_t *__restrict__ idx) {
+__global__ void k_gather_lo(int64_t n, const uint64_t *__restrict__ klo, const uint32_t *__restrict__ idx,
+                            uint64_t *__restrict__ out) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
+    if (s < n) out[s] = klo[idx[s]];
+}
+
+// The sort orders the high key's bits from lo_bit up; runs of bodies equal
+// in those bits are put in (high key, low key, index) order here, by one
+// insertion sort per run (the runs are short -- the pass count is chosen for
+// that -- and nearly sorted, since the input is the previous tree's order).
+// With the full high key sorted (lo_bit 0) only bodies sharing 21 levels
+// form runs, and they are ordered by the low key.
+__global__ void k_fix_ties(int64_t n, int lo_bit, uint64_t *__restrict__ khi, uint64_t *__restrict__ klo,
+                           uint32_t *__restrict__ idx) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n - 1) return;
     const uint64_t top = khi[s] >> lo_bit;
-    const bool after = s > 0 && (khi[s - 1] >> lo_bit) == top;
-    const bool before = s + 1 < n && (khi[s + 1] >> lo_bit) == top;
-    if (!before) {
-        if (!after) klo[s] = klo_slot[idx[s]];  // a run of one
-        return;
-    }
-    if (after) return;  // inside a run: its first position orders it
+    if ((khi[s + 1] >> lo_bit) != top || (s > 0 && (khi[s - 1] >> lo_bit) == top)) return;
     int64_t e = s + 1;
     while (e + 1 < n && (khi[e + 1] >> lo_bit) == top) ++e;
-    for (int64_t a = s; a <= e; ++a) klo[a] = klo_slot[idx[a]];
     for (int64_t a = s + 1; a <= e; ++a) {
         const uint64_t kh = khi[a], kl = klo[a];
         const uint32_t id = idx[a];
@@ -310,6 +315,7 @@ __global__ void k_fix_ties(int64_t n, int lo_bit, uint64_t *__restrict__ khi, co
         idx[b + 1] = id;
     }
 }
+
 
 __device__ __forceinline__ int shared_digits(uint64_t h0, uint64_t l0, uint64_t h1, uint64_t l1) {
     if (h0 != h1) return (__clzll(h0 ^ h1) - 1) / 3;  // keys use the low 63 bits
@@ -1043,7 +1049,7 @@ __global__ void k_link(int32_t total, const int32_t *__restrict__ parent, const 
                        WalkRecs W, int64_t cstride, const double *__restrict__ mass, const double *__restrict__ com,
                        const double *__restrict__ side, const int32_t *__restrict__ body,
                        const int32_t *__restrict__ first, const int32_t *__restrict__ last,
-                       const float4 *__restrict__ pos32) {
+                       const float4 *__restrict__ pos32, MassCom *__restrict__ mc) {
     const int32_t x = blockIdx.x * kBlock + threadIdx.x;
     if (x >= total) return;
     const int32_t p = parent[x];
@@ -1052,6 +1058,16 @@ __global__ void k_link(int32_t total, const int32_t *__restrict__ parent, const 
     if (at < (uint32_t)total && (x == 0 || at > 0)) kid[at] = x;
     const int32_t b = body[x];
     if (b >= 0) {
+        if (p >= 0 && p < total && skip[p] - p > kSubtree) {
+            // a leaf under a node above the subtrees: no subtree pass sums
+            // it, so its mass and COM record is written here
+            MassCom v;
+            v.m = mass[x];
+            v.x = com[x];
+            v.y = com[cstride + x];
+            v.z = com[2 * cstride + x];
+            mc[x] = v;
+        }
         // A leaf's records are put_walk_recs' for body b: its COM and G m in
         // FP32 are the body's FP32 position stream entry, skip x + 1, one
         // body, no children, and an opening radius that accepts it always.
@@ -1097,7 +1113,7 @@ __global__ __launch_bounds__(kBlock) void k_com_subtrees(
     const int32_t *__restrict__ skip, const int32_t *__restrict__ body, const uint8_t *__restrict__ depth,
     const uint8_t *__restrict__ nchild, const uint32_t *__restrict__ cbase, const int32_t *__restrict__ kid,
     double *__restrict__ mass, double *__restrict__ com, int64_t cstride, WalkRecs W, const double *__restrict__ side,
-    const int32_t *__restrict__ first, const int32_t *__restrict__ last) {
+    const int32_t *__restrict__ first, const int32_t *__restrict__ last, MassCom *__restrict__ mc) {
     constexpr int kPer = kSubtree / kBlock;  // nodes per thread
     __shared__ double s_m[kSubtree], s_x[kSubtree], s_y[kSubtree], s_z[kSubtree];
     __shared__ int16_t s_kid[kSubtree];   // children of the subtree's nodes, local indices, grouped layout order
@@ -1192,6 +1208,14 @@ __global__ __launch_bounds__(kBlock) void k_com_subtrees(
             }
             __syncthreads();  // this depth's sums before the next one up reads them
         }
+        if (t == 0) {
+            MassCom v;
+            v.m = s_m[0];
+            v.x = s_x[0];
+            v.y = s_y[0];
+            v.z = s_z[0];
+            mc[r] = v;  // the root's, for the pass over the nodes above the subtrees
+        }
         // this thread's internal nodes out, with their walk records (the
         // record inputs of all of them loaded at once; leaves read the root's)
         double sd[kPer];
@@ -1227,9 +1251,13 @@ __global__ __launch_bounds__(kBlock) void k_com_subtrees(
 
 // The nodes above the subtrees, one workgroup.  Up to kTopLds of them are
 // sorted by depth into LDS together with their child lists, so each depth
-// costs one round of independent loads (the children's mass and COM) and a
-// barrier; the walk records follow in one sweep.  More such nodes: every
-// depth rescans the list with the child lists in memory.
+// costs one round of independent loads -- each child's mass and COM from its
+// 32-byte MassCom record, and the node's own walk-record inputs -- and a
+// barrier; a node's walk records are written as soon as its sums are.  (One
+// workgroup's loads are limited by the cache lines it can request, so one
+// line per child instead of one per SoA array is what sets the time.)  More
+// such nodes: every depth rescans the list with the child lists in memory,
+// and the walk records follow in one sweep.
 constexpr uint32_t kTopLds = 2048;
 __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *__restrict__ top,
                                                   const uint32_t *__restrict__ ntop, const int32_t *__restrict__ skip,
@@ -1238,9 +1266,11 @@ __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *
                                                   const int32_t *__restrict__ kid, double *__restrict__ mass,
                                                   double *__restrict__ com, int64_t cstride, WalkRecs W,
                                                   const double *__restrict__ side, const int32_t *__restrict__ first,
-                                                  const int32_t *__restrict__ last) {
+                                                  const int32_t *__restrict__ last, MassCom *__restrict__ mc) {
     __shared__ int32_t s_list[kTopLds];
     __shared__ int32_t s_kids[kTopLds][8];
+    __shared__ uint32_t s_cb[kTopLds];
+    __shared__ uint8_t s_nc[kTopLds];
     __shared__ uint32_t s_cur[kDepths], s_off[kDepths + 1];
     const uint32_t cnt = *ntop;
     const int t = threadIdx.x;
@@ -1248,25 +1278,19 @@ __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *
     __syncthreads();
     for (uint32_t i = t; i < cnt; i += blockDim.x) atomicAdd(&s_cur[min((int)depth[top[i]], kDepths - 1)], 1u);
     __syncthreads();
-    if (t == 0) {
-        uint32_t run = 0;
-        for (int L = 0; L < kDepths; ++L) {
-            s_off[L] = run;
-            run += s_cur[L];
-            s_cur[L] = s_off[L];
-        }
-        s_off[kDepths] = run;
-    }
+    depth_offsets(s_cur, s_off);
     __syncthreads();
     if (cnt <= kTopLds) {
         for (uint32_t i = t; i < cnt; i += blockDim.x) {
             const int32_t p = top[i];
             const uint32_t at = atomicAdd(&s_cur[min((int)depth[p], kDepths - 1)], 1u);
             const int nc = min((int)nchild[p], 8);
-            const uint32_t c0 = 1u + cbase[p];
+            const uint32_t cb = cbase[p];
             s_list[at] = p;
+            s_cb[at] = cb;
+            s_nc[at] = (uint8_t)nc;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) s_kids[at][k] = k < nc ? kid[c0 + k] : -1;
+            for (int k = 0; k < 8; ++k) s_kids[at][k] = k < nc ? kid[1u + cb + k] : -1;
         }
         __syncthreads();
         for (int L = kDepths - 1; L >= 0; --L) {
@@ -1274,10 +1298,10 @@ __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *
             if (b == e) continue;  // the same for every thread (LDS values after a barrier)
             for (uint32_t j = b + t; j < e; j += blockDim.x) {
                 const int32_t p = s_list[j];
-                // every child's loads issued together (absent children read node p's, unused)
+                // every child's record loaded together (absent children read node p's, unused)
                 int32_t cc[8];
                 bool ok[8];
-                double mv[8], xv[8], yv[8], zv[8];
+                MassCom v[8];
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     const int32_t c = s_kids[j][k];
@@ -1285,25 +1309,37 @@ __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *
                     cc[k] = ok[k] ? c : p;
                 }
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    mv[k] = mass[cc[k]];
-                    xv[k] = com[cc[k]];
-                    yv[k] = com[cstride + cc[k]];
-                    zv[k] = com[2 * cstride + cc[k]];
+                for (int k = 0; k < 8; ++k) v[k] = mc[cc[k]];
+                double sd = 0.0;
+                int32_t sk = 0, fi = 0, la = 0;
+                uint32_t slot = 0;
+                if (W.wnode) {
+                    sd = side[p];
+                    sk = skip[p];
+                    fi = first[p];
+                    la = last[p];
+                    slot = W.slot[p];
                 }
                 double ms = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll
                 for (int k = 7; k >= 0; --k) {  // reverse octant order (calc_com :262-266)
                     if (!ok[k]) continue;
-                    ms += mv[k];
-                    sx += mv[k] * xv[k];
-                    sy += mv[k] * yv[k];
-                    sz += mv[k] * zv[k];
+                    ms += v[k].m;
+                    sx += v[k].m * v[k].x;
+                    sy += v[k].m * v[k].y;
+                    sz += v[k].m * v[k].z;
                 }
-                mass[p] = ms;
-                com[p] = sx / ms;
-                com[cstride + p] = sy / ms;
-                com[2 * cstride + p] = sz / ms;
+                MassCom o;
+                o.m = ms;
+                o.x = sx / ms;
+                o.y = sy / ms;
+                o.z = sz / ms;
+                mass[p] = o.m;
+                com[p] = o.x;
+                com[cstride + p] = o.y;
+                com[2 * cstride + p] = o.z;
+                mc[p] = o;
+                if (W.wnode) put_walk_recs(W, p, o.m, o.x, o.y, o.z, false, sd, sk, fi, la, s_nc[j], s_cb[j], slot);
             }
             __syncthreads();
         }
@@ -1317,10 +1353,10 @@ __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *
             }
             __syncthreads();
         }
+        if (W.wnode)
+            for (uint32_t i = t; i < cnt; i += blockDim.x)
+                write_walk_recs(W, top[i], cstride, mass, com, side, body, skip, first, last);
     }
-    if (W.wnode)
-        for (uint32_t i = t; i < cnt; i += blockDim.x)
-            write_walk_recs(W, top[i], cstride, mass, com, side, body, skip, first, last);
 }
 
 // Positions, velocities (and accelerations when a walk wrote them) back in
@@ -1562,8 +1598,8 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
     const int passes = s->tune_bh_sort > 0 ? s->tune_bh_sort : w.sort_passes;
     const int lo_bit = 3 * kHiLevels - RadixSorter::kDigitBits * passes;
     w.sorter.sort_swap(st, w.khi, w.idx, n, 3 * kHiLevels, lo_bit);
-    hipLaunchKernelGGL(k_fix_ties, dim3(blocks(n)), dim3(kBlock), 0, st, n, lo_bit, w.khi.p, w.klo_slot.p, w.klo.p,
-                       w.idx.p);
+    hipLaunchKernelGGL(k_gather_lo, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.klo_slot.p, w.idx.p, w.klo.p);
+    hipLaunchKernelGGL(k_fix_ties, dim3(blocks(n)), dim3(kBlock), 0, st, n, lo_bit, w.khi.p, w.klo.p, w.idx.p);
     hipLaunchKernelGGL(k_lcp_counts, dim3(blocks(n)), dim3(kBlock), 0, st, n, w.khi.p, w.klo.p, w.idx.p, s->x.p,
                        w.c.p, w.cm1.p, w.cnt.p, err_d, w.scal.p + kScLong);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
@@ -1580,6 +1616,7 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
         w.depth.ensure(t); w.nchild.ensure(t); w.octant.ensure(t); w.mass.ensure(t); w.com.ensure(3 * t);
         w.side.ensure(t); w.wnode.ensure(t); w.slot.ensure(t); w.kid.ensure(t); w.roots.ensure(t); w.top.ensure(t);
         w.wgrp.ensure(t); w.rank.ensure(t); w.cbase.ensure(t); w.cblk.ensure(t / 1024 + 2); w.ctot.ensure(1);
+        w.mc.ensure(t);
         w.cap_nodes = want;
     };
     grow_nodes(2 * n + 64);
@@ -1641,15 +1678,15 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
                      w.cbase.p, w.nchild.p, s->G, s->theta, total};
     hipLaunchKernelGGL(k_link, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.parent.p, w.skip.p, w.rank.p,
                        w.cbase.p, w.slot.p, w.kid.p, w.roots.p, nroots_d, w.top.p, nroots_d + 1, W, cs, w.mass.p,
-                       w.com.p, w.side.p, w.body.p, w.first.p, w.last.p, w.pos32s.p);
+                       w.com.p, w.side.p, w.body.p, w.first.p, w.last.p, w.pos32s.p, w.mc.p);
     // mass, COM and the internal nodes' walk records, bottom up
     const unsigned sub_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, total / 128 + 1));
     hipLaunchKernelGGL(k_com_subtrees, dim3(sub_grid), dim3(kBlock), 0, st, total, w.roots.p, nroots_d, w.skip.p,
                        w.body.p, w.depth.p, w.nchild.p, w.cbase.p, w.kid.p, w.mass.p, w.com.p, cs, W, w.side.p,
-                       w.first.p, w.last.p);
+                       w.first.p, w.last.p, w.mc.p);
     hipLaunchKernelGGL(k_com_top, dim3(1), dim3(1024), 0, st, total, w.top.p, nroots_d + 1, w.skip.p, w.body.p,
                        w.depth.p, w.nchild.p, w.cbase.p, w.kid.p, w.mass.p, w.com.p, cs, W, w.side.p, w.first.p,
-                       w.last.p);
+                       w.last.p, w.mc.p);
     NB_HIP(hipGetLastError());
 }
 
