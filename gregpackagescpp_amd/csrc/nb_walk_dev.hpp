@@ -134,11 +134,11 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
 // the group satisfies d^2 > r^2_open, a rejected cell with at most `bucket`
 // bodies is summed body by body, and other rejected cells are opened.  The
 // traversal differs: each step pops up to 64 cells off the wave's stack and
-// tests one per lane, first against the group's bounding box (distance from
-// the cell's centre of mass to the box: if that passes, every body passes,
-// in FP32 as well, since rounding is monotonic), then against the boxes of
-// the group's kSubBoxes parts (consecutive bodies), and body by body only
-// inside a part whose box fails.  Accepted cells and bucket ranges go to
+// tests one per lane against the group's bounding box (distance from the
+// cell's centre of mass to the box: if that passes, every body passes, in
+// FP32 as well, since rounding is monotonic); the cells that fail it are
+// then tested one at a time against all the group's bodies at once, one
+// lane per body and a ballot per cell.  Accepted cells and bucket ranges go to
 // per-wave LDS lists that every lane then sums with VGPR operands; opened
 // cells push their children, which the grouped record layout keeps adjacent.
 // The stack's top kLStack entries live in LDS; a push that would overflow
@@ -158,12 +158,6 @@ static_assert(kIL >= 64 && kBL >= 64, "one step adds up to 64 entries to either 
 #endif
 constexpr int kLStack = NB_FRONTIER_LDS_STACK;  // stack entries per wave in LDS
 static_assert(kLStack >= 8, "the root's children start in LDS");
-#ifndef NB_FRONTIER_PARTS
-#define NB_FRONTIER_PARTS 8
-#endif
-constexpr int kSubBoxes = NB_FRONTIER_PARTS;  // parts of a group with their own box
-constexpr int kSubN = 64 / kSubBoxes;         // bodies per part
-static_assert(kSubBoxes * kSubN == 64 && kSubN >= 2, "parts of a wave64 group");
 constexpr int kFWaves = kBlock / 64;
 constexpr int kFStackDefault = 4096;
 
@@ -173,15 +167,6 @@ __device__ __forceinline__ float wave_min(float v) {
 }
 __device__ __forceinline__ float wave_max(float v) {
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return v;
-}
-// minimum / maximum over the lane's part of kSubN consecutive lanes
-__device__ __forceinline__ float part_min(float v) {
-    for (int o = kSubN / 2; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ __forceinline__ float part_max(float v) {
-    for (int o = kSubN / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
     return v;
 }
 __device__ __forceinline__ int lanes_below(uint64_t m, unsigned lane) {
@@ -218,9 +203,8 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
 #pragma clang fp contract(fast)
     __shared__ float4 s_il[kFWaves][kIL];
     __shared__ int2 s_bl[kFWaves][kBL];
-    __shared__ float4 s_grp[kFWaves][64];
+    __shared__ float4 s_fc[kFWaves][64];  // the step's cells that fail the group box: COM, r^2_open
     __shared__ int32_t s_stk[kFWaves][kLStack];
-    __shared__ float s_qbox[kFWaves][kSubBoxes + 1][6];  // the parts' boxes and the group's (lo xyz, hi xyz)
     const unsigned lane = __lane_id(), wv = threadIdx.x >> 6;
     // One wave per group, no loop over groups and no lane-0-only code in any
     // loop: a group handed out by one lane's atomic and broadcast by
@@ -252,26 +236,11 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
         const float4 p = live ? pos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
         const uint64_t active = __ballot(live);
         const int nlive = __popcll(active);
-        s_grp[wv][lane] = p;
-        // the parts' boxes (an empty part's box is empty: every cell passes it)
-        const float qlx = part_min(live ? p.x : INFINITY), qhx = part_max(live ? p.x : -INFINITY);
-        const float qly = part_min(live ? p.y : INFINITY), qhy = part_max(live ? p.y : -INFINITY);
-        const float qlz = part_min(live ? p.z : INFINITY), qhz = part_max(live ? p.z : -INFINITY);
-        if ((lane & (unsigned)(kSubN - 1)) == 0u) {
-            float *bx = s_qbox[wv][lane / kSubN];
-            bx[0] = qlx; bx[1] = qly; bx[2] = qlz;
-            bx[3] = qhx; bx[4] = qhy; bx[5] = qhz;
-        }
-        {
-            const float lox = wave_min(qlx), hix = wave_max(qhx);
-            const float loy = wave_min(qly), hiy = wave_max(qhy);
-            const float loz = wave_min(qlz), hiz = wave_max(qhz);
-            if (lane == 0) {
-                float *bx = s_qbox[wv][kSubBoxes];
-                bx[0] = lox; bx[1] = loy; bx[2] = loz;
-                bx[3] = hix; bx[4] = hiy; bx[5] = hiz;
-            }
-        }
+        // the group's box (live bodies only), wave-uniform
+        auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
+        const float glx = uni(wave_min(live ? p.x : INFINITY)), ghx = uni(wave_max(live ? p.x : -INFINITY));
+        const float gly = uni(wave_min(live ? p.y : INFINITY)), ghy = uni(wave_max(live ? p.y : -INFINITY));
+        const float glz = uni(wave_min(live ? p.z : INFINITY)), ghz = uni(wave_max(live ? p.z : -INFINITY));
         if (lane < (unsigned)nc0) s_stk[wv][lane] = cs0 + (int32_t)lane;
         wave_lds_sync();
         float ax = 0.f, ay = 0.f, az = 0.f;
@@ -359,27 +328,25 @@ __global__ __launch_bounds__(kBlock) NB_WAVES_PER_EU(8) void k_walk_frontier(
             rec.r = make_int4(0, 0, 0, 0);
             if (inb) rec = wgrp[node];
             const float r2o = __int_as_float(rec.r.w);
-            const float *gb = s_qbox[wv][kSubBoxes];
-            bool acc = inb && box_d2(rec.q.x, rec.q.y, rec.q.z, gb[0], gb[1], gb[2], gb[3], gb[4], gb[5]) > r2o;
-            if (inb && !acc) {
-                // the whole box is only sufficient: the parts' boxes, then
-                // body by body with the wave walk's test inside a failing part
-                bool all = true;
-                for (int qq = 0; qq < kSubBoxes && all; ++qq) {
-                    const float *bx = s_qbox[wv][qq];
-                    if (box_d2(rec.q.x, rec.q.y, rec.q.z, bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]) > r2o) continue;
-                    const int i1 = min(nlive, kSubN * qq + kSubN);
-                    for (int i = kSubN * qq; i < i1; ++i) {
-                        const float4 bb = s_grp[wv][i];
-                        const float ex = rec.q.x - bb.x, ey = rec.q.y - bb.y, ez = rec.q.z - bb.z;
-                        const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
-                        if (!(e2 > r2o)) {
-                            all = false;
-                            break;
-                        }
-                    }
+            bool acc = inb && box_d2(rec.q.x, rec.q.y, rec.q.z, glx, gly, glz, ghx, ghy, ghz) > r2o;
+            // the box is only sufficient: a cell that fails it is accepted
+            // when every body passes the wave walk's test, evaluated for all
+            // the group's bodies at once (lane l, body l; one ballot per cell)
+            const bool test = inb && !acc;
+            const uint64_t m_test = __ballot(test);
+            if (m_test) {
+                const int nt = __popcll(m_test);
+                const int at = lanes_below(m_test, lane);
+                if (test) s_fc[wv][at] = make_float4(rec.q.x, rec.q.y, rec.q.z, r2o);
+                wave_lds_sync();
+                uint64_t m_fail = 0;
+                for (int k = 0; k < nt; ++k) {
+                    const float4 c = s_fc[wv][k];
+                    const float ex = c.x - p.x, ey = c.y - p.y, ez = c.z - p.z;
+                    const float e2 = __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex));
+                    if (__ballot(live && !(e2 > c.w))) m_fail |= 1ull << k;
                 }
-                acc = all;
+                if (test) acc = !((m_fail >> at) & 1ull);
             }
             const int32_t nbod = rec.r.z >> 4;
             const bool leafb = inb && !acc && nbod <= bucket;
