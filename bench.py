@@ -112,8 +112,8 @@ PEAK_FP64_TFLOPS = 78.6
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 summed over the build kernels
 # of one BH step at 1,048,576 bodies (profiles/bh_walk_build_two_plummer_1M.md),
 # and the same for the frontier walk launch; None when not profiled.
-PROFILED_BH_BUILD_BYTES = 1.181e9
-PROFILED_BH_WALK_BYTES = 2.197e8
+PROFILED_BH_BUILD_BYTES = 1.188e9
+PROFILED_BH_WALK_BYTES = 2.353e8
 # the same for one 1000-step launch of the mixed batch kernel on the 4096-system
 # comet ensemble (profiles/batch_comet_ensemble4096.md)
 PROFILED_BATCH_LAUNCH_BYTES = 2.535e8
@@ -396,7 +396,7 @@ def profile_run(capi, args):
         m, pos, vel = capi.ic_two_plummer(2, BH_BODIES)
         s.set_bodies(m, pos, vel)
         s.set_bh_opening_angle(BH_THETA)
-        s.set_tuning("bh_bucket", 32)
+        s.set_tuning("bh_bucket", 16)  # the autotuner's usual choice at this size
         s.set_iterations(args.bh_steps)
         s.evolve(capi.LEAPFROG_KDK | capi.BARNES_HUT)
     s.close()
@@ -426,7 +426,7 @@ def main():
     ap.add_argument("--profile-batch", action="store_true",
                     help="for rocprofv3: the 4096-system ensemble for --batch-steps steps, no JSON")
     ap.add_argument("--profile-bh", action="store_true",
-                    help="for rocprofv3: the 1M-body BH step (mixed, bucket 32), no events, no JSON")
+                    help="for rocprofv3: the 1M-body BH step (mixed, bucket 16), no events, no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs one system per GPU and measures one MI355X; --gpus {args.gpus} is not "
