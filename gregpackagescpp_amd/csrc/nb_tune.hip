@@ -154,9 +154,9 @@ int nb_autotune(nb_system *s, int method, const char *cache_path) {
                 } else {
                     double best = 1e30;
                     int arg = 16;
-                    for (int b : {1, 8, 16, 32, 64}) {
+                    for (int b : {1, 4, 8, 12, 16, 32, 64}) {
                         s->tune_bh_bucket = b;
-                        double ms = time_ms(st, 3, [&]() { bh_forces(s, nullptr); });
+                        double ms = time_ms(st, 5, [&]() { bh_forces(s, nullptr); });
                         if (ms < best) {
                             best = ms;
                             arg = b;

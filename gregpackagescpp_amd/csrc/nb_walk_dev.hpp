@@ -150,11 +150,11 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
 // counts.  A stack that
 // would exceed `cap` entries in all sets *werr; the depth-first wave walk
 // then redoes the whole step (its launch follows, gated on that flag).
-constexpr int kIL = 128;        // accepted cells buffered per wave (also the bucket bodies' staging)
+constexpr int kIL = 96;         // accepted cells buffered per wave (also the bucket bodies' staging)
 constexpr int kBL = 64;         // bucket ranges buffered per wave
 static_assert(kIL >= 64 && kBL >= 64, "one step adds up to 64 entries to either list after a flush");
 #ifndef NB_FRONTIER_LDS_STACK
-#define NB_FRONTIER_LDS_STACK 320  // 8 waves per SIMD fit the LDS; a test build sets a small one
+#define NB_FRONTIER_LDS_STACK 512  // 8 waves per SIMD fit the LDS (5120 B per wave); a test build sets a small one
 #endif
 constexpr int kLStack = NB_FRONTIER_LDS_STACK;  // stack entries per wave in LDS
 static_assert(kLStack >= 8, "the root's children start in LDS");
