@@ -969,12 +969,15 @@ __global__ void k_bhc_replay(int64_t n, uint32_t nent, const uint64_t *__restric
 // grouped copy puts the root at 0 and the children of y at 1 + cbase[y] +
 // rank, in octant order, with r = (first child's slot, first sorted body,
 // min(count, 2^27 - 1) << 4 | child count, r^2_open).
+// The walk records to write: the depth-first layout (wnode, indexed by node)
+// and/or the grouped layout (wgrp, children adjacent); neither when null.
 struct WalkRecs {
     WalkNode *wnode, *wgrp;
     const uint32_t *slot, *cbase;
     const uint8_t *nchild;
     double G, theta;
     int32_t total;
+    __device__ bool on() const { return wnode || wgrp; }
 };
 // The walk records of node x from its values: FP64 mass and COM, whether it
 // is a leaf, its side, skip, first and last body, child count and base, and
@@ -991,7 +994,7 @@ __device__ __forceinline__ void put_walk_recs(const WalkRecs &W, int32_t x, doub
         r2o = (float)(r * r);
     } else r2o = INFINITY;
     w.r = make_int4(sk, fi, la - fi + 1, __float_as_int(r2o));
-    W.wnode[x] = w;
+    if (W.wnode) W.wnode[x] = w;
     if (!W.wgrp) return;
     const int32_t cnt = min(la - fi + 1, (1 << 27) - 1);
     WalkNode g;
@@ -1071,10 +1074,10 @@ __global__ void k_link(int32_t total, const int32_t *__restrict__ parent, const 
         // A leaf's records are put_walk_recs' for body b: its COM and G m in
         // FP32 are the body's FP32 position stream entry, skip x + 1, one
         // body, no children, and an opening radius that accepts it always.
-        if (W.wnode) {
+        if (W.on()) {
             const float4 q = pos32[b];
             const int r2o = __float_as_int(-1.0f);
-            W.wnode[x] = WalkNode{q, make_int4(x + 1, b, 1, r2o)};
+            if (W.wnode) W.wnode[x] = WalkNode{q, make_int4(x + 1, b, 1, r2o)};
             if (W.wgrp && at < (uint32_t)total) W.wgrp[at] = WalkNode{q, make_int4(0, b, 1 << 4, r2o)};
         }
         return;
@@ -1240,7 +1243,7 @@ __global__ __launch_bounds__(kBlock) void k_com_subtrees(
                 com[p] = s_x[i];
                 com[cstride + p] = s_y[i];
                 com[2 * cstride + p] = s_z[i];
-                if (W.wnode)
+                if (W.on())
                     put_walk_recs(W, p, s_m[i], s_x[i], s_y[i], s_z[i], false, sd[j], sk[j], fi[j], la[j], nc[j],
                                   cb[j], at[j]);
             }
@@ -1313,7 +1316,7 @@ __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *
                 double sd = 0.0;
                 int32_t sk = 0, fi = 0, la = 0;
                 uint32_t slot = 0;
-                if (W.wnode) {
+                if (W.on()) {
                     sd = side[p];
                     sk = skip[p];
                     fi = first[p];
@@ -1339,7 +1342,7 @@ __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *
                 com[cstride + p] = o.y;
                 com[2 * cstride + p] = o.z;
                 mc[p] = o;
-                if (W.wnode) put_walk_recs(W, p, o.m, o.x, o.y, o.z, false, sd, sk, fi, la, s_nc[j], s_cb[j], slot);
+                if (W.on()) put_walk_recs(W, p, o.m, o.x, o.y, o.z, false, sd, sk, fi, la, s_nc[j], s_cb[j], slot);
             }
             __syncthreads();
         }
@@ -1353,7 +1356,7 @@ __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *
             }
             __syncthreads();
         }
-        if (W.wnode)
+        if (W.on())
             for (uint32_t i = t; i < cnt; i += blockDim.x)
                 write_walk_recs(W, top[i], cstride, mass, com, side, body, skip, first, last);
     }
@@ -1673,8 +1676,11 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
     hipLaunchKernelGGL(k_scan_blocks, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nbn, w.cblk.p, w.ctot.p);
     hipLaunchKernelGGL(k_scan_add, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
+    // the FP32 walks' records: the grouped layout for the frontier walk (its
+    // depth-first fallback reads that too), the depth-first one for bh_walk 2
     const bool records = s->precision == NB_PREC_MIXED;
-    const WalkRecs W{records ? w.wnode.p : nullptr, records && s->tune_bh_walk != 2 ? w.wgrp.p : nullptr, w.slot.p,
+    const bool dfs = s->tune_bh_walk == 2;
+    const WalkRecs W{records && dfs ? w.wnode.p : nullptr, records && !dfs ? w.wgrp.p : nullptr, w.slot.p,
                      w.cbase.p, w.nchild.p, s->G, s->theta, total};
     hipLaunchKernelGGL(k_link, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.parent.p, w.skip.p, w.rank.p,
                        w.cbase.p, w.slot.p, w.kid.p, w.roots.p, nroots_d, w.top.p, nroots_d + 1, W, cs, w.mass.p,
@@ -1768,15 +1774,16 @@ static void bh_forces_step(nb_system *s, int64_t *interactions, BHDrift dr) {
             auto wave = s->eps > 0 ? k_walk_wave<true> : k_walk_wave<false>;
             if (!frontier_walk) {
                 hipLaunchKernelGGL(wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wnode.p,
-                                   bucket, eps2, s->a.p, w.icount.p, nullptr);
+                                   bucket, eps2, s->a.p, w.icount.p, nullptr, nullptr, nullptr, nullptr);
             } else {
                 NB_HIP(hipStreamWaitEvent(st, w.ev_sched_out, 0));
                 auto front = s->eps > 0 ? k_walk_frontier<true> : k_walk_frontier<false>;
                 hipLaunchKernelGGL(front, dim3((unsigned)G), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wgrp.p,
                                    bucket, eps2, s->a.p, w.icount.p, w.fstack.p, cap, w.gorder.p, w.gxr.p,
                                    w.gcost.p, w.werr.p);
-                hipLaunchKernelGGL(wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p, w.wnode.p,
-                                   bucket, eps2, s->a.p, w.icount2.p, w.werr.p);
+                hipLaunchKernelGGL(wave, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.pos32s.p,
+                                   (const WalkNode *)nullptr, bucket, eps2, s->a.p, w.icount2.p, w.werr.p, w.wgrp.p,
+                                   w.slot.p, w.skip.p);
                 frontier = true;
             }
         } else {

@@ -48,12 +48,18 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // With a gate, the walk runs only when *gate is set (the frontier walk's
 // overflow flag), so it can follow that walk on the stream unconditionally.
+// Without wnode (the build wrote only the frontier walk's grouped layout)
+// each node's record is put together from its grouped record (wgrp at
+// slot[node]) and its skip pointer: the same values, two loads per visit.
 template <bool SOFT>
 __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, const float4 *__restrict__ pos,
                                                       const WalkNode *__restrict__ wnode, int bucket, float eps2,
                                                       double *__restrict__ accs,
                                                       unsigned long long *__restrict__ icount,
-                                                      const int *__restrict__ gate) {
+                                                      const int *__restrict__ gate,
+                                                      const WalkNode *__restrict__ wgrp,
+                                                      const uint32_t *__restrict__ slot,
+                                                      const int32_t *__restrict__ skip) {
 #pragma clang fp contract(fast)
     if (gate && __builtin_amdgcn_readfirstlane(*gate) == 0) return;
     // Workgroups are dealt to the 8 XCDs round robin; renumber them so that
@@ -73,7 +79,14 @@ __global__ __launch_bounds__(kBlock) void k_walk_wave(int64_t n, int32_t total, 
     int32_t node = 1;
     while (node < total) {
         node = __builtin_amdgcn_readfirstlane(node);
-        const WalkNode nd = wnode[node];
+        WalkNode nd;
+        if (wnode) {
+            nd = wnode[node];
+        } else {
+            const WalkNode g = wgrp[slot[node]];
+            nd.q = g.q;
+            nd.r = make_int4(skip[node], g.r.y, g.r.z >> 4, g.r.w);
+        }
         const float4 q = nd.q;
         const int4 r = nd.r;
         const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;

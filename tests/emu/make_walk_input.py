@@ -62,10 +62,10 @@ def write(path, m, pos, theta, eps, bucket):
     return len(wnode)
 
 
-def read_output(path, n):
+def read_output(path, n, runs=3):
     out = []
     with open(path, "rb") as f:
-        for _ in range(2):
+        for _ in range(runs):
             cnt, flag = struct.unpack("<qq", f.read(16))
             acc = np.frombuffer(f.read(24 * n), "<f8").reshape(3, n).T.copy()
             out.append((cnt, flag, acc))

@@ -338,8 +338,12 @@ int main(int argc, char **argv) {
                 g_lane_base = wv * 64;
                 w.rounds = 0;
                 run_wave(w, [&]() {
-                    if (soft) nbe::k_walk_wave<true>(n, (int32_t)total, pos, wnode, (int)bucket, eps2, accs, icount, nullptr);
-                    else nbe::k_walk_wave<false>(n, (int32_t)total, pos, wnode, (int)bucket, eps2, accs, icount, nullptr);
+                    if (soft)
+                        nbe::k_walk_wave<true>(n, (int32_t)total, pos, wnode, (int)bucket, eps2, accs, icount, nullptr,
+                                               nullptr, nullptr, nullptr);
+                    else
+                        nbe::k_walk_wave<false>(n, (int32_t)total, pos, wnode, (int)bucket, eps2, accs, icount, nullptr,
+                                                nullptr, nullptr, nullptr);
                 });
             }
         printf("wave walk: interactions %llu\n", *icount);
@@ -394,6 +398,50 @@ int main(int argc, char **argv) {
         const int64_t cnt = (int64_t)*icount, ov = *werr;
         fwrite(&cnt, 8, 1, o);
         fwrite(&ov, 8, 1, o);
+        fwrite(accs, 8, 3 * (size_t)n, o);
+    }
+    // the depth-first walk again, reading the grouped layout (slot and skip
+    // derived here: a node's children follow it in pre-order along the skip
+    // chain and sit from its record's first-child slot on)
+    {
+        int32_t *skip = galloc<int32_t>("skip", (size_t)total);
+        uint32_t *slot = galloc<uint32_t>("slot", (size_t)total);
+        for (int64_t x = 0; x < total; ++x) {
+            skip[x] = wnode[x].r.x;
+            slot[x] = 0;
+        }
+        for (int64_t x = 0; x < total; ++x) {
+            const int4 g = wgrp[slot[x]].r;
+            if ((g.z & 15) == 0) continue;
+            int32_t c = (int32_t)x + 1, k = 0;
+            while (c < skip[x] && c < total) {
+                slot[c] = (uint32_t)(g.x + k++);
+                c = skip[c];
+            }
+        }
+        double *accs = galloc<double>("accs (wave walk, grouped)", 3 * (size_t)n);
+        unsigned long long *icount = galloc<unsigned long long>("icount (wave walk, grouped)", 1);
+        const unsigned nblk = (unsigned)((n + kBlock - 1) / kBlock);
+        gridDim.x = nblk;
+        for (unsigned b = 0; b < nblk; ++b)
+            for (unsigned wv = 0; wv < kBlock / 64; ++wv) {
+                blockIdx.x = b;
+                g_block = b;
+                g_lane_base = wv * 64;
+                w.rounds = 0;
+                run_wave(w, [&]() {
+                    if (soft)
+                        nbe::k_walk_wave<true>(n, (int32_t)total, pos, nullptr, (int)bucket, eps2, accs, icount,
+                                               nullptr, wgrp, slot, skip);
+                    else
+                        nbe::k_walk_wave<false>(n, (int32_t)total, pos, nullptr, (int)bucket, eps2, accs, icount,
+                                                nullptr, wgrp, slot, skip);
+                });
+            }
+        printf("wave walk (grouped layout): interactions %llu\n", *icount);
+        const int64_t cnt = (int64_t)*icount, zero = 0;
+        fwrite(&cnt, 8, 1, o);
+        fwrite(&zero, 8, 1, o);
         fwrite(accs, 8, 3 * (size_t)n, o);
     }
     fclose(o);

@@ -9,6 +9,9 @@ arithmetic of the GPU build:
     outliers (64-body groups whose box spans the domain), with eps = 0 and
     with a ragged last group;
   * their accelerations differ only by FP32 summation order;
+  * the depth-first walk reading the grouped layout (as the frontier walk's
+    overflow fallback does) gives the same counts and accelerations, bit for
+    bit, as reading its own layout;
   * no access leaves a buffer, no intrinsic is reached by part of a wave,
     and every launch ends;
   * a frontier stack too small for the tree reports the overflow.
@@ -46,7 +49,11 @@ def run(emu, tmp_path, m, pos, bucket, eps=0.01, theta=0.5, cap=None):
         cmd += ["--cap", str(cap)]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr
-    return M.read_output(out, len(m))
+    wave, front, grouped = M.read_output(out, len(m))
+    # the depth-first walk reading the grouped layout (the frontier walk's
+    # fallback) repeats the one reading its own layout bit for bit
+    assert grouped[0] == wave[0] and np.array_equal(grouped[2], wave[2])
+    return wave, front
 
 
 def rel(a, b):
