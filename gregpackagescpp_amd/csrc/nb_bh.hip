@@ -376,20 +376,56 @@ __global__ __launch_bounds__(kBlock) void k_lcp_counts(int64_t n, const uint64_t
 }
 
 // Exclusive scan in three launches (block scan, scan of block sums, add).
-__global__ __launch_bounds__(1024) void k_scan_blocks(int64_t n, uint32_t *__restrict__ a, uint32_t *__restrict__ blk) {
-    __shared__ uint32_t sm[1024];
-    const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
-    uint32_t v = i < n ? a[i] : 0;
-    sm[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t t = threadIdx.x >= (unsigned)off ? sm[threadIdx.x - off] : 0;
-        __syncthreads();
-        sm[threadIdx.x] += t;
-        __syncthreads();
+// Exclusive scan over a 1024-thread block: wave scans by shuffles, then the
+// 16 wave totals scanned by the first wave (two barriers).  Returns the
+// exclusive prefix of v within the block; *block_total gets the block's sum.
+__device__ __forceinline__ uint32_t block_scan_1024(uint32_t v, uint32_t *s_w, uint32_t *block_total) {
+    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= (unsigned)o) x += y;
     }
-    if (i < n) a[i] = sm[threadIdx.x] - v;
-    if (threadIdx.x == 1023) blk[blockIdx.x] = sm[1023];
+    if (lane == 63u) s_w[wv] = x;
+    __syncthreads();
+    if (wv == 0) {
+        uint32_t t = lane < 16u ? s_w[lane] : 0u;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const uint32_t y = __shfl_up(t, o);
+            if (lane >= (unsigned)o) t += y;
+        }
+        if (lane < 16u) s_w[lane] = t;  // inclusive over the waves (every lane read before any wrote)
+    }
+    __syncthreads();
+    *block_total = s_w[15];
+    return (wv ? s_w[wv - 1] : 0u) + x - v;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_blocks(int64_t n, uint32_t *__restrict__ a, uint32_t *__restrict__ blk) {
+    __shared__ uint32_t s_w[16];
+    const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    const uint32_t v = i < n ? a[i] : 0;
+    uint32_t tot;
+    const uint32_t ex = block_scan_1024(v, s_w, &tot);
+    if (i < n) a[i] = ex;
+    if (threadIdx.x == 1023) blk[blockIdx.x] = tot;
+}
+
+// Minimum of the 64 bytes a[64 i ...] below `valid` (255 past it); the
+// array is padded to whole chunks of 64, so four 16-byte loads stay inside.
+__device__ __forceinline__ uint8_t chunk_min64(const uint8_t *__restrict__ a, int64_t i, int64_t valid) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(a + (i << 6));
+    const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
+    const uint32_t wd[16] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
+                             w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w};
+    const int64_t hi = valid - (i << 6);
+    unsigned m = 255u;
+#pragma unroll
+    for (int k = 0; k < 64; ++k)
+        if (k < hi) m = min(m, (wd[k >> 2] >> (8 * (k & 3))) & 0xFFu);
+    return (uint8_t)m;
 }
 
 // The scan of the block sums; with cm1 also the minima of c over chunks of
@@ -398,40 +434,25 @@ __global__ __launch_bounds__(1024) void k_scan_sums(int nb, uint32_t *__restrict
                                                     const uint8_t *__restrict__ cm1 = nullptr, int64_t n1 = 0,
                                                     uint8_t *__restrict__ cm2 = nullptr, int64_t n2 = 0,
                                                     uint8_t *__restrict__ cm3 = nullptr, int64_t n3 = 0) {
-    __shared__ uint32_t sm[1024];
+    __shared__ uint32_t s_w[16];
     // sequential chunks per thread: nb may exceed 1024
     const int per = (nb + 1023) / 1024;
     const int b = threadIdx.x * per, e = min(nb, b + per);
     uint32_t s = 0;
     for (int i = b; i < e; ++i) s += blk[i];
-    sm[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t t = threadIdx.x >= (unsigned)off ? sm[threadIdx.x - off] : 0;
-        __syncthreads();
-        sm[threadIdx.x] += t;
-        __syncthreads();
-    }
-    uint32_t run = threadIdx.x ? sm[threadIdx.x - 1] : 0;
+    uint32_t tot;
+    uint32_t run = block_scan_1024(s, s_w, &tot);
     for (int i = b; i < e; ++i) {
         uint32_t v = blk[i];
         blk[i] = run;
         run += v;
     }
-    if (threadIdx.x == 1023) *total = 1u + sm[1023];  // + root
+    if (threadIdx.x == 1023) *total = 1u + tot;  // + root
     if (!cm1) return;
-    for (int64_t i = threadIdx.x; i < n2; i += 1024) {
-        unsigned m = 255;
-        for (int64_t j = i << 6; j < min(n1, (i + 1) << 6); ++j) m = min(m, (unsigned)cm1[j]);
-        cm2[i] = (uint8_t)m;
-    }
+    for (int64_t i = threadIdx.x; i < n2; i += 1024) cm2[i] = chunk_min64(cm1, i, n1);
     __threadfence_block();
     __syncthreads();
-    for (int64_t i = threadIdx.x; i < n3; i += 1024) {
-        unsigned m = 255;
-        for (int64_t j = i << 6; j < min(n2, (i + 1) << 6); ++j) m = min(m, (unsigned)cm2[j]);
-        cm3[i] = (uint8_t)m;
-    }
+    for (int64_t i = threadIdx.x; i < n3; i += 1024) cm3[i] = chunk_min64(cm2, i, n2);
 }
 
 __global__ __launch_bounds__(1024) void k_scan_add(int64_t n, uint32_t *__restrict__ a, const uint32_t *__restrict__ blk) {
