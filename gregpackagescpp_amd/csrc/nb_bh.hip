@@ -168,6 +168,8 @@ constexpr int kSortStats = 3;
 // slots of BHWork::scal
 constexpr int kScTotal = 0, kScErr = 1, kScLong = 2, kScRoots = kScLong + kSortStats, kScalars = kScRoots + 2;
 constexpr int kScHost = kScRoots;  // total, error and long-run flags come back to the host
+// subtrees of at most this many nodes get their centres of mass in one workgroup's LDS
+constexpr int32_t kSubtree = 1024;
 
 // ---- 1. bounding cube ----
 constexpr int kBoxParts = 16384;  // most partial boxes of k_bbox_partial
@@ -644,31 +646,78 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
 }
 
 // Children of each internal node, in octant order along the skip chain:
-// their count, each child's index among its siblings, and its parent.
-__global__ void k_children(int32_t total, const int32_t *__restrict__ skip, const int32_t *__restrict__ body,
-                           uint8_t *__restrict__ nchild, uint8_t *__restrict__ rank, uint32_t *__restrict__ cbase,
-                           int32_t *__restrict__ parent) {
+// their count, each child's index among its siblings, and its parent.  The same pass lists the nodes above the subtrees (more
+// than kSubtree nodes: `top`) and the subtree roots (internal nodes of at
+// most kSubtree nodes under such a node, or the root itself when the whole
+// tree is one subtree): a top node lists itself and its root children, and
+// writes the mass and COM record of its leaf children, which no subtree
+// pass sums.  The lists take one atomic per workgroup and list (returning
+// atomics on one address serialise in the L2, about 10 ns each); their order
+// does not matter to the passes that read them.
+__global__ __launch_bounds__(kBlock) void k_children(int32_t total, const int32_t *__restrict__ skip,
+                                                     const int32_t *__restrict__ body, uint8_t *__restrict__ nchild,
+                                                     uint8_t *__restrict__ rank, uint32_t *__restrict__ cbase,
+                                                     int32_t *__restrict__ parent, int32_t *__restrict__ roots,
+                                                     uint32_t *__restrict__ nroots, int32_t *__restrict__ top,
+                                                     uint32_t *__restrict__ ntop, const double *__restrict__ mass,
+                                                     const double *__restrict__ com, int64_t cstride,
+                                                     MassCom *__restrict__ mc) {
+    __shared__ uint32_t s_nt, s_nr, s_bt, s_br;
     const int32_t x = blockIdx.x * kBlock + threadIdx.x;
-    if (x >= total) return;
-    if (x == 0) {
-        rank[0] = 0;
-        parent[0] = -1;
+    if (threadIdx.x == 0) {
+        s_nt = 0u;
+        s_nr = 0u;
     }
-    if (body[x] >= 0) {
-        nchild[x] = 0;
-        cbase[x] = 0;
-        return;
+    __syncthreads();
+    bool is_top = false;
+    int nr = 0;
+    int32_t rc[8];
+    if (x < total) {
+        if (x == 0) {
+            rank[0] = 0;
+            parent[0] = -1;
+        }
+        if (body[x] >= 0) {
+            nchild[x] = 0;
+            cbase[x] = 0;
+        } else {
+            const int32_t end = min(skip[x], total);
+            is_top = end - x > kSubtree;
+            if (x == 0 && !is_top) rc[nr++] = 0;  // the whole tree is one subtree
+            int32_t ch = x + 1, k = 0;
+            while (ch < end && k < 8) {  // an octree node has at most 8 children
+                rank[ch] = (uint8_t)k;
+                parent[ch] = x;
+                const int32_t next = max(skip[ch], ch + 1);
+                if (is_top) {
+                    if (body[ch] < 0) {
+                        if (next - ch <= kSubtree) rc[nr++] = ch;
+                    } else {
+                        MassCom v;
+                        v.m = mass[ch];
+                        v.x = com[ch];
+                        v.y = com[cstride + ch];
+                        v.z = com[2 * cstride + ch];
+                        mc[ch] = v;
+                    }
+                }
+                ch = next;
+                ++k;
+            }
+            nchild[x] = (uint8_t)k;
+            cbase[x] = (uint32_t)k;  // scanned into the first child's slot
+        }
     }
-    int32_t ch = x + 1, k = 0;
-    const int32_t end = min(skip[x], total);
-    while (ch < end && k < 8) {  // an octree node has at most 8 children
-        rank[ch] = (uint8_t)k;
-        parent[ch] = x;
-        ch = max(skip[ch], ch + 1);
-        ++k;
+    const uint32_t lt = is_top ? atomicAdd(&s_nt, 1u) : 0u;
+    const uint32_t lr = nr ? atomicAdd(&s_nr, (uint32_t)nr) : 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_bt = s_nt ? atomicAdd(ntop, s_nt) : 0u;
+        s_br = s_nr ? atomicAdd(nroots, s_nr) : 0u;
     }
-    nchild[x] = (uint8_t)k;
-    cbase[x] = (uint32_t)k;  // scanned into the first child's slot
+    __syncthreads();
+    if (is_top) top[s_bt + lt] = x;
+    for (int i = 0; i < nr; ++i) roots[s_br + lr + i] = rc[i];
 }
 
 // ---- collision pass on the tree: system::bh_s_coll (gregsys.hpp:631-686) ----
@@ -1039,8 +1088,8 @@ __device__ void write_walk_recs(const WalkRecs &W, int32_t x, int64_t cstride, c
 // node's slot in the grouped walk layout (kid[] lists the children of x at
 // 1 + cbase[x] in octant order), the subtree roots (internal nodes of at
 // most kSubtree nodes under a parent of more) and the nodes above them are
-// found first; the leaves' walk records are written on the way.
-constexpr int32_t kSubtree = 1024;
+// found first (k_children, k_link); the leaves' walk records are written on
+// the way.
 constexpr int kDepths = kLevels + 1;  // node depths 0..42
 
 // Exclusive offsets of the per-depth counts s_cur (kDepths <= 64 entries)
@@ -1066,14 +1115,9 @@ __device__ __forceinline__ void depth_offsets(uint32_t *s_cur, uint32_t *s_off) 
     }
 }
 
-__global__ void k_link(int32_t total, const int32_t *__restrict__ parent, const int32_t *__restrict__ skip,
-                       const uint8_t *__restrict__ rank, const uint32_t *__restrict__ cbase,
-                       uint32_t *__restrict__ slot, int32_t *__restrict__ kid, int32_t *__restrict__ roots,
-                       uint32_t *__restrict__ nroots, int32_t *__restrict__ top, uint32_t *__restrict__ ntop,
-                       WalkRecs W, int64_t cstride, const double *__restrict__ mass, const double *__restrict__ com,
-                       const double *__restrict__ side, const int32_t *__restrict__ body,
-                       const int32_t *__restrict__ first, const int32_t *__restrict__ last,
-                       const float4 *__restrict__ pos32, MassCom *__restrict__ mc) {
+__global__ void k_link(int32_t total, const int32_t *__restrict__ parent, const uint8_t *__restrict__ rank,
+                       const uint32_t *__restrict__ cbase, uint32_t *__restrict__ slot, int32_t *__restrict__ kid,
+                       WalkRecs W, const int32_t *__restrict__ body, const float4 *__restrict__ pos32) {
     const int32_t x = blockIdx.x * kBlock + threadIdx.x;
     if (x >= total) return;
     const int32_t p = parent[x];
@@ -1082,16 +1126,6 @@ __global__ void k_link(int32_t total, const int32_t *__restrict__ parent, const 
     if (at < (uint32_t)total && (x == 0 || at > 0)) kid[at] = x;
     const int32_t b = body[x];
     if (b >= 0) {
-        if (p >= 0 && p < total && skip[p] - p > kSubtree) {
-            // a leaf under a node above the subtrees: no subtree pass sums
-            // it, so its mass and COM record is written here
-            MassCom v;
-            v.m = mass[x];
-            v.x = com[x];
-            v.y = com[cstride + x];
-            v.z = com[2 * cstride + x];
-            mc[x] = v;
-        }
         // A leaf's records are put_walk_recs' for body b: its COM and G m in
         // FP32 are the body's FP32 position stream entry, skip x + 1, one
         // body, no children, and an opening radius that accepts it always.
@@ -1101,11 +1135,7 @@ __global__ void k_link(int32_t total, const int32_t *__restrict__ parent, const 
             if (W.wnode) W.wnode[x] = WalkNode{q, make_int4(x + 1, b, 1, r2o)};
             if (W.wgrp && at < (uint32_t)total) W.wgrp[at] = WalkNode{q, make_int4(0, b, 1 << 4, r2o)};
         }
-        return;
     }
-    const int32_t size = skip[x] - x;
-    if (size > kSubtree) top[atomicAdd(ntop, 1u)] = x;
-    else if (x == 0 || p < 0 || p >= total || skip[p] - p > kSubtree) roots[atomicAdd(nroots, 1u)] = x;
 }
 
 // calc_com (greg8tree.hpp:211-282) for internal node p from its children
@@ -1514,39 +1544,43 @@ __global__ __launch_bounds__(kBlock) void k_walk_exact(int64_t n, int32_t total,
                                                        double theta, double *__restrict__ accs,
                                                        unsigned long long *__restrict__ icount) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t b = idx[s];
-    const double px = x[b], py = x[n + b], pz = x[2 * n + b];
-    const int cs = c[s], cp = s > 0 ? c[s - 1] : 0;
-    const int32_t own = 1 + (int32_t)base[s] + max(0, cs - cp);
-    double ax = 0.0, ay = 0.0, az = 0.0;
     unsigned long long cnt = 0;
-    int32_t node = 1;
-    while (node < total) {
-        if (node == own) {
-            node = node + 1;
-            continue;
+    if (s < n) {
+        const uint32_t b = idx[s];
+        const double px = x[b], py = x[n + b], pz = x[2 * n + b];
+        const int cs = c[s], cp = s > 0 ? c[s - 1] : 0;
+        const int32_t own = 1 + (int32_t)base[s] + max(0, cs - cp);
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        int32_t node = 1;
+        while (node < total) {
+            if (node == own) {
+                node = node + 1;
+                continue;
+            }
+            const double dx = com[node] - px, dy = com[cstride + node] - py, dz = com[2 * cstride + node] - pz;
+            bool accept = body[node] >= 0;
+            if (!accept) accept = side[node] / __builtin_sqrt(dx * dx + dy * dy + dz * dz) < theta;  // is_source_cell
+            if (accept) {
+                double s2 = dx * dx + dy * dy + dz * dz + eps2;
+                double sq = __builtin_sqrt(s2);
+                double k = G * (mass[node] / s2);
+                ax += k * (dx / sq);
+                ay += k * (dy / sq);
+                az += k * (dz / sq);
+                ++cnt;
+                node = skip[node];
+            } else {
+                node = node + 1;
+            }
         }
-        const double dx = com[node] - px, dy = com[cstride + node] - py, dz = com[2 * cstride + node] - pz;
-        bool accept = body[node] >= 0;
-        if (!accept) accept = side[node] / __builtin_sqrt(dx * dx + dy * dy + dz * dz) < theta;  // is_source_cell
-        if (accept) {
-            double s2 = dx * dx + dy * dy + dz * dz + eps2;
-            double sq = __builtin_sqrt(s2);
-            double k = G * (mass[node] / s2);
-            ax += k * (dx / sq);
-            ay += k * (dy / sq);
-            az += k * (dz / sq);
-            ++cnt;
-            node = skip[node];
-        } else {
-            node = node + 1;
-        }
+        accs[s] = ax;
+        accs[n + s] = ay;
+        accs[2 * n + s] = az;
     }
-    accs[s] = ax;
-    accs[n + s] = ay;
-    accs[2 * n + s] = az;
-    atomicAdd(icount, cnt);
+    // one atomic per wave instead of one per lane: atomics on one address
+    // are serialised in the L2
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if (__lane_id() == 0) atomicAdd(icount, cnt);
 }
 
 
@@ -1692,7 +1726,8 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
     w.acc_permuted = false;  // until a walk writes accelerations in tree order
     // children, their slots in the grouped walk layout, subtree lists
     hipLaunchKernelGGL(k_children, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.skip.p, w.body.p,
-                       w.nchild.p, w.rank.p, w.cbase.p, w.parent.p);
+                       w.nchild.p, w.rank.p, w.cbase.p, w.parent.p, w.roots.p, nroots_d, w.top.p, nroots_d + 1,
+                       w.mass.p, w.com.p, cs, w.mc.p);
     const int nbn = (int)((total + 1023) / 1024);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nbn, w.cblk.p, w.ctot.p);
@@ -1703,9 +1738,8 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
     const bool dfs = s->tune_bh_walk == 2;
     const WalkRecs W{records && dfs ? w.wnode.p : nullptr, records && !dfs ? w.wgrp.p : nullptr, w.slot.p,
                      w.cbase.p, w.nchild.p, s->G, s->theta, total};
-    hipLaunchKernelGGL(k_link, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.parent.p, w.skip.p, w.rank.p,
-                       w.cbase.p, w.slot.p, w.kid.p, w.roots.p, nroots_d, w.top.p, nroots_d + 1, W, cs, w.mass.p,
-                       w.com.p, w.side.p, w.body.p, w.first.p, w.last.p, w.pos32s.p, w.mc.p);
+    hipLaunchKernelGGL(k_link, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.parent.p, w.rank.p, w.cbase.p,
+                       w.slot.p, w.kid.p, W, w.body.p, w.pos32s.p);
     // mass, COM and the internal nodes' walk records, bottom up
     const unsigned sub_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, total / 128 + 1));
     hipLaunchKernelGGL(k_com_subtrees, dim3(sub_grid), dim3(kBlock), 0, st, total, w.roots.p, nroots_d, w.skip.p,
