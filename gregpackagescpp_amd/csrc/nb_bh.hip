@@ -1245,15 +1245,31 @@ __global__ __launch_bounds__(kBlock) void k_com_subtrees(
                 const uint32_t rec = s_list[j];
                 const int32_t i = (int32_t)(rec & 2047u);
                 const int32_t c0 = (int32_t)((rec >> 11) & 2047u), ncj = (int32_t)(rec >> 22);
+                // every child's values read at once (absent children read node i's, unused)
+                int32_t cc[8];
+                bool ok[8];
+                double mv[8], xv[8], yv[8], zv[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int32_t ch = k < ncj ? s_kid[min(c0 + k, kSubtree - 1)] : 0;
+                    ok[k] = k < ncj && ch > i;
+                    cc[k] = ok[k] ? ch : i;
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    mv[k] = s_m[cc[k]];
+                    xv[k] = s_x[cc[k]];
+                    yv[k] = s_y[cc[k]];
+                    zv[k] = s_z[cc[k]];
+                }
                 double ms = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-                for (int k = ncj - 1; k >= 0; --k) {  // reverse octant order (calc_com :262-266)
-                    const int32_t ch = s_kid[min(c0 + k, kSubtree - 1)];
-                    if (ch <= i) continue;
-                    const double mc = s_m[ch];
-                    ms += mc;
-                    sx += mc * s_x[ch];
-                    sy += mc * s_y[ch];
-                    sz += mc * s_z[ch];
+#pragma unroll
+                for (int k = 7; k >= 0; --k) {  // reverse octant order (calc_com :262-266)
+                    if (!ok[k]) continue;
+                    ms += mv[k];
+                    sx += mv[k] * xv[k];
+                    sy += mv[k] * yv[k];
+                    sz += mv[k] * zv[k];
                 }
                 s_m[i] = ms;
                 s_x[i] = sx / ms;
