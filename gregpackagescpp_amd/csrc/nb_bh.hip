@@ -264,19 +264,24 @@ __global__ __launch_bounds__(kBlock) void k_keys(int64_t n, const double *__rest
         inside = inside && lo[c] <= p[c] && p[c] < hi[c];  // between (gregvec.hpp:2115-2120)
     }
     if (!inside) atomicMax(err, BH_ERR_PLACEMENT);
-    uint64_t h = 0, l = 0;
-    for (int lev = 1; lev <= kLevels; ++lev) {
+    // one level's octant digit, halving the cell towards p
+    auto digit = [&]() {
         unsigned d = 0;
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
-            double mid = (lo[c] + hi[c]) * 0.5;  // (btm + top)/2.0l, exact halving
-            bool up = p[c] >= mid;
+            const double mid = (lo[c] + hi[c]) * 0.5;  // (btm + top)/2.0l, exact halving
+            const bool up = p[c] >= mid;
             d |= (unsigned)up << c;
             lo[c] = up ? mid : lo[c];
             hi[c] = up ? hi[c] : mid;
         }
-        if (lev <= kHiLevels) h = (h << 3) | d;
-        else l = (l << 3) | d;
-    }
+        return (uint64_t)d;
+    };
+    uint64_t h = 0, l = 0;
+#pragma unroll
+    for (int lev = 1; lev <= kHiLevels; ++lev) h = (h << 3) | digit();
+#pragma unroll
+    for (int lev = kHiLevels + 1; lev <= kLevels; ++lev) l = (l << 3) | digit();
     khi[i] = h;
     klo[i] = l;
     idx[i] = (uint32_t)i;
