@@ -129,7 +129,7 @@ struct BHWork {
     hipStream_t sched = nullptr;
     hipEvent_t ev_sched_in = nullptr, ev_sched_out = nullptr;
     // collision pass
-    DBuf<double> nbox;      // node corners btm[3], top[3]
+    DBuf<double> nbox;      // node corners btm[3], top[3], padded to kBoxStride doubles
     DBuf<int32_t> leaf_of;  // sorted position -> leaf node
     DBuf<uint32_t> bc_cnt, bc_blk, bc_total;  // overlaps per sorted body, then their exclusive scan
     DBuf<int32_t> e_own, e_nb;                // overlaps: owner body, neighbour body
@@ -725,6 +725,9 @@ __global__ __launch_bounds__(kBlock) void k_children(int32_t total, const int32_
     for (int i = 0; i < nr; ++i) roots[s_br + lr + i] = rc[i];
 }
 
+// doubles per node in the corner array: a node's six corners in one aligned 64-byte segment
+constexpr int kBoxStride = 8;
+
 // ---- collision pass on the tree: system::bh_s_coll (gregsys.hpp:631-686) ----
 // Node cube corners (btm xyz, top xyz), replaying set_corners' recursion on
 // all three axes along the node's digits (the same FP64 midpoints as k_keys).
@@ -747,8 +750,8 @@ __global__ void k_node_box(int32_t total, const double *__restrict__ box, const 
         }
     }
     for (int c = 0; c < 3; ++c) {
-        nbox[(size_t)6 * x + c] = lo[c];
-        nbox[(size_t)6 * x + 3 + c] = hi[c];
+        nbox[(size_t)kBoxStride * x + c] = lo[c];
+        nbox[(size_t)kBoxStride * x + 3 + c] = hi[c];
     }
 }
 
@@ -804,7 +807,7 @@ __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, c
     }
     const int32_t own = leaf_of[s];
     double bb[6];
-    for (int k = 0; k < 6; ++k) bb[k] = nbox[(size_t)6 * own + k];
+    for (int k = 0; k < 6; ++k) bb[k] = nbox[(size_t)kBoxStride * own + k];
     const double xo = x[o], yo = x[n + o], zo = x[2 * n + o], ro = r[o];
     // margins: relative to the radii, and relative to the coordinates (the
     // rounding of the box gaps below)
@@ -823,7 +826,7 @@ __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, c
     // pre-order, so the overlaps come in the same order)
     int32_t top = parent[own];
     while (top > 0) {
-        const double *cb = nbox + (size_t)6 * top;
+        const double *cb = nbox + (size_t)kBoxStride * top;
         if (xo - cb[0] > reach && cb[3] - xo > reach && yo - cb[1] > reach && cb[4] - yo > reach &&
             zo - cb[2] > reach && cb[5] - zo > reach)
             break;
@@ -837,7 +840,7 @@ __global__ __launch_bounds__(kBlock) void k_bhc_scan(int64_t n, int32_t total, c
             p = skip[p];
             continue;
         }
-        const double *cb = nbox + (size_t)6 * p;
+        const double *cb = nbox + (size_t)kBoxStride * p;
         const bool touch = !(bb[3] < cb[0] || bb[0] > cb[3] || bb[4] < cb[1] || bb[1] > cb[4] || bb[5] < cb[2] ||
                              bb[2] > cb[5]);
         const double gx = fmax(fmax(cb[0] - xo, xo - cb[3]), 0.0);
@@ -1891,7 +1894,7 @@ static void bh_collision_pass(nb_system *s) {
     const int32_t total = (int32_t)w.nodes;
     hipStream_t st = s->stream;
     ScopedTimer tm(s->prof, "bh_coll", st);
-    w.nbox.ensure(6 * (size_t)w.cap_nodes);
+    w.nbox.ensure((size_t)kBoxStride * w.cap_nodes);
     w.leaf_of.ensure((size_t)w.cap_n);
     w.bc_cnt.ensure((size_t)w.cap_n);
     w.bc_blk.ensure((size_t)(w.cap_n + 1023) / 1024 + 1);
