@@ -1550,10 +1550,12 @@ __global__ __launch_bounds__(1024) void k_group_order(int64_t ngroups, int64_t s
     }
 }
 
-__global__ void k_frontier_reset(int *__restrict__ werr, unsigned long long *__restrict__ icount2) {
+__global__ void k_frontier_reset(int *__restrict__ werr, unsigned long long *__restrict__ icount2,
+                                 unsigned long long *__restrict__ icount) {
     if (threadIdx.x == 0) {
         werr[0] = 0;
         icount2[0] = 0;
+        icount[0] = 0;
     }
 }
 
@@ -1820,7 +1822,7 @@ static void bh_forces_step(nb_system *s, int64_t *interactions, BHDrift dr) {
     const int32_t cap = s->tune_bh_stack > 0 ? s->tune_bh_stack : kFStackDefault;
     if (frontier_walk) {
         w.fstack.ensure((size_t)G * kFWaves * (size_t)cap);
-        w.werr.ensure(1); w.icount2.ensure(1); w.gxr.ensure(9);
+        w.werr.ensure(1); w.icount2.ensure(1); w.icount.ensure(1); w.gxr.ensure(9);
         if (!w.sched) {
             NB_HIP(hipStreamCreateWithFlags(&w.sched, hipStreamNonBlocking));
             NB_HIP(hipEventCreateWithFlags(&w.ev_sched_in, hipEventDisableTiming));
@@ -1834,7 +1836,7 @@ static void bh_forces_step(nb_system *s, int64_t *interactions, BHDrift dr) {
             NB_HIP(hipMemsetAsync(w.gcost.p, 0, (size_t)ngroups * sizeof(uint32_t), w.sched));
             w.gcost_n = ngroups;
         }
-        hipLaunchKernelGGL(k_frontier_reset, dim3(1), dim3(64), 0, w.sched, w.werr.p, w.icount2.p);
+        hipLaunchKernelGGL(k_frontier_reset, dim3(1), dim3(64), 0, w.sched, w.werr.p, w.icount2.p, w.icount.p);
         hipLaunchKernelGGL(k_group_order, dim3(8), dim3(1024), 0, w.sched, ngroups, span, w.gcost.p, w.gorder.p,
                            w.gxr.p);
         NB_HIP(hipGetLastError());
@@ -1843,7 +1845,7 @@ static void bh_forces_step(nb_system *s, int64_t *interactions, BHDrift dr) {
     bh_build(s, dr);
     const int32_t total = (int32_t)w.nodes;
     const int64_t cs = w.cap_nodes;
-    NB_HIP(hipMemsetAsync(w.icount.p, 0, sizeof(unsigned long long), st));
+    if (!frontier_walk) NB_HIP(hipMemsetAsync(w.icount.p, 0, sizeof(unsigned long long), st));  // else the schedule's reset
     bool frontier = false;
     {
         ScopedTimer tm(s->prof, "bh_walk", st);
