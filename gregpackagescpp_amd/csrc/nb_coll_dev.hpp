@@ -615,7 +615,31 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MAXB <
                                 mr = fmaxf(mr, s_R[j] * inv);
                             }
                         } else {
-                            for (int j = jt; j < je; ++j) {
+                            // the block's own tile: pairs of j, the self pair taking inv = 0
+                            int j = jt;
+                            for (; j + 1 < je; j += 2) {
+                                float inv[2], wq[2], dq[2][3];
+#pragma unroll
+                                for (int u = 0; u < 2; ++u) {
+                                    const float4 pj = s_pos[j + u];
+                                    dq[u][0] = pj.x - pi.x;
+                                    dq[u][1] = pj.y - pi.y;
+                                    dq[u][2] = pj.z - pi.z;
+                                    const float r2 = __builtin_fmaf(
+                                        dq[u][2], dq[u][2], __builtin_fmaf(dq[u][1], dq[u][1], dq[u][0] * dq[u][0]));
+                                    inv[u] = j + u == i ? 0.f : __builtin_amdgcn_rsqf(r2);
+                                    wq[u] = pj.w * inv[u] * (inv[u] * inv[u]);
+                                }
+#pragma unroll
+                                for (int u = 0; u < 2; ++u) {
+                                    px = __builtin_fmaf(wq[u], dq[u][0], px);
+                                    py = __builtin_fmaf(wq[u], dq[u][1], py);
+                                    pz = __builtin_fmaf(wq[u], dq[u][2], pz);
+                                }
+                                mi = fmaxf(mi, fmaxf(inv[0], inv[1]));
+                                mr = fmaxf(mr, fmaxf(s_R[j] * inv[0], s_R[j + 1] * inv[1]));
+                            }
+                            for (; j < je; ++j) {
                                 const float4 pj = s_pos[j];
                                 const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
                                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
