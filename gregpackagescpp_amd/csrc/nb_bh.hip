@@ -480,29 +480,34 @@ __device__ __forceinline__ unsigned digit_at(uint64_t hi, uint64_t lo, int level
 // loads) per step, so a long run costs a handful of loads instead of a
 // search over the keys.
 
-// First index in [start, min(64 (chunk + 1), valid)) of chunk `chunk` of a
-// with a[i] < L (L <= 42), or -1.  Arrays are padded to whole chunks.
-__device__ __forceinline__ int64_t first_below_in(const uint8_t *__restrict__ a, int64_t chunk, int64_t start,
-                                                  int64_t valid, int L) {
+// First index in [start, 64 (chunk + 1)) of chunk `chunk` of a with
+// a[i] < L (1 <= L <= 42), or -1.  Arrays are padded to whole chunks.  The
+// padding needs no mask: a search starts at or before the last valid entry,
+// and the entry of position n - 1 (c = 0, and the minimum 0 of every chunk
+// holding it) lies before any padding, so the first entry below L is valid.
+__device__ __forceinline__ int64_t first_below_in(const uint8_t *__restrict__ a, int64_t chunk, int64_t start, int L) {
     const int64_t b = chunk << 6;
     const uint4 *p = reinterpret_cast<const uint4 *>(a + b);
     const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
     const uint32_t wd[16] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
                              w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w};
-    const int lo = (int)max<int64_t>(0, start - b), hi = (int)min<int64_t>(64, valid - b);
-    int found = -1;
+    const int lo = (int)max<int64_t>(0, start - b);
+    const uint32_t sub = 0x01010101u * (uint32_t)L;
+    int fq = -1;
+    uint32_t fw = 0u;
 #pragma unroll
     for (int q = 15; q >= 0; --q) {
-        uint32_t x = wd[q];
-        // bytes outside [lo, hi) read as 0xFF (never below L)
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (4 * q + k < lo || 4 * q + k >= hi) x |= 0xFFu << (8 * k);
-        // bytes below L: the lowest flagged byte is exact (borrows only run upwards)
-        const uint32_t f = (x - 0x01010101u * (uint32_t)L) & ~x & 0x80808080u;
-        if (f) found = 4 * q + (__builtin_ctz(f) >> 3);
+        // per byte, (x | 0x80) - L keeps its high bit unless x < L (x < 128,
+        // so no byte borrows from the next one); bytes before lo are dropped
+        uint32_t f = ~((wd[q] | 0x80808080u) - sub) & 0x80808080u;
+        const int sh = lo - 4 * q;
+        f = sh <= 0 ? f : sh >= 4 ? 0u : f & (~0u << (8 * sh));
+        if (f) {
+            fw = f;
+            fq = q;
+        }
     }
-    return found < 0 ? -1 : b + found;
+    return fq < 0 ? -1 : b + 4 * fq + (__builtin_ctz(fw) >> 3);
 }
 
 struct RunIndex {
@@ -510,7 +515,8 @@ struct RunIndex {
     int64_t n, n1, n2, n3;
 };
 
-// Last position of the run sharing at least L >= 1 digits with s: up the
+// First position t >= s with c[t] < L (L >= 1), which is the last position
+// of the run sharing at least L digits with s when c[s - 1] < L: up the
 // levels through the rest of each chunk, then down inside the chunk found
 // (one inlined chunk scan, so the code stays small).
 __device__ int64_t run_end(const RunIndex &R, int64_t s, int L) {
@@ -518,8 +524,7 @@ __device__ int64_t run_end(const RunIndex &R, int64_t s, int L) {
     int64_t start = s, chunk = s >> 6, found;
     for (;;) {
         const uint8_t *a = lev == 0 ? R.c : lev == 1 ? R.cm1 : lev == 2 ? R.cm2 : R.cm3;
-        const int64_t valid = lev == 0 ? R.n : lev == 1 ? R.n1 : lev == 2 ? R.n2 : R.n3;
-        if (found = first_below_in(a, chunk, start, valid, L); found >= 0) {
+        if (found = first_below_in(a, chunk, start, L); found >= 0) {
             if (lev == 0) return found;
             break;
         }
@@ -536,8 +541,7 @@ __device__ int64_t run_end(const RunIndex &R, int64_t s, int L) {
     while (lev > 0) {  // down: the first entry below L inside the chunk found
         --lev;
         const uint8_t *a = lev == 0 ? R.c : lev == 1 ? R.cm1 : lev == 2 ? R.cm2 : R.cm3;
-        const int64_t valid = lev == 0 ? R.n : lev == 1 ? R.n1 : lev == 2 ? R.n2 : R.n3;
-        found = first_below_in(a, found, found << 6, valid, L);
+        found = first_below_in(a, found, found << 6, L);
         if (found < 0) return R.n - 1;  // unreachable: that chunk's minimum is below L
     }
     return found;
@@ -611,6 +615,16 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
         octant[0] = -1;
         side[0] = hi - lo;
     }
+    // run ends, deepest level first: the run at level L holds the one at
+    // L + 1, which ends at re with c[re] <= L, so it ends at re too unless
+    // c[re] == L, and then at the first t > re with c[t] < L
+    int64_t re = -1;
+    for (int L = cs; L > cp; --L) {
+        if (re < 0 || R.c[re] == L) re = run_end(R, re < 0 ? s : re + 1, L);
+        const int32_t node = n0 + (L - cp - 1);
+        last[node] = (int32_t)re;
+        skip[node] = re >= n - 1 ? total : 1 + (int32_t)base[re + 1];
+    }
     int lev = 0;
     for (int L = cp + 1; L <= cs; ++L) {
         for (; lev < L; ++lev) {
@@ -620,10 +634,7 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
             hi = up ? hi : mid;
         }
         const int32_t node = n0 + (L - cp - 1);
-        const int64_t end = run_end(R, s, L);
         first[node] = (int32_t)s;
-        last[node] = (int32_t)end;
-        skip[node] = end >= n - 1 ? total : 1 + (int32_t)base[end + 1];
         body[node] = -1;
         depth[node] = (uint8_t)L;
         octant[node] = (int8_t)digit_at(h, l, L);
