@@ -99,6 +99,7 @@ struct BHWork {
     DBuf<WalkNode> wgrp;    // the same records with each node's children adjacent
     DBuf<uint8_t> rank;     // index of a node among its parent's children
     DBuf<uint32_t> cbase;   // exclusive scan of nchild (children of x at 1 + cbase[x])
+    DBuf<uint32_t> cbraw;   // nchild, then its scan within blocks of 1024 (cbase less cblk[x >> 10])
     DBuf<uint32_t> slot;    // position of each node in the grouped layout
     DBuf<int32_t> kid;      // node at each position of the grouped layout (children of x: 1 + cbase[x] ...)
     DBuf<uint32_t> cblk, ctot;
@@ -555,7 +556,9 @@ constexpr int kReSkip = 0, kReFromPrevious = 1, kReFromInput = 2;
 // Node numbering: the root is 0 and the nodes starting at sorted position s
 // are 1 + base[s] + j, internal levels (c[s-1], c[s]] first, then the leaf
 // of s, so an exclusive scan of cnt numbers every node in pre-order with
-// octants 0..7 (the order cell_iterator visits them).  One thread per s
+// octants 0..7 (the order cell_iterator visits them).  base[s] holds that
+// scan within s's block of 1024 and bblk[s >> 10] the blocks before it (the
+// readers add the two; no pass writes the sum).  One thread per s
 // writes the nodes starting at s:
 //  * first = s, last = the end of the run sharing L digits (a node's bodies
 //    are contiguous), skip = the first node starting at last + 1 (the next
@@ -572,8 +575,8 @@ constexpr int kReSkip = 0, kReFromPrevious = 1, kReFromInput = 2;
 // sequential addresses.
 __global__ __launch_bounds__(kBlock) void k_build_nodes(
     int64_t n, const RunIndex R, const uint32_t *__restrict__ base, const uint32_t *__restrict__ total_p,
-    const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo, const double *__restrict__ box,
-    uint32_t *__restrict__ idx, const double *__restrict__ x, const double *__restrict__ v,
+    const uint32_t *__restrict__ bblk, const uint64_t *__restrict__ khi, const uint64_t *__restrict__ klo,
+    const double *__restrict__ box, uint32_t *__restrict__ idx, const double *__restrict__ x, const double *__restrict__ v,
     const double *__restrict__ m, const double *__restrict__ r, const double *__restrict__ e, int re_mode,
     const int32_t *__restrict__ perm, double *__restrict__ xo, double *__restrict__ vo, double *__restrict__ mo,
     double *__restrict__ ro, double *__restrict__ eo, int32_t *__restrict__ permo, float4 *__restrict__ pos32,
@@ -604,7 +607,7 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
     // nodes starting at s
     const int cs = R.c[s], cp = s > 0 ? R.c[s - 1] : 0;
     const uint64_t h = khi[s], l = klo[s];
-    const int32_t n0 = 1 + (int32_t)base[s];
+    const int32_t n0 = 1 + (int32_t)(base[s] + bblk[s >> 10]);
     double lo = box[0], hi = box[3];
     if (s == 0) {
         first[0] = 0;
@@ -623,7 +626,7 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
         if (re < 0 || R.c[re] == L) re = run_end(R, re < 0 ? s : re + 1, L);
         const int32_t node = n0 + (L - cp - 1);
         last[node] = (int32_t)re;
-        skip[node] = re >= n - 1 ? total : 1 + (int32_t)base[re + 1];
+        skip[node] = re >= n - 1 ? total : 1 + (int32_t)(base[re + 1] + bblk[(re + 1) >> 10]);
     }
     int lev = 0;
     for (int L = cp + 1; L <= cs; ++L) {
@@ -1134,13 +1137,17 @@ __device__ __forceinline__ void depth_offsets(uint32_t *s_cur, uint32_t *s_off) 
     }
 }
 
+// It also completes the scan of the child counts: cbase[x] = cbraw[x] (the
+// scan within x's block of 1024) + cblk[x >> 10] (the blocks before it).
 __global__ void k_link(int32_t total, const int32_t *__restrict__ parent, const uint8_t *__restrict__ rank,
-                       const uint32_t *__restrict__ cbase, uint32_t *__restrict__ slot, int32_t *__restrict__ kid,
+                       const uint32_t *__restrict__ cbraw, const uint32_t *__restrict__ cblk,
+                       uint32_t *__restrict__ cbase, uint32_t *__restrict__ slot, int32_t *__restrict__ kid,
                        WalkRecs W, const int32_t *__restrict__ body, const float4 *__restrict__ pos32) {
     const int32_t x = blockIdx.x * kBlock + threadIdx.x;
     if (x >= total) return;
+    cbase[x] = cbraw[x] + cblk[x >> 10];
     const int32_t p = parent[x];
-    const uint32_t at = x == 0 ? 0u : (p >= 0 && p < total ? 1u + cbase[p] + rank[x] : 0u);
+    const uint32_t at = x == 0 ? 0u : (p >= 0 && p < total ? 1u + cbraw[p] + cblk[p >> 10] + rank[x] : 0u);
     slot[x] = at;
     if (at < (uint32_t)total && (x == 0 || at > 0)) kid[at] = x;
     const int32_t b = body[x];
@@ -1572,7 +1579,8 @@ __global__ void k_frontier_reset(int *__restrict__ werr, unsigned long long *__r
 
 // ---- 6b. per-lane walk with the reference's per-body MAC, FP64 ----
 __global__ __launch_bounds__(kBlock) void k_walk_exact(int64_t n, int32_t total, const uint32_t *__restrict__ idx,
-                                                       const uint32_t *__restrict__ base, const uint8_t *__restrict__ c,
+                                                       const uint32_t *__restrict__ base,
+                                                       const uint32_t *__restrict__ bblk, const uint8_t *__restrict__ c,
                                                        const double *__restrict__ x, const double *__restrict__ mass,
                                                        const double *__restrict__ com, int64_t cstride,
                                                        const double *__restrict__ side,
@@ -1586,7 +1594,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_exact(int64_t n, int32_t total,
         const uint32_t b = idx[s];
         const double px = x[b], py = x[n + b], pz = x[2 * n + b];
         const int cs = c[s], cp = s > 0 ? c[s - 1] : 0;
-        const int32_t own = 1 + (int32_t)base[s] + max(0, cs - cp);
+        const int32_t own = 1 + (int32_t)(base[s] + bblk[s >> 10]) + max(0, cs - cp);
         double ax = 0.0, ay = 0.0, az = 0.0;
         int32_t node = 1;
         while (node < total) {
@@ -1700,7 +1708,6 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nb, w.blk.p, total_d, w.cm1.p, n1, w.cm2.p, n2,
                        w.cm3.p, n3);
-    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, st, n, w.cnt.p, w.blk.p);
     // Node arrays for at least 2n nodes (or the last build's count and a
     // quarter), so the nodes are written while the host waits for the count;
     // with more nodes, k_build_nodes writes nothing and runs again below.
@@ -1710,7 +1717,7 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
         w.first.ensure(t); w.last.ensure(t); w.skip.ensure(t); w.parent.ensure(t); w.body.ensure(t);
         w.depth.ensure(t); w.nchild.ensure(t); w.octant.ensure(t); w.mass.ensure(t); w.com.ensure(3 * t);
         w.side.ensure(t); w.wnode.ensure(t); w.slot.ensure(t); w.kid.ensure(t); w.roots.ensure(t); w.top.ensure(t);
-        w.wgrp.ensure(t); w.rank.ensure(t); w.cbase.ensure(t); w.cblk.ensure(t / 1024 + 2); w.ctot.ensure(1);
+        w.wgrp.ensure(t); w.rank.ensure(t); w.cbase.ensure(t); w.cbraw.ensure(t); w.cblk.ensure(t / 1024 + 2); w.ctot.ensure(1);
         w.mc.ensure(t);
         w.cap_nodes = want;
     };
@@ -1722,8 +1729,8 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
     const int re_mode = !need_re ? kReSkip : re_from_tree ? kReFromPrevious : kReFromInput;
     // nodes, and the bodies into tree order
     auto build_nodes = [&]() {
-        hipLaunchKernelGGL(k_build_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, R, w.cnt.p, total_d, w.khi.p,
-                           w.klo.p, w.box.p, w.idx.p, s->x.p, s->v.p, w.permuted ? w.mt.p : s->m.p,
+        hipLaunchKernelGGL(k_build_nodes, dim3(blocks(n)), dim3(kBlock), 0, st, n, R, w.cnt.p, total_d, w.blk.p,
+                           w.khi.p, w.klo.p, w.box.p, w.idx.p, s->x.p, s->v.p, w.permuted ? w.mt.p : s->m.p,
                            re_from_tree ? w.rt.p : s->r.p, re_from_tree ? w.et.p : s->e.p, re_mode,
                            w.permuted ? w.perm.p : nullptr, w.xa.p, w.va.p, w.mt_alt.p, w.rt_alt.p, w.et_alt.p,
                            w.perm_alt.p, w.pos32s.p, s->G, w.first.p, w.last.p, w.skip.p, w.body.p, w.depth.p,
@@ -1763,20 +1770,19 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
     w.acc_permuted = false;  // until a walk writes accelerations in tree order
     // children, their slots in the grouped walk layout, subtree lists
     hipLaunchKernelGGL(k_children, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.skip.p, w.body.p,
-                       w.nchild.p, w.rank.p, w.cbase.p, w.parent.p, w.roots.p, nroots_d, w.top.p, nroots_d + 1,
+                       w.nchild.p, w.rank.p, w.cbraw.p, w.parent.p, w.roots.p, nroots_d, w.top.p, nroots_d + 1,
                        w.mass.p, w.com.p, cs, w.mc.p);
     const int nbn = (int)((total + 1023) / 1024);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbraw.p, w.cblk.p);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nbn, w.cblk.p, w.ctot.p);
-    hipLaunchKernelGGL(k_scan_add, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbase.p, w.cblk.p);
     // the FP32 walks' records: the grouped layout for the frontier walk (its
     // depth-first fallback reads that too), the depth-first one for bh_walk 2
     const bool records = s->precision == NB_PREC_MIXED;
     const bool dfs = s->tune_bh_walk == 2;
     const WalkRecs W{records && dfs ? w.wnode.p : nullptr, records && !dfs ? w.wgrp.p : nullptr, w.slot.p,
                      w.cbase.p, w.nchild.p, s->G, s->theta, total};
-    hipLaunchKernelGGL(k_link, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.parent.p, w.rank.p, w.cbase.p,
-                       w.slot.p, w.kid.p, W, w.body.p, w.pos32s.p);
+    hipLaunchKernelGGL(k_link, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.parent.p, w.rank.p, w.cbraw.p,
+                       w.cblk.p, w.cbase.p, w.slot.p, w.kid.p, W, w.body.p, w.pos32s.p);
     // mass, COM and the internal nodes' walk records, bottom up
     const unsigned sub_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, total / 128 + 1));
     hipLaunchKernelGGL(k_com_subtrees, dim3(sub_grid), dim3(kBlock), 0, st, total, w.roots.p, nroots_d, w.skip.p,
@@ -1879,7 +1885,8 @@ static void bh_forces_step(nb_system *s, int64_t *interactions, BHDrift dr) {
                 frontier = true;
             }
         } else {
-            hipLaunchKernelGGL(k_walk_exact, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.cnt.p, w.c.p,
+            hipLaunchKernelGGL(k_walk_exact, dim3(blocks(n)), dim3(kBlock), 0, st, n, total, w.idx.p, w.cnt.p, w.blk.p,
+                               w.c.p,
                                s->x.p, w.mass.p, w.com.p, cs, w.side.p, w.body.p, w.skip.p, s->G, s->eps * s->eps,
                                s->theta, s->a.p, w.icount.p);
         }
