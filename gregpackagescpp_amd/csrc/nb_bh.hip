@@ -65,6 +65,18 @@ enum { BH_ERR_NONE = 0, BH_ERR_PLACEMENT = 1, BH_ERR_INDIST = 2, BH_ERR_DEPTH = 
 struct __align__(32) MassCom {
     double m, x, y, z;
 };
+// The nodes above the subtrees, when there are at most kTopOwn of them:
+// each one's record by its index in the `top` list (written by k_children),
+// and its children's MassCom records at 8 index + rank (k_children for leaf
+// children, k_com_subtrees for subtree roots), so the one workgroup that
+// sums them reads adjacent lines instead of one line per array and child.
+constexpr int kTopOwn = 1024;
+struct __align__(16) TopRec {
+    double side;
+    int32_t skip, first, last;
+    uint8_t depth, nc;
+    int32_t kids[8];  // children in octant order, -1 past nc
+};
 
 struct BHWork {
     int64_t cap_n = 0, cap_nodes = 0;
@@ -105,6 +117,9 @@ struct BHWork {
     DBuf<uint32_t> cblk, ctot;
     DBuf<int32_t> roots, top;  // subtree roots and the nodes above them (bottom-up pass)
     DBuf<MassCom> mc;          // mass and COM of the subtree roots and the nodes above them
+    DBuf<TopRec> ttop;         // records of the nodes above the subtrees (up to kTopOwn)
+    DBuf<MassCom> tk;          // their children's mass and COM, 8 per node
+    DBuf<int32_t> rdest;       // per subtree root: its record's index in tk, or -1
     DBuf<int32_t> fstack;   // frontier walk: one node stack per wave
     DBuf<int64_t> gxr;      // frontier walk: the XCD ranges of the group order
     DBuf<int> werr;         // frontier walk: a stack would have overflowed
@@ -670,7 +685,9 @@ __global__ __launch_bounds__(kBlock) void k_build_nodes(
 // most kSubtree nodes under such a node, or the root itself when the whole
 // tree is one subtree): a top node lists itself and its root children, and
 // writes the mass and COM record of its leaf children, which no subtree
-// pass sums.  The lists take one atomic per workgroup and list (returning
+// pass sums.  The first kTopOwn top nodes also write their TopRec, their
+// leaf children's records into tk, and each root child's tk index into
+// rdest.  The lists take one atomic per workgroup and list (returning
 // atomics on one address serialise in the L2, about 10 ns each); their order
 // does not matter to the passes that read them.
 __global__ __launch_bounds__(kBlock) void k_children(int32_t total, const int32_t *__restrict__ skip,
@@ -680,7 +697,11 @@ __global__ __launch_bounds__(kBlock) void k_children(int32_t total, const int32_
                                                      uint32_t *__restrict__ nroots, int32_t *__restrict__ top,
                                                      uint32_t *__restrict__ ntop, const double *__restrict__ mass,
                                                      const double *__restrict__ com, int64_t cstride,
-                                                     MassCom *__restrict__ mc) {
+                                                     MassCom *__restrict__ mc, const double *__restrict__ side,
+                                                     const int32_t *__restrict__ first,
+                                                     const int32_t *__restrict__ last,
+                                                     const uint8_t *__restrict__ depth, TopRec *__restrict__ ttop,
+                                                     MassCom *__restrict__ tk, int32_t *__restrict__ rdest) {
     __shared__ uint32_t s_nt, s_nr, s_bt, s_br;
     const int32_t x = blockIdx.x * kBlock + threadIdx.x;
     if (threadIdx.x == 0) {
@@ -689,8 +710,8 @@ __global__ __launch_bounds__(kBlock) void k_children(int32_t total, const int32_
     }
     __syncthreads();
     bool is_top = false;
-    int nr = 0;
-    int32_t rc[8];
+    int nr = 0, nk = 0;
+    int32_t rc[8], rk[8];
     if (x < total) {
         if (x == 0) {
             rank[0] = 0;
@@ -702,7 +723,10 @@ __global__ __launch_bounds__(kBlock) void k_children(int32_t total, const int32_
         } else {
             const int32_t end = min(skip[x], total);
             is_top = end - x > kSubtree;
-            if (x == 0 && !is_top) rc[nr++] = 0;  // the whole tree is one subtree
+            if (x == 0 && !is_top) {  // the whole tree is one subtree
+                rk[nr] = -1;
+                rc[nr++] = 0;
+            }
             int32_t ch = x + 1, k = 0;
             while (ch < end && k < 8) {  // an octree node has at most 8 children
                 rank[ch] = (uint8_t)k;
@@ -710,7 +734,10 @@ __global__ __launch_bounds__(kBlock) void k_children(int32_t total, const int32_
                 const int32_t next = max(skip[ch], ch + 1);
                 if (is_top) {
                     if (body[ch] < 0) {
-                        if (next - ch <= kSubtree) rc[nr++] = ch;
+                        if (next - ch <= kSubtree) {
+                            rk[nr] = k;
+                            rc[nr++] = ch;
+                        }
                     } else {
                         MassCom v;
                         v.m = mass[ch];
@@ -725,6 +752,7 @@ __global__ __launch_bounds__(kBlock) void k_children(int32_t total, const int32_
             }
             nchild[x] = (uint8_t)k;
             cbase[x] = (uint32_t)k;  // scanned into the first child's slot
+            nk = k;
         }
     }
     const uint32_t lt = is_top ? atomicAdd(&s_nt, 1u) : 0u;
@@ -735,8 +763,39 @@ __global__ __launch_bounds__(kBlock) void k_children(int32_t total, const int32_
         s_br = s_nr ? atomicAdd(nroots, s_nr) : 0u;
     }
     __syncthreads();
-    if (is_top) top[s_bt + lt] = x;
-    for (int i = 0; i < nr; ++i) roots[s_br + lr + i] = rc[i];
+    const uint32_t ti = s_bt + lt;
+    const bool own = is_top && ti < (uint32_t)kTopOwn;
+    if (is_top) top[ti] = x;
+    for (int i = 0; i < nr; ++i) {
+        roots[s_br + lr + i] = rc[i];
+        rdest[s_br + lr + i] = own && rk[i] >= 0 ? (int32_t)(8 * ti) + rk[i] : -1;
+    }
+    if (own) {  // the children again (top nodes are few): the record, and the leaves' sums
+        TopRec tr;
+        tr.side = side[x];
+        tr.skip = skip[x];
+        tr.first = first[x];
+        tr.last = last[x];
+        tr.depth = depth[x];
+        tr.nc = (uint8_t)nk;
+        int32_t ch = x + 1;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            tr.kids[k] = k < nk ? ch : -1;
+            if (k < nk) {
+                if (body[ch] >= 0) {
+                    MassCom v;
+                    v.m = mass[ch];
+                    v.x = com[ch];
+                    v.y = com[cstride + ch];
+                    v.z = com[2 * cstride + ch];
+                    tk[8 * ti + k] = v;
+                }
+                ch = max(skip[ch], ch + 1);
+            }
+        }
+        ttop[ti] = tr;
+    }
 }
 
 // doubles per node in the corner array: a node's six corners in one aligned 64-byte segment
@@ -1193,7 +1252,8 @@ __global__ __launch_bounds__(kBlock) void k_com_subtrees(
     const int32_t *__restrict__ skip, const int32_t *__restrict__ body, const uint8_t *__restrict__ depth,
     const uint8_t *__restrict__ nchild, const uint32_t *__restrict__ cbase, const int32_t *__restrict__ kid,
     double *__restrict__ mass, double *__restrict__ com, int64_t cstride, WalkRecs W, const double *__restrict__ side,
-    const int32_t *__restrict__ first, const int32_t *__restrict__ last, MassCom *__restrict__ mc) {
+    const int32_t *__restrict__ first, const int32_t *__restrict__ last, MassCom *__restrict__ mc,
+    const int32_t *__restrict__ rdest, MassCom *__restrict__ tk) {
     constexpr int kPer = kSubtree / kBlock;  // nodes per thread
     __shared__ double s_m[kSubtree], s_x[kSubtree], s_y[kSubtree], s_z[kSubtree];
     __shared__ int16_t s_kid[kSubtree];   // children of the subtree's nodes, local indices, grouped layout order
@@ -1311,6 +1371,8 @@ __global__ __launch_bounds__(kBlock) void k_com_subtrees(
             v.y = s_y[0];
             v.z = s_z[0];
             mc[r] = v;  // the root's, for the pass over the nodes above the subtrees
+            const int32_t d = rdest[q];
+            if (d >= 0) tk[d] = v;
         }
         // this thread's internal nodes out, with their walk records (the
         // record inputs of all of them loaded at once; leaves read the root's)
@@ -1345,16 +1407,22 @@ __global__ __launch_bounds__(kBlock) void k_com_subtrees(
     }
 }
 
-// The nodes above the subtrees, one workgroup.  Up to kTopLds of them are
-// sorted by depth into LDS together with their child lists, so each depth
-// costs one round of independent loads -- each child's mass and COM from its
-// 32-byte MassCom record, and the node's own walk-record inputs -- and a
-// barrier; a node's walk records are written as soon as its sums are.  (One
-// workgroup's loads are limited by the cache lines it can request, so one
-// line per child instead of one per SoA array is what sets the time.)  More
-// such nodes: every depth rescans the list with the child lists in memory,
-// and the walk records follow in one sweep.
+// The nodes above the subtrees, one workgroup.  They are sorted by depth
+// into LDS together with their child lists.  Up to kTopOwn of them, each
+// thread owns one and loads all its inputs before the first depth: every
+// child's mass and COM from its 32-byte MassCom record (the children that
+// are themselves above the subtrees excepted: their sums come from LDS, by
+// way of a node -> list position table) and its own walk-record inputs.
+// Each depth then costs LDS reads and a barrier, not a round trip to
+// memory.  Up to kTopLds of them, each depth takes one round of independent
+// loads of the children's records.  (One workgroup's loads are limited by
+// the cache lines it can request, so one line per child instead of one per
+// SoA array is what sets the time.)  More such nodes: every depth rescans
+// the list with the child lists in memory, and the walk records follow in
+// one sweep.  A node's walk records are written as soon as its sums are.
 constexpr uint32_t kTopLds = 2048;
+constexpr int kTopHash = 2048;  // open-addressing table of node -> list position (half full at most)
+__device__ __forceinline__ uint32_t top_hash(int32_t x) { return ((uint32_t)x * 2654435761u) >> 21; }
 __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *__restrict__ top,
                                                   const uint32_t *__restrict__ ntop, const int32_t *__restrict__ skip,
                                                   const int32_t *__restrict__ body, const uint8_t *__restrict__ depth,
@@ -1362,21 +1430,117 @@ __global__ __launch_bounds__(1024) void k_com_top(int32_t total, const int32_t *
                                                   const int32_t *__restrict__ kid, double *__restrict__ mass,
                                                   double *__restrict__ com, int64_t cstride, WalkRecs W,
                                                   const double *__restrict__ side, const int32_t *__restrict__ first,
-                                                  const int32_t *__restrict__ last, MassCom *__restrict__ mc) {
+                                                  const int32_t *__restrict__ last, MassCom *__restrict__ mc,
+                                                  const TopRec *__restrict__ ttop, const MassCom *__restrict__ tk) {
     __shared__ int32_t s_list[kTopLds];
     __shared__ int32_t s_kids[kTopLds][8];
     __shared__ uint32_t s_cb[kTopLds];
     __shared__ uint8_t s_nc[kTopLds];
     __shared__ uint32_t s_cur[kDepths], s_off[kDepths + 1];
+    __shared__ MassCom s_mc[kTopOwn];       // results of the nodes listed, by list position
+    __shared__ int32_t s_hk[kTopHash];      // node -> list position (open addressing)
+    __shared__ uint16_t s_hv[kTopHash];
+    __shared__ uint16_t s_orig[kTopOwn];    // index in `top` of each list position
+    __shared__ uint32_t s_slot[kTopOwn];    // slot in the grouped layout of each list position
     const uint32_t cnt = *ntop;
     const int t = threadIdx.x;
     for (int i = t; i < kDepths; i += blockDim.x) s_cur[i] = 0;
+    for (int i = t; i < kTopHash; i += blockDim.x) s_hk[i] = -1;
     __syncthreads();
-    for (uint32_t i = t; i < cnt; i += blockDim.x) atomicAdd(&s_cur[min((int)depth[top[i]], kDepths - 1)], 1u);
+    for (uint32_t i = t; i < cnt; i += blockDim.x)
+        atomicAdd(&s_cur[min(cnt <= (uint32_t)kTopOwn ? (int)ttop[i].depth : (int)depth[top[i]], kDepths - 1)], 1u);
     __syncthreads();
     depth_offsets(s_cur, s_off);
     __syncthreads();
-    if (cnt <= kTopLds) {
+    if (cnt <= (uint32_t)kTopOwn && blockDim.x >= (unsigned)kTopOwn) {
+        // One node per thread (list position t), every input loaded before
+        // the first depth: the node's TopRec and its children's records
+        // (adjacent, from tk), except children that are themselves above
+        // the subtrees, whose sums come from LDS (found through a node ->
+        // list position table), as do their slots (1 + the parent's cbase +
+        // rank).
+        if ((uint32_t)t < cnt) {
+            const TopRec tr = ttop[t];
+            const int32_t p = top[t];
+            const uint32_t at = atomicAdd(&s_cur[min((int)tr.depth, kDepths - 1)], 1u);
+            s_list[at] = p;
+            s_cb[at] = cbase[p];
+            s_nc[at] = tr.nc;
+            s_orig[at] = (uint16_t)t;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s_kids[at][k] = tr.kids[k];
+            uint32_t h = top_hash(p);
+            while (atomicCAS(&s_hk[h], -1, p) != -1) h = (h + 1) & (kTopHash - 1);  // keys are distinct
+            s_hv[h] = (uint16_t)at;
+        }
+        __syncthreads();
+        const uint32_t j = (uint32_t)t;
+        const bool mine = j < cnt;
+        const int32_t p = mine ? s_list[j] : 0;
+        const uint32_t orig = mine ? s_orig[j] : 0u;
+        int32_t pos[8];
+        bool ok[8];
+        MassCom v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int32_t c = mine ? s_kids[j][k] : -1;
+            ok[k] = c > p && c < total;
+            pos[k] = -1;
+            if (ok[k]) {
+                uint32_t h = top_hash(c);
+                for (int32_t key; (key = s_hk[h]) != -1; h = (h + 1) & (kTopHash - 1))
+                    if (key == c) {
+                        pos[k] = s_hv[h];
+                        break;
+                    }
+            }
+            if (pos[k] >= 0) s_slot[pos[k]] = 1u + s_cb[j] + (uint32_t)k;
+        }
+        if (mine && p == 0) s_slot[j] = 0u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (ok[k] && pos[k] < 0) v[k] = tk[8 * orig + k];
+        double sd = 0.0;
+        int32_t sk = 0, fi = 0, la = 0;
+        if (mine && W.on()) {
+            const TopRec tr = ttop[orig];
+            sd = tr.side;
+            sk = tr.skip;
+            fi = tr.first;
+            la = tr.last;
+        }
+        __syncthreads();  // s_slot
+        const uint32_t slot = mine ? s_slot[j] : 0u;
+        for (int L = kDepths - 1; L >= 0; --L) {
+            const uint32_t b = s_off[L], e = s_off[L + 1];
+            if (b == e) continue;  // the same for every thread (LDS values after a barrier)
+            if (j >= b && j < e) {
+                double ms = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+                for (int k = 7; k >= 0; --k) {  // reverse octant order (calc_com :262-266)
+                    if (!ok[k]) continue;
+                    const MassCom u = pos[k] >= 0 ? s_mc[pos[k]] : v[k];
+                    ms += u.m;
+                    sx += u.m * u.x;
+                    sy += u.m * u.y;
+                    sz += u.m * u.z;
+                }
+                MassCom o;
+                o.m = ms;
+                o.x = sx / ms;
+                o.y = sy / ms;
+                o.z = sz / ms;
+                s_mc[j] = o;
+                mass[p] = o.m;
+                com[p] = o.x;
+                com[cstride + p] = o.y;
+                com[2 * cstride + p] = o.z;
+                mc[p] = o;
+                if (W.on()) put_walk_recs(W, p, o.m, o.x, o.y, o.z, false, sd, sk, fi, la, s_nc[j], s_cb[j], slot);
+            }
+            __syncthreads();  // this depth's sums (LDS) before the next depth up
+        }
+    } else if (cnt <= kTopLds) {
         for (uint32_t i = t; i < cnt; i += blockDim.x) {
             const int32_t p = top[i];
             const uint32_t at = atomicAdd(&s_cur[min((int)depth[p], kDepths - 1)], 1u);
@@ -1718,7 +1882,7 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
         w.depth.ensure(t); w.nchild.ensure(t); w.octant.ensure(t); w.mass.ensure(t); w.com.ensure(3 * t);
         w.side.ensure(t); w.wnode.ensure(t); w.slot.ensure(t); w.kid.ensure(t); w.roots.ensure(t); w.top.ensure(t);
         w.wgrp.ensure(t); w.rank.ensure(t); w.cbase.ensure(t); w.cbraw.ensure(t); w.cblk.ensure(t / 1024 + 2); w.ctot.ensure(1);
-        w.mc.ensure(t);
+        w.mc.ensure(t); w.rdest.ensure(t); w.ttop.ensure(kTopOwn); w.tk.ensure(8 * kTopOwn);
         w.cap_nodes = want;
     };
     grow_nodes(2 * n + 64);
@@ -1771,7 +1935,8 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
     // children, their slots in the grouped walk layout, subtree lists
     hipLaunchKernelGGL(k_children, dim3(blocks(total)), dim3(kBlock), 0, st, total, w.skip.p, w.body.p,
                        w.nchild.p, w.rank.p, w.cbraw.p, w.parent.p, w.roots.p, nroots_d, w.top.p, nroots_d + 1,
-                       w.mass.p, w.com.p, cs, w.mc.p);
+                       w.mass.p, w.com.p, cs, w.mc.p, w.side.p, w.first.p, w.last.p, w.depth.p, w.ttop.p, w.tk.p,
+                       w.rdest.p);
     const int nbn = (int)((total + 1023) / 1024);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nbn), dim3(1024), 0, st, (int64_t)total, w.cbraw.p, w.cblk.p);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, nbn, w.cblk.p, w.ctot.p);
@@ -1787,10 +1952,10 @@ static void bh_build(nb_system *s, BHDrift dr = BHDrift()) {
     const unsigned sub_grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(2048, total / 128 + 1));
     hipLaunchKernelGGL(k_com_subtrees, dim3(sub_grid), dim3(kBlock), 0, st, total, w.roots.p, nroots_d, w.skip.p,
                        w.body.p, w.depth.p, w.nchild.p, w.cbase.p, w.kid.p, w.mass.p, w.com.p, cs, W, w.side.p,
-                       w.first.p, w.last.p, w.mc.p);
+                       w.first.p, w.last.p, w.mc.p, w.rdest.p, w.tk.p);
     hipLaunchKernelGGL(k_com_top, dim3(1), dim3(1024), 0, st, total, w.top.p, nroots_d + 1, w.skip.p, w.body.p,
                        w.depth.p, w.nchild.p, w.cbase.p, w.kid.p, w.mass.p, w.com.p, cs, W, w.side.p, w.first.p,
-                       w.last.p, w.mc.p);
+                       w.last.p, w.mc.p, w.ttop.p, w.tk.p);
     NB_HIP(hipGetLastError());
 }
 
