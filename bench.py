@@ -112,7 +112,7 @@ PEAK_FP64_TFLOPS = 78.6
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 summed over the build kernels
 # of one BH step at 1,048,576 bodies (profiles/bh_walk_build_two_plummer_1M.md),
 # and the same for the frontier walk launch; None when not profiled.
-PROFILED_BH_BUILD_BYTES = 1.128e9
+PROFILED_BH_BUILD_BYTES = 1.101e9
 PROFILED_BH_WALK_BYTES = 1.935e8
 # the same for one 1000-step launch of the mixed batch kernel on the 4096-system
 # comet ensemble (profiles/batch_comet_ensemble4096.md)
