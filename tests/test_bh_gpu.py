@@ -134,17 +134,26 @@ def test_topology_after_adaptive_sort_steps():
     s.close()
 
 
-def test_full_size_topology():
-    """Two 524,288-body Plummer spheres (the 1M workload): node count and leaf
-    membership bit-exact against the oracle's insertion-built tree."""
-    m, pos, _ = capi.ic_two_plummer(2, 1 << 20)
-    print("case full-size topology n=1048576", flush=True)
+@pytest.mark.parametrize("n", [1 << 20, 1 << 21, 3 << 20])
+def test_full_size_topology(n):
+    """Two Plummer spheres of n/2 bodies (n = 1M is the benchmark workload):
+    node count, leaf membership, masses and centres of mass bit-exact against
+    the oracle's insertion-built tree.  The nodes above the 1024-node
+    subtrees number about 780 at 1M, 1,580 at 2M and 2,300 at 3M, so the
+    three sizes run the three forms of that pass (one node per thread, a
+    list of up to 2048 in LDS, and the rescan).  Every body gets the power
+    of two at or below 1/n (1/n itself at 1M and 2M), so every mass sum is
+    exact in any order and the reference's insertion-order sums agree."""
+    _, pos, _ = capi.ic_two_plummer(2, n)
+    m = np.full(n, 2.0 ** -np.ceil(np.log2(n)))
+    print(f"case full-size topology n={n}", flush=True)
     t = system(m, pos).tree()
     o = O.bh_tree(m, pos)
     assert len(t["depth"]) == len(o["depth"])
     for k in ("depth", "octant", "body", "parent"):
         assert np.array_equal(t[k], o[k]), k
     assert np.array_equal(t["mass"], o["mass"])
+    assert np.array_equal(t["com"], o["com"])
 
 
 def test_exact_walk_bit_identical():
